@@ -1,0 +1,49 @@
+/*
+ * phc_setup.h -- host-side setup entry points of libphc.so (no device work).
+ * They compute what the reference's FDTDInit/FDTDUtils/SolverUtils compute
+ * before the time loop, with the reference's own operand types.
+ */
+#ifndef PHC_SETUP_H
+#define PHC_SETUP_H
+
+#include <stdint.h>
+
+#include "phc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* exp variant of evaluateRicker: the FD translation unit binds ::exp(double),
+ * the SEM one std::exp(float) (src/utils/utils.hpp:31-44 under each unit's
+ * using-directives). */
+#define PHC_EXP_DOUBLE 0
+#define PHC_EXP_FLOAT 1
+
+/* FDTDUtils::init_coef (src/fd/utilsFDTD/FDTDutils.hpp:13-58): coef[0..L]. */
+phc_status phc_fd_init_coef(int L, float dx, float *coef);
+/* FDTDInit::init_coefficients coef0 (src/fd/utilsFDTD/FDTDinit.hpp:212-227). */
+double phc_fd_coef0(const float *cx, int Lx, const float *cy, int Ly, const float *cz, int Lz);
+/* FDTDUtils::compute_dt_sch (src/fd/utilsFDTD/FDTDutils.hpp:59-76); the
+ * reference fixes cfl = 0.8. */
+float phc_fd_compute_dt(float vmax, const float *cx, int Lx, const float *cy, int Ly,
+                        const float *cz, int Lz, float cfl);
+/* SolverUtils::evaluateRicker (src/utils/utils.hpp:11-46). */
+float phc_ricker(float time_n, float f0, int order, int exp_variant);
+/* SolverUtils::computeSourceTerm (src/utils/utils.hpp:48-57). */
+phc_status phc_source_term(int nsamples, float dt, float f0, int order, int exp_variant,
+                           float *out);
+/* FDTDInit::init_models (src/fd/utilsFDTD/FDTDinit.hpp:277-301): vp^2 dt^2,
+ * doubled for k >= nz/2.  IDX3 order. */
+phc_status phc_fd_model_two_layer(int nx, int ny, int nz, float vmin, float dt, float *vp);
+/* Seeded layered model (no reference counterpart; the FD benchmark model):
+ * nlayers layers along k with nlayers-1 distinct sorted interface depths drawn
+ * with splitmix64(seed) from 1..nz-1, velocities evenly spaced vtop..vbottom.
+ * Writes vp^2 dt^2 in IDX3 order; interfaces (may be NULL) gets the depths. */
+phc_status phc_fd_model_layered(int nx, int ny, int nz, int nlayers, float vtop, float vbottom,
+                                uint64_t seed, float dt, int *interfaces, float *vp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,222 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product.  The C sources beside this
+file restate the reference's hot path; see oracle.h for the file:line map.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class FdGrid(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "nx", "ny", "nz", "lx", "ly", "lz", "xs", "ys", "zs",
+        "x3", "x4", "y3", "y4", "z3", "z4")]
+
+
+class SemMesh(ctypes.Structure):
+    _fields_ = [("ex", ctypes.c_int), ("ey", ctypes.c_int), ("ez", ctypes.c_int),
+                ("lx", ctypes.c_float), ("ly", ctypes.c_float), ("lz", ctypes.c_float),
+                ("order", ctypes.c_int),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("hx", ctypes.c_float), ("hy", ctypes.c_float), ("hz", ctypes.c_float)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(_LIB_PATH)
+        c_int, c_float, c_double = ctypes.c_int, ctypes.c_float, ctypes.c_double
+        P = ctypes.POINTER
+        L.orc_ricker.restype = c_float
+        L.orc_ricker.argtypes = [c_float, c_float, c_int, c_int]
+        L.orc_source_term.restype = None
+        L.orc_source_term.argtypes = [c_int, c_float, c_float, c_int, c_int, f32p]
+        L.orc_ricker_f64.restype = c_double
+        L.orc_ricker_f64.argtypes = [c_double, c_double, c_int]
+        L.orc_fd_init_coef.restype = None
+        L.orc_fd_init_coef.argtypes = [c_int, c_float, f32p]
+        L.orc_fd_coef0.restype = c_double
+        L.orc_fd_coef0.argtypes = [f32p, c_int, f32p, c_int, f32p, c_int]
+        L.orc_fd_compute_dt.restype = c_float
+        L.orc_fd_compute_dt.argtypes = [c_float, f32p, c_int, f32p, c_int, f32p, c_int, c_float]
+        L.orc_fd_sponge.restype = None
+        L.orc_fd_sponge.argtypes = [c_int, c_int, c_int, c_int, c_float, f32p]
+        L.orc_fd_sponge_table.restype = None
+        L.orc_fd_sponge_table.argtypes = [c_int, c_float, f32p]
+        L.orc_fd_add_rhs.restype = None
+        L.orc_fd_add_rhs.argtypes = [P(FdGrid), f32p, f32p, c_float]
+        L.orc_fd_inner3d.restype = None
+        L.orc_fd_inner3d.argtypes = [P(FdGrid), f32p, f32p, f32p, c_double, f32p, f32p, f32p]
+        L.orc_fd_apply_sponge.restype = None
+        L.orc_fd_apply_sponge.argtypes = [P(FdGrid), f32p, f32p, f32p]
+        L.orc_fd_run.restype = None
+        L.orc_fd_run.argtypes = [P(FdGrid), f32p, f32p, f32p, c_double, f32p, f32p, f32p,
+                                 c_int, c_int, f32p, f32p, P(c_int), ctypes.c_void_p,
+                                 c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_fd_run_f64.restype = None
+        L.orc_fd_run_f64.argtypes = [P(FdGrid), f64p, f64p, f64p, c_double, f64p, f64p, f64p,
+                                     c_int, c_int, f64p, f64p, P(c_int), ctypes.c_void_p,
+                                     c_int, ctypes.c_void_p, ctypes.c_void_p]
+        _bind_sem(L)
+        _lib = L
+    return _lib
+
+
+def _bind_sem(L):
+    c_int, c_float, c_double = ctypes.c_int, ctypes.c_float, ctypes.c_double
+    P = ctypes.POINTER
+    if not hasattr(L, "orc_gll_points"):
+        return
+    L.orc_gll_points.restype = None
+    L.orc_gll_points.argtypes = [c_int, f64p]
+    L.orc_gll_weights.restype = None
+    L.orc_gll_weights.argtypes = [c_int, f64p, c_int]
+    L.orc_gll_deriv.restype = None
+    L.orc_gll_deriv.argtypes = [c_int, f64p, f64p]
+    L.orc_sem_mesh_init.restype = None
+    L.orc_sem_mesh_init.argtypes = [P(SemMesh), c_int, c_int, c_int, c_float, c_float, c_float, c_int]
+    L.orc_sem_coord_1d.restype = None
+    L.orc_sem_coord_1d.argtypes = [c_int, c_int, c_int, f32p]
+    L.orc_sem_nodes_list.restype = None
+    L.orc_sem_nodes_list.argtypes = [P(SemMesh), i32p]
+    L.orc_sem_element_from_point.restype = c_int
+    L.orc_sem_element_from_point.argtypes = [P(SemMesh), c_float, c_float, c_float]
+    L.orc_sem_interior_nodes.restype = c_int
+    L.orc_sem_interior_nodes.argtypes = [P(SemMesh), i32p]
+    L.orc_sem_create.restype = ctypes.c_void_p
+    L.orc_sem_create.argtypes = [P(SemMesh), f32p, c_float]
+    L.orc_sem_destroy.restype = None
+    L.orc_sem_destroy.argtypes = [ctypes.c_void_p]
+    L.orc_sem_num_nodes.restype = c_int
+    L.orc_sem_num_nodes.argtypes = [ctypes.c_void_p]
+    L.orc_sem_step.restype = None
+    L.orc_sem_step.argtypes = [ctypes.c_void_p, f32p, f32p, c_int, c_float]
+    L.orc_sem_get_my.restype = None
+    L.orc_sem_get_my.argtypes = [ctypes.c_void_p, f32p, f32p]
+    L.orc_sem_apply_k.restype = None
+    L.orc_sem_apply_k.argtypes = [ctypes.c_void_p, f32p, f32p, f32p]
+    L.orc_sem64_create.restype = ctypes.c_void_p
+    L.orc_sem64_create.argtypes = [P(SemMesh), f64p, c_double]
+    L.orc_sem64_destroy.restype = None
+    L.orc_sem64_destroy.argtypes = [ctypes.c_void_p]
+    L.orc_sem64_step.restype = None
+    L.orc_sem64_step.argtypes = [ctypes.c_void_p, f64p, f64p, c_int, c_double]
+    L.orc_sem64_apply_k.restype = None
+    L.orc_sem64_apply_k.argtypes = [ctypes.c_void_p, f64p, f64p, f64p]
+    L.orc_sem_dense_element_f64.restype = None
+    L.orc_sem_dense_element_f64.argtypes = [c_int, f64p, f64p, f64p, f64p, f64p]
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ------------------------------------------------------------------------ FD
+def fd_coefs(L, dx):
+    c = np.zeros(L + 1, np.float32)
+    lib().orc_fd_init_coef(L, dx, c)
+    return c
+
+
+class FdProblem:
+    """Everything the reference's FDTDInit builds (FDTDinit.hpp:36-396),
+    restated.  `model` is "layered2" (the reference's init_models: vmin^2 dt^2
+    above nz/2, twice that below), "homogeneous" (vmin everywhere, the model of
+    the published test_20240512 logs) or an explicit float32 array of vp^2 dt^2
+    in IDX3 order."""
+
+    def __init__(self, nx=150, ny=None, nz=None, L=4, dx=10.0, xs=None, ys=None, zs=None,
+                 f0=15.0, time_max=1.0, vmin=1500.0, vmax=4500.0, cfl=0.8,
+                 source_order=1, sponge_width=20, sponge_alpha=-0.00015,
+                 model="layered2", dt=None, nsamples=None):
+        ny = nx if ny is None else ny
+        nz = nx if nz is None else nz
+        self.nx, self.ny, self.nz, self.L = nx, ny, nz, L
+        xs = nx // 2 if xs is None else xs
+        ys = ny // 2 if ys is None else ys
+        zs = nz // 2 if zs is None else zs
+        g = FdGrid()
+        g.nx, g.ny, g.nz = nx, ny, nz
+        g.lx = g.ly = g.lz = L
+        g.xs, g.ys, g.zs = xs, ys, zs
+        g.x3, g.x4, g.y3, g.y4, g.z3, g.z4 = 0, nx, 0, ny, 0, nz
+        self.g = g
+        self.cx = fd_coefs(L, dx)
+        self.cy = self.cx.copy()
+        self.cz = self.cx.copy()
+        self.coef0 = lib().orc_fd_coef0(self.cx, L, self.cy, L, self.cz, L)
+        if dt is None:
+            dt = lib().orc_fd_compute_dt(vmax, self.cx, L, self.cy, L, self.cz, L, cfl)
+        self.dt = np.float32(dt)
+        if nsamples is None:
+            nsamples = int(np.float32(time_max) / self.dt)
+        self.nsamples = nsamples
+        self.f0 = f0
+        self.rhs = np.zeros(nsamples, np.float32)
+        lib().orc_source_term(nsamples, self.dt, f0, source_order, 1, self.rhs)
+        v0 = np.float32(np.float32(np.float32(vmin) * np.float32(vmin)) * self.dt) * self.dt
+        if isinstance(model, np.ndarray):
+            self.vp = np.ascontiguousarray(model, dtype=np.float32).reshape(-1)
+        elif model == "homogeneous":
+            self.vp = np.full(nx * ny * nz, v0, np.float32)
+        elif model == "layered2":
+            vp = np.full((nx, ny, nz), v0, np.float32)
+            vp[:, :, nz // 2:] = np.float32(2) * v0
+            self.vp = vp.reshape(-1)
+        else:
+            raise ValueError(model)
+        self.sponge = np.zeros(nx * ny * nz, np.float32)
+        lib().orc_fd_sponge(nx, ny, nz, sponge_width, sponge_alpha, self.sponge)
+        self.sponge_width = sponge_width
+        self.sponge_alpha = sponge_alpha
+        ext = (nx + 2 * L) * (ny + 2 * L) * (nz + 2 * L)
+        self.p0 = np.full(ext, np.float32(0.000001), np.float32)
+        self.p1 = np.full(ext, np.float32(0.000001), np.float32)
+        self.cur_is_p1 = ctypes.c_int(1)
+        self.it = 0
+
+    def ext_shape(self):
+        L = self.L
+        return (self.nx + 2 * L, self.ny + 2 * L, self.nz + 2 * L)
+
+    def run(self, nsteps, receivers=None):
+        """Run nsteps reference steps; returns (source trace, receiver traces)."""
+        tr = np.zeros(nsteps, np.float32)
+        nrec = 0 if receivers is None else len(receivers)
+        rec = None if not nrec else np.ascontiguousarray(receivers, np.int32).reshape(-1)
+        rv = None if not nrec else np.zeros(nrec * nsteps, np.float32)
+        lib().orc_fd_run(ctypes.byref(self.g), self.cx, self.cy, self.cz, self.coef0, self.vp,
+                         self.sponge, self.rhs, self.it, nsteps, self.p0, self.p1,
+                         ctypes.byref(self.cur_is_p1), _vp(tr), nrec, _vp(rec), _vp(rv))
+        self.it += nsteps
+        return tr, (None if rv is None else rv.reshape(nrec, nsteps))
+
+    def current(self):
+        return (self.p1 if self.cur_is_p1.value else self.p0).reshape(self.ext_shape())
+
+    def previous(self):
+        return (self.p0 if self.cur_is_p1.value else self.p1).reshape(self.ext_shape())
+
+
+def fd_exact_coefs(L, dx):
+    """Exact central second-difference weights (double) / dx^2, L <= 5."""
+    from fractions import Fraction as F
+    tab = {1: [F(-2), F(1)],
+           2: [F(-5, 2), F(4, 3), F(-1, 12)],
+           3: [F(-49, 18), F(3, 2), F(-3, 20), F(1, 90)],
+           4: [F(-205, 72), F(8, 5), F(-1, 5), F(8, 315), F(-1, 560)],
+           5: [F(-5269, 1800), F(5, 3), F(-5, 21), F(5, 126), F(-5, 1008), F(1, 3150)]}
+    return np.array([float(c) for c in tab[L]], np.float64) / (dx * dx)

@@ -1,0 +1,94 @@
+"""Build recipes: libphc.so (HIP, gfx950), the C++ drop-in drivers, and the
+CPU oracle used by the tests.
+
+Everything is built in-tree so that the shared objects travel with the source
+tree.  Object files go to _build/ (git-ignored).
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+HOST = os.path.join(PKG, "host")
+INCLUDE = os.path.join(ROOT, "include")
+ORACLE = os.path.join(ROOT, "oracle")
+BUILD = os.path.join(ROOT, "_build")
+
+LIBPHC = os.path.join(PKG, "libphc.so")
+LIBORACLE = os.path.join(ORACLE, "liboracle.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 in generated code.
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-I" + INCLUDE, "-I" + CSRC]
+
+HIP_SOURCES = ["phc_common.hip", "setup.hip", "fd.hip", "sem.hip", "batch.hip", "mlp.hip"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("command failed (%d): %s\n%s" % (r.returncode, " ".join(cmd), out))
+    return out
+
+
+def build_libphc(jobs=8):
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = [s for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objs = [os.path.join(BUILD, s.replace(".hip", ".o")) for s in srcs]
+
+    def compile_one(pair):
+        src, obj = pair
+        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(compile_one, zip(srcs, objs)))
+    tmp = LIBPHC + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIBPHC)
+    return LIBPHC
+
+
+def build_oracle():
+    srcs = [os.path.join(ORACLE, s) for s in ("fd_oracle.c", "sem_oracle.c")
+            if os.path.exists(os.path.join(ORACLE, s))]
+    tmp = LIBORACLE + ".tmp"
+    # -ffp-contract=off: no FMA contraction, so float/double operations round
+    # exactly where the reference's expressions round on x86-64.
+    _run(["gcc", "-O2", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
+          "-o", tmp] + srcs + ["-lm"])
+    os.replace(tmp, LIBORACLE)
+    return LIBORACLE
+
+
+def build_host():
+    """C++ drop-in drivers (fd_hip, sem_hip) linked against libphc.so."""
+    outs = []
+    for name, src in (("fd_hip", "fd_main.cpp"), ("sem_hip", "sem_main.cpp")):
+        path = os.path.join(HOST, src)
+        if not os.path.exists(path):
+            continue
+        exe = os.path.join(PKG, name)
+        _run(["g++", "-O2", "-std=c++17", "-I" + INCLUDE, "-I" + HOST, path, "-o", exe,
+              "-L" + PKG, "-lphc", "-Wl,-rpath,$ORIGIN"])
+        outs.append(exe)
+    return outs
+
+
+def build_all(jobs=8):
+    with cf.ThreadPoolExecutor(max_workers=2) as ex:
+        f1 = ex.submit(build_libphc, jobs)
+        f2 = ex.submit(build_oracle)
+        f1.result()
+        f2.result()
+    build_host()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    {"all": build_all, "phc": build_libphc, "oracle": build_oracle, "host": build_host}[what]()

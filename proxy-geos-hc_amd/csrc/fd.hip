@@ -1,0 +1,783 @@
+// fd.hip -- FD step kernels (radius-R star stencil + leapfrog + sponge +
+// source injection) and the phc_fd_* C ABI.
+//
+// Reference hot path (src/fd/utilsFDTD/):
+//   FDTDKernel::addRHS           FDTDkernels.hpp:121-128
+//   FDTDKernel::inner3D          FDTDkernels.hpp:8-34
+//   FDTDKernel::applySponge      FDTDkernels.hpp:108-118
+//   FDTDKernel::computeOneStepSB FDTDkernels.hpp:75-84
+//   time loop                    FDTDmain.cpp:55-81, output FDTDutils.hpp:193-232
+//
+// One launch does inner3D + applySponge for every interior point.  A block
+// owns a 64(k) x TJ(j) column of the grid and marches along i over chunk_i
+// planes.  Each thread keeps the 2R+1 values of pn along i for its NJ points
+// in registers; the current i-plane of the tile plus its j/k halo is staged in
+// LDS for the in-plane neighbours.  HBM traffic per point: read pn, pnm1, vp,
+// write the new level (16 B); halo re-reads are served from L2/MALL.
+//
+// The reference damps the current level in place after the stencil
+// (applySponge's second product).  Here that product is applied when the
+// level is next read as "previous" (DAMP_PREV), which gives the same float
+// values without a second pass over the grid.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fd_kernels.hpp"
+#include "phc_common.hpp"
+
+namespace phc {
+
+// ----------------------------------------------------------------- kernels
+
+// New value of one point, reference rounding: float lap accumulators, the
+// update evaluated in double exactly as FDTDkernels.hpp:30-31 writes it, then
+// the sponge product in float (FDTDkernels.hpp:114).
+__device__ __forceinline__ float fd_update_ref(float pc, float pm, float v, double coef0,
+                                               float lapx, float lapy, float lapz, float s)
+{
+#pragma clang fp contract(off)
+    const double dpc = (double)pc;
+    const double a = 2.0 * dpc - (double)pm;
+    const double b = (((coef0 * dpc + (double)lapx) + (double)lapy) + (double)lapz);
+    const float r = (float)(a + (double)v * b);
+    return r * s;
+}
+
+__device__ __forceinline__ float mul_nc(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+
+__device__ __forceinline__ float add_nc(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a + b;
+}
+
+template <int R, int NJ, int ARITH, bool DAMP_PREV>
+__global__ void __launch_bounds__(FD_TK *FD_TY)
+fd_step_kernel(FdStepArgs a)
+{
+    constexpr int TJ = FD_TY * NJ;
+    constexpr int LW = FD_TK + 2 * R;
+    constexpr int LH = TJ + 2 * R;
+    __shared__ float tile[LH][LW];
+
+    const FdLayout &L = a.L;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    int b = blockIdx.x;
+    const int nb = gridDim.x;
+    // XCD-aware remap: hardware deals consecutive block ids round-robin over
+    // the 8 XCDs; give each XCD a contiguous range of tiles so halo re-reads
+    // hit its own L2.
+    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    const int kt = b % a.tiles_k;
+    const int rest = b / a.tiles_k;
+    const int jt = rest % a.tiles_j;
+    const int ic = rest / a.tiles_j;
+    const int k0 = kt * FD_TK, j0 = jt * TJ;
+    const int i0 = ic * a.chunk_i;
+    const int i1 = min(i0 + a.chunk_i, L.nx);
+    const int k = k0 + tx;
+
+    const bool kload = k < L.nz + R;  // inside the stored halo
+    const bool kcomp = k < L.nz;
+    int jr[NJ];
+    bool jload[NJ], jcomp[NJ];
+    float fyv[NJ];
+    bool inJ[NJ];
+#pragma unroll
+    for (int jj = 0; jj < NJ; ++jj) {
+        jr[jj] = j0 + ty * NJ + jj;
+        jload[jj] = kload && (jr[jj] < L.ny + R);
+        jcomp[jj] = kcomp && (jr[jj] < L.ny);
+        inJ[jj] = (jr[jj] >= a.sw) && (jr[jj] < L.ny - a.sw);
+        fyv[jj] = (a.sw > 0 && jr[jj] < L.ny) ? a.fy[jr[jj]] : 1.f;
+    }
+    const bool inK = (k >= a.sw) && (k < L.nz - a.sw);
+    const float fzv = (a.sw > 0 && kcomp) ? a.fz[k] : 1.f;
+
+    float q[NJ][2 * R + 1];
+#pragma unroll
+    for (int jj = 0; jj < NJ; ++jj)
+#pragma unroll
+        for (int l = 0; l < 2 * R; ++l)
+            q[jj][l] = jload[jj] ? a.pn[fd_off(L, i0 - R + l, jr[jj], k)] : 0.f;
+
+    for (int i = i0; i < i1; ++i) {
+        float pm[NJ], vv[NJ];
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj) {
+            q[jj][2 * R] = jload[jj] ? a.pn[fd_off(L, i + R, jr[jj], k)] : 0.f;
+            const int64_t o = fd_off(L, i, jr[jj], k);
+            pm[jj] = jcomp[jj] ? a.pnm1[o] : 0.f;
+            vv[jj] = jcomp[jj] ? a.vp[o] : 0.f;
+        }
+        // stage plane i: centre values, k-halo, j-halo
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj) tile[ty * NJ + jj + R][tx + R] = q[jj][R];
+        if (tx < 2 * R) {
+            const int kk = (tx < R) ? (k0 - R + tx) : (k0 + FD_TK + tx - R);
+            const int col = kk - k0 + R;
+            if (kk < L.nz + R) {
+#pragma unroll
+                for (int jj = 0; jj < NJ; ++jj) {
+                    const int j = jr[jj];
+                    if (j < L.ny + R) tile[ty * NJ + jj + R][col] = a.pn[fd_off(L, i, j, kk)];
+                }
+            }
+        }
+        for (int h = ty; h < 2 * R; h += FD_TY) {
+            const int jh = (h < R) ? (j0 - R + h) : (j0 + TJ + h - R);
+            if (jh < L.ny + R && kload) tile[jh - j0 + R][tx + R] = a.pn[fd_off(L, i, jh, k)];
+        }
+        __syncthreads();
+
+        const bool inI = (i >= a.sw) && (i < L.nx - a.sw);
+        const float fxv = (a.sw > 0) ? a.fx[i] : 1.f;
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj) {
+            if (!jcomp[jj]) continue;
+            const int r = ty * NJ + jj + R;
+            const int c = tx + R;
+            float s = 1.f;
+            if (a.sw > 0) {
+                if (inJ[jj] && inK && !inI) s = fxv;
+                if (inK && inI && !inJ[jj]) s = fyv[jj];
+                if (inJ[jj] && inI && !inK) s = fzv;
+            }
+            float val;
+            if constexpr (ARITH == PHC_FD_ARITH_REF) {
+                float lapx, lapy, lapz;
+                {
+#pragma clang fp contract(off)
+                    lapx = 0.f;
+#pragma unroll
+                    for (int l = 1; l <= R; ++l) lapx += a.cx[l] * (q[jj][R + l] + q[jj][R - l]);
+                    lapy = 0.f;
+#pragma unroll
+                    for (int l = 1; l <= R; ++l) lapy += a.cy[l] * (tile[r + l][c] + tile[r - l][c]);
+                    lapz = 0.f;
+#pragma unroll
+                    for (int l = 1; l <= R; ++l) lapz += a.cz[l] * (tile[r][c + l] + tile[r][c - l]);
+                }
+                const float pmv = DAMP_PREV ? mul_nc(pm[jj], s) : pm[jj];
+                val = fd_update_ref(q[jj][R], pmv, vv[jj], a.coef0, lapx, lapy, lapz, s);
+            } else {
+                float lap = a.c0f * q[jj][R];
+#pragma unroll
+                for (int l = 1; l <= R; ++l) {
+                    lap = fmaf(a.cx[l], q[jj][R + l] + q[jj][R - l], lap);
+                    lap = fmaf(a.cy[l], tile[r + l][c] + tile[r - l][c], lap);
+                    lap = fmaf(a.cz[l], tile[r][c + l] + tile[r][c - l], lap);
+                }
+                const float pmv = DAMP_PREV ? pm[jj] * s : pm[jj];
+                val = s * fmaf(vv[jj], lap, 2.f * q[jj][R] - pmv);
+            }
+            const int64_t o = fd_off(L, i, jr[jj], k);
+            if (o == a.src_off) {
+                if (a.trace) *a.trace = val;
+                if (a.inj) val = add_nc(val, mul_nc(vv[jj], a.rhs_next));
+            }
+            a.pnm1[o] = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj)
+#pragma unroll
+            for (int l = 0; l < 2 * R; ++l) q[jj][l] = q[jj][l + 1];
+    }
+}
+
+// addRHS (FDTDkernels.hpp:121-128): one point.
+__global__ void fd_add_rhs_kernel(float *p, const float *vp, int64_t off, const float *rhs, int it)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[off] = add_nc(p[off], mul_nc(vp[off], rhs[it]));
+}
+
+// Deferred sponge product of a level: p *= s on every interior point.
+__global__ void fd_damp_kernel(float *p, FdLayout L, int sw, const float *fx, const float *fy,
+                               const float *fz)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= L.nz) return;
+    const bool inI = (i >= sw) && (i < L.nx - sw);
+    const bool inJ = (j >= sw) && (j < L.ny - sw);
+    const bool inK = (k >= sw) && (k < L.nz - sw);
+    float s = 1.f;
+    if (inJ && inK && !inI) s = fx[i];
+    if (inK && inI && !inJ) s = fy[j];
+    if (inJ && inI && !inK) s = fz[k];
+    const int64_t o = fd_off(L, i, j, k);
+    p[o] = mul_nc(p[o], s);
+}
+
+// Receiver sampling after a step: out[r*stride + step] = level[off[r]].  A
+// receiver on the source point reads the value recorded before the fused
+// injection of the next step.
+__global__ void fd_sample_kernel(const float *p, const int64_t *offs, int nrec, float *out,
+                                 int stride, int step, int64_t src_off, const float *src_trace)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrec) return;
+    const int64_t o = offs[r];
+    out[(int64_t)r * stride + step] = (o == src_off) ? src_trace[step] : p[o];
+}
+
+// ------------------------------------------------------------ dispatching
+
+using FdKernelFn = void (*)(FdStepArgs);
+
+template <int R, int NJ>
+static FdKernelFn pick_arith(int arith, bool damp)
+{
+    if (arith == PHC_FD_ARITH_REF)
+        return damp ? fd_step_kernel<R, NJ, PHC_FD_ARITH_REF, true>
+                    : fd_step_kernel<R, NJ, PHC_FD_ARITH_REF, false>;
+    return damp ? fd_step_kernel<R, NJ, PHC_FD_ARITH_FP32, true>
+                : fd_step_kernel<R, NJ, PHC_FD_ARITH_FP32, false>;
+}
+
+template <int R>
+static FdKernelFn pick_nj(int nj, int arith, bool damp)
+{
+    switch (nj) {
+    case 2: return pick_arith<R, 2>(arith, damp);
+    case 4: return pick_arith<R, 4>(arith, damp);
+    case 8: return pick_arith<R, 8>(arith, damp);
+    default: return nullptr;
+    }
+}
+
+FdKernelFn fd_pick_kernel(int R, int nj, int arith, bool damp)
+{
+    switch (R) {
+    case 1: return pick_nj<1>(nj, arith, damp);
+    case 2: return pick_nj<2>(nj, arith, damp);
+    case 3: return pick_nj<3>(nj, arith, damp);
+    case 4: return pick_nj<4>(nj, arith, damp);
+    case 5: return pick_nj<5>(nj, arith, damp);
+    case 6: return pick_nj<6>(nj, arith, damp);
+    default: return nullptr;
+    }
+}
+
+// Per-axis face damping values (FDTDinit.hpp:343-395): for the low face
+// exp(alpha (W-i)^2), for the high face exp(alpha (W-(n-i))^2); the high-face
+// loop runs second in the reference, so it wins where the two overlap.
+void fd_axis_damping(int n, int W, float alpha, std::vector<float> &f)
+{
+    f.assign(n, 1.f);
+    for (int i = 0; i < n; ++i) {
+        int d = -1;
+        if (i >= n - W) d = W - (n - i);
+        else if (i < W) d = W - i;
+        if (d >= 0) f[i] = (float)std::exp((double)alpha * std::pow((double)d, 2.0));
+    }
+}
+
+} // namespace phc
+
+// ====================================================================== API
+using namespace phc;
+
+struct phc_fd {
+    int nx, ny, nz, R;
+    int xs, ys, zs;
+    int arith;
+    int sw;
+    float alpha;
+    float cx[FD_MAX_R + 1], cy[FD_MAX_R + 1], cz[FD_MAX_R + 1];
+    double coef0;
+    int nsamples;
+    FdLayout L;
+    DevBuf lev[2], vp, fx, fy, fz, rhs, trace, rec, recoff;
+    std::vector<float> hfx, hfy, hfz, hvp, hrhs;
+    int pending[2] = {0, 0};
+    hipStream_t stream = nullptr;
+    int nj = 4, chunk_i = 0;
+    int nrec = 0;
+    std::vector<int64_t> rec_offs;
+    int last_n = 0, last_nrec = 0;
+    int timing = 0;
+    double last_ms = 0.0;
+    // cached graph of the last phc_fd_run
+    hipGraphExec_t gexec = nullptr;
+    int g_it0 = -1, g_n = -1, g_cur = -1, g_damp = -1, g_nrec = -1;
+};
+
+static phc_status fd_validate(const phc_fd_params *p)
+{
+    if (!p) return fail(PHC_ERR_INVALID, "params is NULL");
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1)
+        return fail(PHC_ERR_INVALID, "grid %d x %d x %d is empty", p->nx, p->ny, p->nz);
+    if (p->lx != p->ly || p->lx != p->lz)
+        return fail(PHC_ERR_INVALID, "unequal half widths lx=%d ly=%d lz=%d are not supported",
+                    p->lx, p->ly, p->lz);
+    if (p->lx < 1 || p->lx > FD_MAX_R)
+        return fail(PHC_ERR_INVALID, "half width %d outside 1..%d", p->lx, FD_MAX_R);
+    if (p->xs < 0 || p->xs >= p->nx || p->ys < 0 || p->ys >= p->ny || p->zs < 0 || p->zs >= p->nz)
+        return fail(PHC_ERR_INVALID, "source (%d,%d,%d) outside the %d x %d x %d grid", p->xs,
+                    p->ys, p->zs, p->nx, p->ny, p->nz);
+    if (!p->coefx || !p->coefy || !p->coefz || !p->vp)
+        return fail(PHC_ERR_INVALID, "coefficient or vp array is NULL");
+    if (p->nsamples < 0 || (p->nsamples > 0 && !p->rhs))
+        return fail(PHC_ERR_INVALID, "rhs is NULL with nsamples = %d", p->nsamples);
+    if (p->sponge_width < 0) return fail(PHC_ERR_INVALID, "negative sponge width");
+    if (p->arith != PHC_FD_ARITH_REF && p->arith != PHC_FD_ARITH_FP32)
+        return fail(PHC_ERR_INVALID, "unknown arithmetic mode %d", p->arith);
+    if (p->tile_j != 0 && p->tile_j != 8 && p->tile_j != 16 && p->tile_j != 32)
+        return fail(PHC_ERR_INVALID, "tile_j must be 8, 16 or 32 (got %d)", p->tile_j);
+    if (p->chunk_i < 0) return fail(PHC_ERR_INVALID, "negative chunk_i");
+    const double pts = (double)(p->nx + 2 * p->lx) * (p->ny + 2 * p->ly) * (p->nz + 2 * p->lz + 32);
+    if (pts > 1.5e10) return fail(PHC_ERR_INVALID, "grid too large (%.3g points)", pts);
+    for (int64_t n = 0; n < (int64_t)p->nx * p->ny * p->nz; ++n)
+        if (!(p->vp[n] >= 0.f) || !std::isfinite(p->vp[n]))
+            return fail(PHC_ERR_INVALID, "vp^2 dt^2 at %lld is negative or not finite",
+                        (long long)n);
+    return PHC_OK;
+}
+
+static phc_status fd_upload_interior(phc_fd *h, const float *host_idx3, float *dev)
+{
+    // host array in IDX3 order (k fastest, no halo) -> device layout
+    std::vector<float> buf((size_t)h->L.total, 0.f);
+    for (int i = 0; i < h->nx; ++i)
+        for (int j = 0; j < h->ny; ++j)
+            std::memcpy(&buf[(size_t)fd_off(h->L, i, j, 0)],
+                        host_idx3 + ((int64_t)i * h->ny + j) * h->nz, sizeof(float) * h->nz);
+    PHC_HIP(hipMemcpy(dev, buf.data(), sizeof(float) * h->L.total, hipMemcpyHostToDevice));
+    return PHC_OK;
+}
+
+static float fd_sponge_at(const phc_fd *h, int i, int j, int k)
+{
+    if (h->sw == 0) return 1.f;
+    const int W = h->sw;
+    const bool inI = i >= W && i < h->nx - W, inJ = j >= W && j < h->ny - W,
+               inK = k >= W && k < h->nz - W;
+    float s = 1.f;
+    if (inJ && inK && !inI) s = h->hfx[i];
+    if (inK && inI && !inJ) s = h->hfy[j];
+    if (inJ && inI && !inK) s = h->hfz[k];
+    return s;
+}
+
+static void fd_fill_args(const phc_fd *h, FdStepArgs &a, int ca, int cb)
+{
+    std::memset(&a, 0, sizeof(a));
+    a.pn = h->lev[cb].as<float>();
+    a.pnm1 = h->lev[ca].as<float>();
+    a.vp = h->vp.as<float>();
+    a.L = h->L;
+    std::memcpy(a.cx, h->cx, sizeof(a.cx));
+    std::memcpy(a.cy, h->cy, sizeof(a.cy));
+    std::memcpy(a.cz, h->cz, sizeof(a.cz));
+    a.coef0 = h->coef0;
+    a.c0f = (float)h->coef0;
+    a.sw = h->sw;
+    a.fx = h->fx.as<float>();
+    a.fy = h->fy.as<float>();
+    a.fz = h->fz.as<float>();
+    a.src_off = fd_off(h->L, h->xs, h->ys, h->zs);
+    a.inj = 0;
+    a.rhs_next = 0.f;
+    a.trace = nullptr;
+    const int TJ = FD_TY * h->nj;
+    a.tiles_k = (h->nz + FD_TK - 1) / FD_TK;
+    a.tiles_j = (h->ny + TJ - 1) / TJ;
+    a.chunk_i = h->chunk_i;
+}
+
+static int fd_num_blocks(const FdStepArgs &a, int nx)
+{
+    const int chunks = (nx + a.chunk_i - 1) / a.chunk_i;
+    return a.tiles_k * a.tiles_j * chunks;
+}
+
+static phc_status fd_materialize(phc_fd *h, int level)
+{
+    if (!h->pending[level]) return PHC_OK;
+    if (h->sw > 0) {
+        dim3 grid((h->nz + 255) / 256, h->ny, h->nx);
+        hipLaunchKernelGGL(fd_damp_kernel, grid, dim3(256), 0, h->stream, h->lev[level].as<float>(),
+                           h->L, h->sw, h->fx.as<float>(), h->fy.as<float>(), h->fz.as<float>());
+        PHC_HIP(hipGetLastError());
+    }
+    h->pending[level] = 0;
+    return PHC_OK;
+}
+
+static phc_status fd_launch_step(phc_fd *h, const FdStepArgs &a, bool damp)
+{
+    FdKernelFn fn = fd_pick_kernel(h->R, h->nj, h->arith, damp);
+    if (!fn) return fail(PHC_ERR_INVALID, "no kernel for R=%d nj=%d", h->R, h->nj);
+    const int nb = fd_num_blocks(a, h->nx);
+    hipLaunchKernelGGL(fn, dim3(nb), dim3(FD_TK, FD_TY), 0, h->stream, a);
+    PHC_HIP(hipGetLastError());
+    return PHC_OK;
+}
+
+static void fd_drop_graph(phc_fd *h)
+{
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+    h->g_it0 = h->g_n = h->g_cur = h->g_damp = h->g_nrec = -1;
+}
+
+extern "C" {
+
+phc_status phc_fd_create(const phc_fd_params *p, phc_fd **out)
+{
+    if (!out) return fail(PHC_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    PHC_TRY(fd_validate(p));
+    phc_fd *h = new phc_fd();
+    h->nx = p->nx; h->ny = p->ny; h->nz = p->nz; h->R = p->lx;
+    h->xs = p->xs; h->ys = p->ys; h->zs = p->zs;
+    h->arith = p->arith;
+    h->sw = p->sponge_width;
+    h->alpha = p->sponge_alpha;
+    std::memset(h->cx, 0, sizeof(h->cx));
+    std::memset(h->cy, 0, sizeof(h->cy));
+    std::memset(h->cz, 0, sizeof(h->cz));
+    for (int l = 0; l <= h->R; ++l) {
+        h->cx[l] = p->coefx[l];
+        h->cy[l] = p->coefy[l];
+        h->cz[l] = p->coefz[l];
+    }
+    h->coef0 = p->coef0;
+    h->nsamples = p->nsamples;
+    h->nj = p->tile_j ? p->tile_j / FD_TY : 4;
+    h->chunk_i = p->chunk_i ? p->chunk_i : 128;
+    h->L = make_fd_layout(h->nx, h->ny, h->nz, h->R);
+    phc_status st = PHC_OK;
+    auto bail = [&](phc_status s) { phc_fd_destroy(h); return s; };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(PHC_ERR_HIP, "hipStreamCreate failed"));
+    const size_t bytes = sizeof(float) * (size_t)h->L.total;
+    if ((st = h->lev[0].alloc(bytes)) || (st = h->lev[1].alloc(bytes)) || (st = h->vp.alloc(bytes)))
+        return bail(st);
+    // both levels (and their halo and padding) start at init_value
+    {
+        std::vector<float> init((size_t)h->L.total, p->init_value);
+        if (hipMemcpy(h->lev[0].ptr, init.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(h->lev[1].ptr, init.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(PHC_ERR_HIP, "initial upload failed"));
+    }
+    if ((st = fd_upload_interior(h, p->vp, h->vp.as<float>()))) return bail(st);
+    h->hvp.assign(p->vp, p->vp + (size_t)h->nx * h->ny * h->nz);
+    fd_axis_damping(h->nx, h->sw, h->alpha, h->hfx);
+    fd_axis_damping(h->ny, h->sw, h->alpha, h->hfy);
+    fd_axis_damping(h->nz, h->sw, h->alpha, h->hfz);
+    if ((st = h->fx.alloc(sizeof(float) * h->nx)) || (st = h->fy.alloc(sizeof(float) * h->ny)) ||
+        (st = h->fz.alloc(sizeof(float) * h->nz)))
+        return bail(st);
+    if (hipMemcpy(h->fx.ptr, h->hfx.data(), sizeof(float) * h->nx, hipMemcpyHostToDevice) ||
+        hipMemcpy(h->fy.ptr, h->hfy.data(), sizeof(float) * h->ny, hipMemcpyHostToDevice) ||
+        hipMemcpy(h->fz.ptr, h->hfz.data(), sizeof(float) * h->nz, hipMemcpyHostToDevice))
+        return bail(fail(PHC_ERR_HIP, "sponge upload failed"));
+    h->hrhs.assign(p->rhs, p->rhs + h->nsamples);
+    if (h->nsamples > 0) {
+        if ((st = h->rhs.alloc(sizeof(float) * h->nsamples))) return bail(st);
+        if (hipMemcpy(h->rhs.ptr, p->rhs, sizeof(float) * h->nsamples, hipMemcpyHostToDevice))
+            return bail(fail(PHC_ERR_HIP, "rhs upload failed"));
+    }
+    *out = h;
+    return PHC_OK;
+}
+
+phc_status phc_fd_destroy(phc_fd *h)
+{
+    if (!h) return PHC_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    fd_drop_graph(h);
+    for (DevBuf *b : {&h->lev[0], &h->lev[1], &h->vp, &h->fx, &h->fy, &h->fz, &h->rhs, &h->trace,
+                      &h->rec, &h->recoff})
+        b->release();
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return PHC_OK;
+}
+
+phc_status phc_fd_add_rhs(phc_fd *h, int it, int cb)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    if (cb < 0 || cb > 1) return fail(PHC_ERR_INVALID, "level %d is not 0 or 1", cb);
+    if (it < 0 || it >= h->nsamples)
+        return fail(PHC_ERR_INVALID, "time sample %d outside 0..%d", it, h->nsamples - 1);
+    PHC_TRY(fd_materialize(h, cb));
+    hipLaunchKernelGGL(fd_add_rhs_kernel, dim3(1), dim3(64), 0, h->stream, h->lev[cb].as<float>(),
+                       h->vp.as<float>(), fd_off(h->L, h->xs, h->ys, h->zs), h->rhs.as<float>(), it);
+    PHC_HIP(hipGetLastError());
+    return PHC_OK;
+}
+
+phc_status phc_fd_compute_one_step_sb(phc_fd *h, int ca, int cb)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    if (ca < 0 || ca > 1 || cb < 0 || cb > 1 || ca == cb)
+        return fail(PHC_ERR_INVALID, "levels ca=%d cb=%d must be 0 and 1", ca, cb);
+    PHC_TRY(fd_materialize(h, cb));
+    FdStepArgs a;
+    fd_fill_args(h, a, ca, cb);
+    PHC_TRY(fd_launch_step(h, a, h->pending[ca] != 0));
+    h->pending[ca] = 0;
+    h->pending[cb] = (h->sw > 0) ? 1 : 0;
+    return PHC_OK;
+}
+
+phc_status phc_fd_set_receivers(phc_fd *h, int nrec, const int *ijk)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    if (nrec < 0 || (nrec > 0 && !ijk)) return fail(PHC_ERR_INVALID, "bad receiver list");
+    std::vector<int64_t> offs(nrec);
+    for (int r = 0; r < nrec; ++r) {
+        const int i = ijk[3 * r], j = ijk[3 * r + 1], k = ijk[3 * r + 2];
+        if (i < 0 || i >= h->nx || j < 0 || j >= h->ny || k < 0 || k >= h->nz)
+            return fail(PHC_ERR_INVALID, "receiver %d at (%d,%d,%d) is outside the grid", r, i, j, k);
+        offs[r] = fd_off(h->L, i, j, k);
+    }
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    fd_drop_graph(h);
+    h->rec_offs = offs;
+    h->nrec = nrec;
+    if (nrec > 0) {
+        PHC_TRY(h->recoff.alloc(sizeof(int64_t) * nrec));
+        PHC_HIP(hipMemcpy(h->recoff.ptr, offs.data(), sizeof(int64_t) * nrec, hipMemcpyHostToDevice));
+    }
+    return PHC_OK;
+}
+
+
+} // extern "C"
+
+// Enqueue the whole run on h->stream: prologue injection of rhs[it0] into the
+// current level, then one fused step per time sample (injection of the next
+// sample folded into the store), plus receiver sampling.  With `ev`, events
+// bracket every stencil launch (timing mode).
+static phc_status fd_enqueue_run(phc_fd *h, int it0, int nsteps, int c0, bool damp0, hipEvent_t *ev)
+{
+    hipLaunchKernelGGL(fd_add_rhs_kernel, dim3(1), dim3(64), 0, h->stream, h->lev[c0].as<float>(),
+                       h->vp.as<float>(), fd_off(h->L, h->xs, h->ys, h->zs), h->rhs.as<float>(),
+                       it0);
+    PHC_HIP(hipGetLastError());
+    int c = c0;
+    for (int s = 0; s < nsteps; ++s) {
+        const int it = it0 + s;
+        const int ca = 1 - c, cb = c;
+        FdStepArgs a;
+        fd_fill_args(h, a, ca, cb);
+        a.trace = h->trace.as<float>() + s;
+        if (s + 1 < nsteps) {
+            a.inj = 1;
+            a.rhs_next = h->hrhs[it + 1];
+        }
+        if (ev) PHC_HIP(hipEventRecord(ev[2 * s], h->stream));
+        PHC_TRY(fd_launch_step(h, a, s == 0 ? damp0 : (h->sw > 0)));
+        if (ev) PHC_HIP(hipEventRecord(ev[2 * s + 1], h->stream));
+        if (h->nrec > 0) {
+            hipLaunchKernelGGL(fd_sample_kernel, dim3((h->nrec + 255) / 256), dim3(256), 0,
+                               h->stream, h->lev[ca].as<float>(), h->recoff.as<int64_t>(), h->nrec,
+                               h->rec.as<float>(), nsteps, s, a.src_off, h->trace.as<float>());
+            PHC_HIP(hipGetLastError());
+        }
+        c = ca;
+    }
+    return PHC_OK;
+}
+
+extern "C" {
+
+phc_status phc_fd_run(phc_fd *h, int it0, int nsteps, int *cur)
+{
+    if (!h || !cur) return fail(PHC_ERR_INVALID, "handle or cur is NULL");
+    if (*cur < 0 || *cur > 1) return fail(PHC_ERR_INVALID, "cur level %d is not 0 or 1", *cur);
+    if (nsteps < 0) return fail(PHC_ERR_INVALID, "negative step count");
+    if (nsteps == 0) return PHC_OK;
+    if (it0 < 0 || it0 + nsteps > h->nsamples)
+        return fail(PHC_ERR_INVALID, "steps %d..%d outside the %d source samples", it0,
+                    it0 + nsteps - 1, h->nsamples);
+    const int c0 = *cur;
+    PHC_TRY(fd_materialize(h, c0));
+    const bool damp0 = h->pending[1 - c0] != 0;
+    if (h->trace.bytes < sizeof(float) * nsteps) {
+        PHC_HIP(hipStreamSynchronize(h->stream));
+        fd_drop_graph(h);
+        PHC_TRY(h->trace.alloc(sizeof(float) * nsteps));
+    }
+    if (h->nrec > 0 && h->rec.bytes < sizeof(float) * (size_t)nsteps * h->nrec) {
+        PHC_HIP(hipStreamSynchronize(h->stream));
+        fd_drop_graph(h);
+        PHC_TRY(h->rec.alloc(sizeof(float) * (size_t)nsteps * h->nrec));
+    }
+    if (h->timing) {
+        std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
+        for (auto &e : ev) PHC_HIP(hipEventCreate(&e));
+        phc_status st = fd_enqueue_run(h, it0, nsteps, c0, damp0, ev.data());
+        if (st == PHC_OK) {
+            PHC_HIP(hipStreamSynchronize(h->stream));
+            double tot = 0.0;
+            for (int s = 0; s < nsteps; ++s) {
+                float ms = 0.f;
+                PHC_HIP(hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]));
+                tot += ms;
+            }
+            h->last_ms = tot / nsteps;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        PHC_TRY(st);
+    } else {
+        const int damp_key = damp0 ? 1 : 0;
+        if (!(h->gexec && h->g_it0 == it0 && h->g_n == nsteps && h->g_cur == c0 &&
+              h->g_damp == damp_key && h->g_nrec == h->nrec)) {
+            fd_drop_graph(h);
+            hipGraph_t graph = nullptr;
+            PHC_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            phc_status st = fd_enqueue_run(h, it0, nsteps, c0, damp0, nullptr);
+            hipError_t ce = hipStreamEndCapture(h->stream, &graph);
+            if (st != PHC_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return st;
+            }
+            if (ce != hipSuccess) return fail(PHC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+            hipError_t ie = hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) {
+                h->gexec = nullptr;
+                return fail(PHC_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+            }
+            h->g_it0 = it0; h->g_n = nsteps; h->g_cur = c0; h->g_damp = damp_key; h->g_nrec = h->nrec;
+        }
+        PHC_HIP(hipGraphLaunch(h->gexec, h->stream));
+    }
+    const int cend = (nsteps % 2 == 0) ? c0 : 1 - c0;
+    h->pending[cend] = 0;
+    h->pending[1 - cend] = (h->sw > 0) ? 1 : 0;
+    h->last_n = nsteps;
+    h->last_nrec = h->nrec;
+    *cur = cend;
+    return PHC_OK;
+}
+
+phc_status phc_fd_get_traces(phc_fd *h, float *src, float *rec)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    if (src && h->last_n > 0)
+        PHC_HIP(hipMemcpy(src, h->trace.ptr, sizeof(float) * h->last_n, hipMemcpyDeviceToHost));
+    if (rec && h->last_n > 0 && h->last_nrec > 0)
+        PHC_HIP(hipMemcpy(rec, h->rec.ptr, sizeof(float) * (size_t)h->last_n * h->last_nrec,
+                          hipMemcpyDeviceToHost));
+    return PHC_OK;
+}
+
+phc_status phc_fd_get_field(phc_fd *h, int level, float *host)
+{
+    if (!h || !host) return fail(PHC_ERR_INVALID, "handle or host buffer is NULL");
+    if (level < 0 || level > 1) return fail(PHC_ERR_INVALID, "level %d is not 0 or 1", level);
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    std::vector<float> buf((size_t)h->L.total);
+    PHC_HIP(hipMemcpy(buf.data(), h->lev[level].ptr, sizeof(float) * h->L.total, hipMemcpyDeviceToHost));
+    const int R = h->R;
+    const int NYe = h->ny + 2 * R, NZe = h->nz + 2 * R;
+    for (int i = -R; i < h->nx + R; ++i)
+        for (int j = -R; j < h->ny + R; ++j) {
+            float *dst = host + ((int64_t)(i + R) * NYe + (j + R)) * NZe;
+            const float *srcp = &buf[(size_t)fd_off(h->L, i, j, -R)];
+            std::memcpy(dst, srcp, sizeof(float) * NZe);
+            const bool row_in = i >= 0 && i < h->nx && j >= 0 && j < h->ny;
+            if (h->pending[level] && row_in)
+                for (int k = 0; k < h->nz; ++k) dst[k + R] = dst[k + R] * fd_sponge_at(h, i, j, k);
+        }
+    return PHC_OK;
+}
+
+phc_status phc_fd_set_field(phc_fd *h, int level, const float *host)
+{
+    if (!h || !host) return fail(PHC_ERR_INVALID, "handle or host buffer is NULL");
+    if (level < 0 || level > 1) return fail(PHC_ERR_INVALID, "level %d is not 0 or 1", level);
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    std::vector<float> buf((size_t)h->L.total);
+    PHC_HIP(hipMemcpy(buf.data(), h->lev[level].ptr, sizeof(float) * h->L.total, hipMemcpyDeviceToHost));
+    const int R = h->R;
+    const int NYe = h->ny + 2 * R, NZe = h->nz + 2 * R;
+    for (int i = -R; i < h->nx + R; ++i)
+        for (int j = -R; j < h->ny + R; ++j)
+            std::memcpy(&buf[(size_t)fd_off(h->L, i, j, -R)],
+                        host + ((int64_t)(i + R) * NYe + (j + R)) * NZe, sizeof(float) * NZe);
+    PHC_HIP(hipMemcpy(h->lev[level].ptr, buf.data(), sizeof(float) * h->L.total, hipMemcpyHostToDevice));
+    h->pending[level] = 0;
+    return PHC_OK;
+}
+
+phc_status phc_fd_read_point(phc_fd *h, int level, int i, int j, int k, float *v)
+{
+    if (!h || !v) return fail(PHC_ERR_INVALID, "handle or output is NULL");
+    if (level < 0 || level > 1) return fail(PHC_ERR_INVALID, "level %d is not 0 or 1", level);
+    const int R = h->R;
+    if (i < -R || i >= h->nx + R || j < -R || j >= h->ny + R || k < -R || k >= h->nz + R)
+        return fail(PHC_ERR_INVALID, "point (%d,%d,%d) outside the grid and halo", i, j, k);
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    float x = 0.f;
+    PHC_HIP(hipMemcpy(&x, h->lev[level].as<float>() + fd_off(h->L, i, j, k), sizeof(float),
+                      hipMemcpyDeviceToHost));
+    const bool in = i >= 0 && i < h->nx && j >= 0 && j < h->ny && k >= 0 && k < h->nz;
+    if (h->pending[level] && in) x = x * fd_sponge_at(h, i, j, k);
+    *v = x;
+    return PHC_OK;
+}
+
+phc_status phc_fd_get_slice_xz(phc_fd *h, int level, int j, float *host)
+{
+    if (!h || !host) return fail(PHC_ERR_INVALID, "handle or host buffer is NULL");
+    if (j < 0 || j >= h->ny) return fail(PHC_ERR_INVALID, "slice j=%d outside 0..%d", j, h->ny - 1);
+    const int R = h->R;
+    std::vector<float> full((size_t)(h->nx + 2 * R) * (h->ny + 2 * R) * (h->nz + 2 * R));
+    PHC_TRY(phc_fd_get_field(h, level, full.data()));
+    const int NYe = h->ny + 2 * R, NZe = h->nz + 2 * R;
+    for (int k = 0; k < h->nz; ++k)
+        for (int i = 0; i < h->nx; ++i)
+            host[(int64_t)k * h->nx + i] = full[((int64_t)(i + R) * NYe + (j + R)) * NZe + (k + R)];
+    return PHC_OK;
+}
+
+phc_status phc_fd_check_finite(phc_fd *h, int level)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    const int R = h->R;
+    std::vector<float> full((size_t)(h->nx + 2 * R) * (h->ny + 2 * R) * (h->nz + 2 * R));
+    PHC_TRY(phc_fd_get_field(h, level, full.data()));
+    for (size_t n = 0; n < full.size(); ++n)
+        if (!std::isfinite(full[n]))
+            return fail(PHC_ERR_NONFINITE, "level %d holds a non-finite value at %zu", level, n);
+    return PHC_OK;
+}
+
+phc_status phc_fd_last_run_kernel_ms(phc_fd *h, double *ms)
+{
+    if (!h || !ms) return fail(PHC_ERR_INVALID, "handle or output is NULL");
+    *ms = h->last_ms;
+    return PHC_OK;
+}
+
+phc_status phc_fd_set_timing(phc_fd *h, int enable)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    h->timing = enable ? 1 : 0;
+    return PHC_OK;
+}
+
+phc_status phc_fd_synchronize(phc_fd *h)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    return PHC_OK;
+}
+
+} // extern "C"

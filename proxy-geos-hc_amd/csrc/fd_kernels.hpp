@@ -1,0 +1,63 @@
+// fd_kernels.hpp -- device data layout and kernel argument blocks of the FD
+// (radius-R star stencil) step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace phc {
+
+// Field layout in HBM.  Point (i,j,k), i,j,k in [-R, n+R), lives at
+//   origin + i*pitch_j + j*pitch_k + k.
+// k is the contiguous axis (as in the reference's IDX3_l, FDTDmacros.hpp:14);
+// each k-row is padded so that interior k = 0 starts on a 64-byte boundary
+// (kofs = 16 floats) and every row starts 64-byte aligned.  Levels, vp^2dt^2
+// and (for batched shots) per-shot arrays all use this layout.
+struct FdLayout {
+    int nx, ny, nz, R;
+    int64_t pitch_k;  // floats per k-row
+    int64_t pitch_j;  // floats per i-plane
+    int64_t origin;   // offset of interior point (0,0,0)
+    int64_t total;    // floats per array
+};
+
+inline FdLayout make_fd_layout(int nx, int ny, int nz, int R)
+{
+    FdLayout L;
+    L.nx = nx; L.ny = ny; L.nz = nz; L.R = R;
+    const int64_t kofs = 16;
+    L.pitch_k = ((kofs + nz + R) + 15) / 16 * 16;
+    L.pitch_j = L.pitch_k * (ny + 2 * R);
+    L.origin = (int64_t)R * L.pitch_j + (int64_t)R * L.pitch_k + kofs;
+    L.total = (int64_t)(nx + 2 * R) * L.pitch_j;
+    return L;
+}
+
+__host__ __device__ inline int64_t fd_off(const FdLayout &L, int i, int j, int k)
+{
+    return L.origin + (int64_t)i * L.pitch_j + (int64_t)j * L.pitch_k + k;
+}
+
+constexpr int FD_MAX_R = 6;
+constexpr int FD_TK = 64;  // k extent of a tile: one wavefront
+constexpr int FD_TY = 4;   // wavefronts per block
+
+struct FdStepArgs {
+    const float *pn;    // current level (read, with neighbours)
+    float *pnm1;        // previous level: read, then overwritten by the new level
+    const float *vp;    // vp^2 dt^2
+    FdLayout L;
+    float cx[FD_MAX_R + 1], cy[FD_MAX_R + 1], cz[FD_MAX_R + 1];
+    double coef0;
+    float c0f;
+    int sw;             // sponge width (0: no sponge)
+    const float *fx, *fy, *fz;  // per-axis face damping values (1 outside the slab)
+    int64_t src_off;    // offset of the source point
+    int inj;            // add vp[src]*rhs_next at src after the update
+    float rhs_next;
+    float *trace;       // if set: *trace = new value at src before injection
+    int tiles_k, tiles_j, chunk_i;
+};
+
+} // namespace phc

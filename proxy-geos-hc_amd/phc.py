@@ -1,0 +1,282 @@
+"""Python binding of libphc.so (ctypes over the C ABI in include/phc.h).
+
+The classes mirror the reference's solver surface so that tests and the
+benchmark read like the reference's own drivers:
+  FdSolver      FDTDKernel::addRHS / computeOneStepSB (src/fd/utilsFDTD/
+                FDTDkernels.hpp:4-131) + FDTDUtils::output (FDTDutils.hpp:193)
+  SemSolver     SEMsolver::computeOneStep / outputPnValues
+                (src/sem/solver/SEMsolver.cpp:21-142)
+  fd_setup      FDTDInit::init_coefficients/init_source/init_models
+                (src/fd/utilsFDTD/FDTDinit.hpp:202-317)
+
+There is no fallback: if libphc.so is missing or a call fails, this module
+raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libphc.so")
+_lib = None
+
+ARITH_REF = 0
+ARITH_FP32 = 1
+EXP_DOUBLE = 0
+EXP_FLOAT = 1
+
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+c_int, c_float, c_double, c_void_p = ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_void_p
+
+
+class PhcError(RuntimeError):
+    pass
+
+
+class FdParams(ctypes.Structure):
+    _fields_ = [("nx", c_int), ("ny", c_int), ("nz", c_int),
+                ("lx", c_int), ("ly", c_int), ("lz", c_int),
+                ("xs", c_int), ("ys", c_int), ("zs", c_int),
+                ("coefx", c_void_p), ("coefy", c_void_p), ("coefz", c_void_p),
+                ("coef0", c_double),
+                ("vp", c_void_p), ("rhs", c_void_p), ("nsamples", c_int),
+                ("sponge_width", c_int), ("sponge_alpha", c_float),
+                ("init_value", c_float), ("arith", c_int),
+                ("tile_j", c_int), ("chunk_i", c_int)]
+
+
+# name -> (restype, argtypes) of every exported C symbol declared in include/*.h
+SIGNATURES = {
+    "phc_last_error": (ctypes.c_char_p, []),
+    "phc_version": (ctypes.c_char_p, []),
+    "phc_device_count": (c_int, []),
+    "phc_device_synchronize": (c_int, []),
+    # setup (include/phc_setup.h)
+    "phc_fd_init_coef": (c_int, [c_int, c_float, f32p]),
+    "phc_fd_coef0": (c_double, [f32p, c_int, f32p, c_int, f32p, c_int]),
+    "phc_fd_compute_dt": (c_float, [c_float, f32p, c_int, f32p, c_int, f32p, c_int, c_float]),
+    "phc_ricker": (c_float, [c_float, c_float, c_int, c_int]),
+    "phc_source_term": (c_int, [c_int, c_float, c_float, c_int, c_int, f32p]),
+    "phc_fd_model_two_layer": (c_int, [c_int, c_int, c_int, c_float, c_float, f32p]),
+    "phc_fd_model_layered": (c_int, [c_int, c_int, c_int, c_int, c_float, c_float,
+                                     ctypes.c_uint64, c_float, c_void_p, f32p]),
+    # FD
+    "phc_fd_create": (c_int, [ctypes.POINTER(FdParams), ctypes.POINTER(c_void_p)]),
+    "phc_fd_destroy": (c_int, [c_void_p]),
+    "phc_fd_add_rhs": (c_int, [c_void_p, c_int, c_int]),
+    "phc_fd_compute_one_step_sb": (c_int, [c_void_p, c_int, c_int]),
+    "phc_fd_run": (c_int, [c_void_p, c_int, c_int, ctypes.POINTER(c_int)]),
+    "phc_fd_set_receivers": (c_int, [c_void_p, c_int, c_void_p]),
+    "phc_fd_get_traces": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "phc_fd_get_field": (c_int, [c_void_p, c_int, f32p]),
+    "phc_fd_set_field": (c_int, [c_void_p, c_int, f32p]),
+    "phc_fd_read_point": (c_int, [c_void_p, c_int, c_int, c_int, c_int, ctypes.POINTER(c_float)]),
+    "phc_fd_get_slice_xz": (c_int, [c_void_p, c_int, c_int, f32p]),
+    "phc_fd_check_finite": (c_int, [c_void_p, c_int]),
+    "phc_fd_last_run_kernel_ms": (c_int, [c_void_p, ctypes.POINTER(c_double)]),
+    "phc_fd_set_timing": (c_int, [c_void_p, c_int]),
+    "phc_fd_synchronize": (c_int, [c_void_p]),
+}
+
+
+def lib():
+    """Load libphc.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PhcError("libphc.so not found at %s: run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names from SIGNATURES that libphc.so actually exports."""
+    L = lib()
+    return [n for n in SIGNATURES if hasattr(L, n)]
+
+
+def check(st):
+    if st != 0:
+        raise PhcError("libphc error %d: %s" % (st, lib().phc_last_error().decode()))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+def device_count():
+    return lib().phc_device_count()
+
+
+# ------------------------------------------------------------------ setup
+def fd_coefs(L, dx):
+    c = np.zeros(L + 1, np.float32)
+    check(lib().phc_fd_init_coef(L, dx, c))
+    return c
+
+
+def source_term(nsamples, dt, f0, order=1, exp_variant=EXP_DOUBLE):
+    out = np.zeros(nsamples, np.float32)
+    check(lib().phc_source_term(nsamples, dt, f0, order, exp_variant, out))
+    return out
+
+
+class FdSetup:
+    """FDTDInit (src/fd/utilsFDTD/FDTDinit.hpp:36-317) for one grid: coefficients,
+    dt, source wavelet and model.  model: "two_layer" (the reference's
+    init_models), "homogeneous" (vmin everywhere) or ("layered", nlayers, seed)."""
+
+    def __init__(self, nx=150, ny=None, nz=None, L=4, dx=10.0, xs=None, ys=None, zs=None,
+                 f0=15.0, time_max=1.0, vmin=1500.0, vmax=4500.0, cfl=0.8, source_order=1,
+                 sponge_width=20, sponge_alpha=-0.00015, model="two_layer", nsamples=None):
+        self.nx = nx
+        self.ny = nx if ny is None else ny
+        self.nz = nx if nz is None else nz
+        self.L = L
+        self.xs = nx // 2 if xs is None else xs
+        self.ys = self.ny // 2 if ys is None else ys
+        self.zs = self.nz // 2 if zs is None else zs
+        self.cx = fd_coefs(L, dx)
+        self.cy = self.cx.copy()
+        self.cz = self.cx.copy()
+        self.coef0 = lib().phc_fd_coef0(self.cx, L, self.cy, L, self.cz, L)
+        self.dt = np.float32(lib().phc_fd_compute_dt(vmax, self.cx, L, self.cy, L, self.cz, L, cfl))
+        self.nsamples = int(np.float32(time_max) / self.dt) if nsamples is None else nsamples
+        self.f0 = f0
+        self.rhs = source_term(self.nsamples, self.dt, f0, source_order, EXP_DOUBLE)
+        n3 = self.nx * self.ny * self.nz
+        self.vp = np.zeros(n3, np.float32)
+        if model == "two_layer":
+            check(lib().phc_fd_model_two_layer(self.nx, self.ny, self.nz, vmin, self.dt, self.vp))
+        elif model == "homogeneous":
+            v0 = np.float32(np.float32(np.float32(vmin) * np.float32(vmin)) * self.dt) * self.dt
+            self.vp[:] = v0
+        elif isinstance(model, tuple) and model[0] == "layered":
+            nl, seed = model[1], model[2]
+            self.interfaces = np.zeros(max(nl - 1, 1), np.int32)
+            check(lib().phc_fd_model_layered(self.nx, self.ny, self.nz, nl, vmin, vmax, seed,
+                                             self.dt, _ptr(self.interfaces), self.vp))
+        else:
+            raise ValueError(model)
+        self.sponge_width = sponge_width
+        self.sponge_alpha = sponge_alpha
+
+
+class FdSolver:
+    """Device-resident FD state: two levels (pnGlobal(:,0), pnGlobal(:,1)) plus
+    vp^2 dt^2, sponge and source, with the FDTDKernel step surface."""
+
+    def __init__(self, setup, arith=ARITH_REF, init_value=1e-6, tile_j=0, chunk_i=0):
+        s = setup
+        self.setup = s
+        self.nx, self.ny, self.nz, self.L = s.nx, s.ny, s.nz, s.L
+        self._keep = [s.cx, s.cy, s.cz, s.vp, s.rhs]
+        p = FdParams()
+        p.nx, p.ny, p.nz = s.nx, s.ny, s.nz
+        p.lx = p.ly = p.lz = s.L
+        p.xs, p.ys, p.zs = s.xs, s.ys, s.zs
+        p.coefx, p.coefy, p.coefz = _ptr(s.cx), _ptr(s.cy), _ptr(s.cz)
+        p.coef0 = s.coef0
+        p.vp = _ptr(s.vp)
+        p.rhs = _ptr(s.rhs)
+        p.nsamples = s.nsamples
+        p.sponge_width = s.sponge_width
+        p.sponge_alpha = s.sponge_alpha
+        p.init_value = init_value
+        p.arith = arith
+        p.tile_j = tile_j
+        p.chunk_i = chunk_i
+        h = c_void_p()
+        check(lib().phc_fd_create(ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        self.i1, self.i2 = 0, 1   # FDTDInit::i1/i2 (FDTDinit.hpp:28-29)
+        self.nrec = 0
+
+    def close(self):
+        if self.h:
+            check(lib().phc_fd_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # FDTDKernel surface
+    def add_rhs(self, it, cb):
+        check(lib().phc_fd_add_rhs(self.h, it, cb))
+
+    def compute_one_step_sb(self, ca, cb):
+        check(lib().phc_fd_compute_one_step_sb(self.h, ca, cb))
+
+    def step(self, it):
+        """One iteration of FDTDmain.cpp:55-81 through the per-call surface."""
+        self.add_rhs(it, self.i2)
+        self.compute_one_step_sb(self.i1, self.i2)
+        self.i1, self.i2 = self.i2, self.i1
+
+    def run(self, it0, nsteps):
+        """Fused, graph-captured loop; returns (source trace, receiver traces)."""
+        cur = c_int(self.i2)
+        check(lib().phc_fd_run(self.h, it0, nsteps, ctypes.byref(cur)))
+        self.i2 = cur.value
+        self.i1 = 1 - cur.value
+        src = np.zeros(nsteps, np.float32)
+        rec = np.zeros(max(self.nrec, 1) * nsteps, np.float32)
+        check(lib().phc_fd_get_traces(self.h, _ptr(src), _ptr(rec) if self.nrec else None))
+        return src, (rec.reshape(self.nrec, nsteps) if self.nrec else None)
+
+    def set_receivers(self, ijk):
+        a = np.ascontiguousarray(ijk, np.int32).reshape(-1)
+        self.nrec = len(a) // 3
+        check(lib().phc_fd_set_receivers(self.h, self.nrec, _ptr(a)))
+
+    def field(self, level):
+        L = self.L
+        out = np.zeros((self.nx + 2 * L) * (self.ny + 2 * L) * (self.nz + 2 * L), np.float32)
+        check(lib().phc_fd_get_field(self.h, level, out))
+        return out.reshape(self.nx + 2 * L, self.ny + 2 * L, self.nz + 2 * L)
+
+    def set_field(self, level, arr):
+        check(lib().phc_fd_set_field(self.h, level, np.ascontiguousarray(arr, np.float32).reshape(-1)))
+
+    def current(self):
+        return self.field(self.i2)
+
+    def previous(self):
+        return self.field(self.i1)
+
+    def read_point(self, level, i, j, k):
+        v = c_float()
+        check(lib().phc_fd_read_point(self.h, level, i, j, k, ctypes.byref(v)))
+        return v.value
+
+    def slice_xz(self, level, j):
+        out = np.zeros(self.nz * self.nx, np.float32)
+        check(lib().phc_fd_get_slice_xz(self.h, level, j, out))
+        return out.reshape(self.nz, self.nx)
+
+    def check_finite(self, level):
+        return lib().phc_fd_check_finite(self.h, level)
+
+    def set_timing(self, on):
+        check(lib().phc_fd_set_timing(self.h, 1 if on else 0))
+
+    def last_kernel_ms(self):
+        v = c_double()
+        check(lib().phc_fd_last_run_kernel_ms(self.h, ctypes.byref(v)))
+        return v.value
+
+    def synchronize(self):
+        check(lib().phc_fd_synchronize(self.h))

@@ -1,0 +1,166 @@
+"""FD parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Reference-rounding mode (ARITH_REF) must be bit-exact with the oracle, which
+itself reproduces the reference's published prints.  FP32 mode is held to a
+relative L2 of 1e-5 after the stated number of steps (the north-star
+tolerance is 1e-4)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def make_pair(phc, oracle, nx, ny, nz, L=4, sponge=20, model="two_layer", nsamples=64,
+              xs=None, ys=None, zs=None, f0=15.0, dx=10.0, **kw):
+    s = phc.FdSetup(nx=nx, ny=ny, nz=nz, L=L, dx=dx, xs=xs, ys=ys, zs=zs, f0=f0,
+                    model=model, nsamples=nsamples, sponge_width=sponge)
+    P = oracle.FdProblem(nx=nx, ny=ny, nz=nz, L=L, dx=dx, xs=s.xs, ys=s.ys, zs=s.zs, f0=f0,
+                         model=s.vp.copy(), nsamples=nsamples, sponge_width=sponge)
+    assert np.array_equal(P.rhs, s.rhs) and P.dt == s.dt
+    return s, P
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def banner(msg):
+    print("\n[fd-gpu] " + msg, flush=True)
+
+
+@pytest.mark.parametrize("shape,L,sponge", [
+    ((40, 40, 40), 4, 6),
+    ((37, 29, 70), 4, 8),       # ragged: partial j and k tiles
+    ((24, 30, 131), 4, 20),     # grid narrower than twice the sponge on two axes
+    ((33, 65, 64), 4, 0),       # no sponge
+])
+def test_per_step_surface_bit_exact(gpu, oracle, shape, L, sponge):
+    nx, ny, nz = shape
+    banner("per-step %s L=%d sponge=%d" % (shape, L, sponge))
+    s, P = make_pair(gpu, oracle, nx, ny, nz, L=L, sponge=sponge, nsamples=40)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF)
+    n = 30
+    for it in range(n):
+        sol.step(it)
+    tr, _ = P.run(n)
+    assert np.array_equal(sol.current(), P.current())
+    assert np.array_equal(sol.previous(), P.previous())
+    assert sol.read_point(sol.i2, s.xs, s.ys, s.zs) == tr[-1]
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 6])
+def test_radius_bit_exact(gpu, oracle, L):
+    banner("radius %d" % L)
+    s, P = make_pair(gpu, oracle, 26, 23, 67, L=L, sponge=5, nsamples=30)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF)
+    src, _ = sol.run(0, 25)
+    tr, _ = P.run(25)
+    assert np.array_equal(src, tr)
+    assert np.array_equal(sol.current(), P.current())
+    assert np.array_equal(sol.previous(), P.previous())
+
+
+@pytest.mark.parametrize("tile_j,chunk_i", [(8, 7), (16, 128), (32, 16)])
+def test_fused_run_bit_exact_with_receivers(gpu, oracle, tile_j, chunk_i):
+    banner("fused run tile_j=%d chunk_i=%d" % (tile_j, chunk_i))
+    s, P = make_pair(gpu, oracle, 45, 50, 90, sponge=10, nsamples=80,
+                     xs=20, ys=22, zs=41)
+    rec = np.array([[20, 22, 41], [0, 0, 0], [44, 49, 89], [10, 30, 64], [20, 22, 40]], np.int32)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF, tile_j=tile_j, chunk_i=chunk_i)
+    sol.set_receivers(rec)
+    src, rv = sol.run(0, 60)
+    tr, orv = P.run(60, receivers=rec)
+    assert np.array_equal(src, tr)
+    assert np.array_equal(rv, orv)
+    assert np.array_equal(sol.current(), P.current())
+    assert np.array_equal(sol.previous(), P.previous())
+    # continuing with a second run and then the per-step surface stays exact
+    src2, _ = sol.run(60, 10)
+    tr2, _ = P.run(10)
+    assert np.array_equal(src2, tr2)
+    for it in range(70, 75):
+        sol.step(it)
+    P.run(5)
+    assert np.array_equal(sol.current(), P.current())
+
+
+def test_fp32_mode_within_tolerance(gpu, oracle):
+    banner("fp32 mode 64^3 x 200")
+    s, P = make_pair(gpu, oracle, 64, 64, 64, sponge=12, nsamples=220, f0=25.0)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_FP32)
+    src, _ = sol.run(0, 200)
+    tr, _ = P.run(200)
+    err = rel_l2(sol.current(), P.current())
+    print("fp32 rel L2 field %.3e trace %.3e" % (err, rel_l2(src, tr)))
+    assert err < 1e-5
+    assert rel_l2(src, tr) < 1e-5
+
+
+def test_published_prints_through_gpu(gpu, oracle):
+    """150^3, homogeneous 1500 m/s, f0 = 15 Hz: the GPU run prints what the
+    reference's CPU builds printed (test_20240512), and is bit-exact with the
+    oracle over the same 401 steps."""
+    banner("published 150^3 x 401")
+    d = json.load(open(os.path.join(GOLDEN, "published_traces.json")))
+    rec = d["result_Cypress_fd_SEQUENTIAL_EPYCROME_01.txt"]
+    s = gpu.FdSetup(nx=150, model="homogeneous")
+    assert s.nsamples == 1242
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF)
+    src, _ = sol.run(0, 401)
+    for step, txt in zip(rec["steps"], rec["values"]):
+        if step > 400:
+            break
+        assert "%f" % src[step] == txt, (step, src[step], txt)
+    P = oracle.FdProblem(nx=150, model="homogeneous")
+    tr, _ = P.run(401)
+    assert np.array_equal(src, tr)
+    assert np.array_equal(sol.current(), P.current())
+
+
+def test_full_size_512_layered_steps(gpu, oracle):
+    """The benchmark configuration (512^3, 6 seeded layers, dt from CFL 0.4,
+    f0 = 10 Hz): a few reference-rounding steps are bit-exact with the oracle,
+    and the FP32 mode stays within 1e-6 of them."""
+    banner("512^3 layered, 3 steps")
+    s = gpu.FdSetup(nx=512, model=("layered", 6, 20240512), f0=10.0, cfl=0.4, nsamples=1000)
+    P = oracle.FdProblem(nx=512, model=s.vp, f0=10.0, dt=s.dt, nsamples=1000)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF)
+    src, _ = sol.run(0, 3)
+    tr, _ = P.run(3)
+    assert np.array_equal(src, tr)
+    assert np.array_equal(sol.current(), P.current())
+    f32 = gpu.FdSolver(s, arith=gpu.ARITH_FP32)
+    f32.run(0, 3)
+    assert rel_l2(f32.current(), P.current()) < 1e-6
+
+
+def test_invalid_inputs_raise(gpu):
+    s = gpu.FdSetup(nx=16, nsamples=4, sponge_width=4)
+    sol = gpu.FdSolver(s)
+    with pytest.raises(gpu.PhcError):
+        sol.run(2, 5)            # past the end of the source samples
+    with pytest.raises(gpu.PhcError):
+        sol.compute_one_step_sb(0, 0)
+    with pytest.raises(gpu.PhcError):
+        sol.set_receivers([[16, 0, 0]])
+    s.xs = 99
+    with pytest.raises(gpu.PhcError):
+        gpu.FdSolver(s)
+
+
+def test_nonfinite_detection(gpu):
+    s = gpu.FdSetup(nx=16, nsamples=8, sponge_width=4)
+    sol = gpu.FdSolver(s)
+    sol.run(0, 2)
+    assert sol.check_finite(sol.i2) == 0
+    f = sol.current()
+    f[5, 6, 7] = np.nan
+    sol.set_field(sol.i2, f)
+    assert sol.check_finite(sol.i2) == -4
