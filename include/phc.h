@@ -40,6 +40,10 @@ const char *phc_version(void);
 int phc_device_count(void);
 /* hipDeviceSynchronize on the current device. */
 phc_status phc_device_synchronize(void);
+/* Plain streaming copy of `bytes` (device to device), iters times; mean ms per
+ * copy.  Calibrates the HBM byte counters and the practical copy bandwidth.
+ * No reference counterpart (measurement utility). */
+phc_status phc_copy_bandwidth(size_t bytes, int iters, double *ms_per_copy);
 
 /* ====================================================================== FD
  * Replaces FDTDKernel (src/fd/utilsFDTD/FDTDkernels.hpp:4-131) and the
@@ -73,6 +77,8 @@ typedef struct {
     /* kernel tiling; 0 selects the built-in default for the radius */
     int tile_j;        /* output rows per block along j (multiple of 4) */
     int chunk_i;       /* planes per block along the marching axis i */
+    int index64;       /* 1: force 64-bit element offsets (default: 32-bit when
+                        * every array is below 4 GiB) */
 } phc_fd_params;
 
 phc_status phc_fd_create(const phc_fd_params *p, phc_fd **out);

@@ -45,6 +45,23 @@ __device__ __forceinline__ float fd_update_ref(float pc, float pm, float v, doub
     return r * s;
 }
 
+// Workgroup barrier that orders LDS only: global loads issued before it stay
+// in flight (a plain __syncthreads() also waits for them).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Face damping of one axis at index i of n (FDTDinit.hpp:343-395: the high
+// face loop runs last, so it wins where the faces overlap).
+__device__ __forceinline__ float fd_face_damping(const FdStepArgs &a, int i, int n)
+{
+    const int d = (i >= n - a.sw) ? a.sw - (n - i) : ((i < a.sw) ? a.sw - i : -1);
+    return d >= 0 ? a.stab[d] : 1.f;
+}
+
 __device__ __forceinline__ float mul_nc(float a, float b)
 {
 #pragma clang fp contract(off)
@@ -57,14 +74,53 @@ __device__ __forceinline__ float add_nc(float a, float b)
     return a + b;
 }
 
-template <int R, int NJ, int ARITH, bool DAMP_PREV>
+// Load through a kernel-argument base pointer.  With 32-bit offsets the
+// offset is in bytes, so the access is base (scalar) + 32-bit vector offset.
+template <typename IT>
+__device__ __forceinline__ float ldo(const float *base, IT off)
+{
+    if constexpr (sizeof(IT) == 4) return *(const float *)((const char *)base + off);
+    else return base[off];
+}
+template <typename IT>
+__device__ __forceinline__ float ldo_nt(const float *base, IT off)
+{
+    if constexpr (sizeof(IT) == 4) return __builtin_nontemporal_load((const float *)((const char *)base + off));
+    else return __builtin_nontemporal_load(base + off);
+}
+template <typename IT>
+__device__ __forceinline__ void sto_nt(float *base, IT off, float v)
+{
+    if constexpr (sizeof(IT) == 4) __builtin_nontemporal_store(v, (float *)((char *)base + off));
+    else __builtin_nontemporal_store(v, base + off);
+}
+
+// One launch = inner3D + applySponge over the grid.
+//
+// A block owns a 64(k) x TJ(j) column and marches along i over chunk_i planes.
+// Per thread and point: a ring of 2R+2 registers holds pn along i (planes
+// i-R..i+R plus the one in flight); a ring of R+1 registers per halo element
+// holds the j/k halo of the planes staged next; two register sets hold pnm1
+// and vp^2dt^2 of planes i and i+1.  The march is unrolled by 2R+2, so every
+// ring index is a compile-time constant and no value is ever moved between
+// registers: a load has a full plane of work to complete before its register
+// is read.  The halo of plane p is requested together with pn(p) of the
+// register queue, i.e. when the neighbouring tile requests the same lines as
+// its interior, so halo reads hit L2.  The current plane plus its halo is
+// staged in LDS (double buffer, one LDS-only barrier per plane).
+// IT is the offset type: 32-bit byte offsets when every array is < 4 GiB.
+template <int R, int NJ, int ARITH, bool DAMP_PREV, typename IT>
 __global__ void __launch_bounds__(FD_TK *FD_TY)
 fd_step_kernel(FdStepArgs a)
 {
     constexpr int TJ = FD_TY * NJ;
     constexpr int LW = FD_TK + 2 * R;
     constexpr int LH = TJ + 2 * R;
-    __shared__ float tile[LH][LW];
+    constexpr int NH = (2 * R + FD_TY - 1) / FD_TY;  // j-halo rows per thread
+    constexpr int Q = 2 * R + 2;                     // queue ring = unroll factor
+    constexpr int F = R + 1;                         // halo ring (Q == 2F)
+    constexpr IT S = sizeof(IT) == 4 ? 4 : 1;        // offset units per element
+    __shared__ float tile[2][LH][LW];
 
     const FdLayout &L = a.L;
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -82,113 +138,165 @@ fd_step_kernel(FdStepArgs a)
     const int i0 = ic * a.chunk_i;
     const int i1 = min(i0 + a.chunk_i, L.nx);
     const int k = k0 + tx;
+    const IT pitch = (IT)L.pitch_j * S;
 
-    const bool kload = k < L.nz + R;  // inside the stored halo
+    // Every load is unconditional: lanes or rows outside the stored grid read
+    // the interior origin instead (their offsets point there), and what they
+    // stage in LDS is never read by a point that is computed.
+    const bool kload = k < L.nz + R;
     const bool kcomp = k < L.nz;
     int jr[NJ];
-    bool jload[NJ], jcomp[NJ];
+    bool jcomp[NJ], inJ[NJ];
     float fyv[NJ];
-    bool inJ[NJ];
+    IT colq[NJ];
 #pragma unroll
     for (int jj = 0; jj < NJ; ++jj) {
         jr[jj] = j0 + ty * NJ + jj;
-        jload[jj] = kload && (jr[jj] < L.ny + R);
+        const bool jl = kload && (jr[jj] < L.ny + R);
         jcomp[jj] = kcomp && (jr[jj] < L.ny);
         inJ[jj] = (jr[jj] >= a.sw) && (jr[jj] < L.ny - a.sw);
         fyv[jj] = (a.sw > 0 && jr[jj] < L.ny) ? a.fy[jr[jj]] : 1.f;
+        colq[jj] = (IT)fd_off(L, 0, jl ? jr[jj] : 0, jl ? k : 0) * S;
     }
     const bool inK = (k >= a.sw) && (k < L.nz - a.sw);
     const float fzv = (a.sw > 0 && kcomp) ? a.fz[k] : 1.f;
+    const bool khalo = tx < 2 * R;
+    const int kk = (tx < R) ? (k0 - R + tx) : (k0 + FD_TK + tx - R);
+    const int kcol = kk - k0 + R;
+    IT colk[NJ];
+#pragma unroll
+    for (int jj = 0; jj < NJ; ++jj) {
+        const bool l = khalo && kk < L.nz + R && jr[jj] < L.ny + R;
+        colk[jj] = (IT)fd_off(L, 0, l ? jr[jj] : 0, l ? kk : 0) * S;
+    }
+    int jhrow[NH];
+    IT colh[NH];
+#pragma unroll
+    for (int m = 0; m < NH; ++m) {
+        const int h = ty + FD_TY * m;
+        const int jh = (h < R) ? (j0 - R + h) : (j0 + TJ + h - R);
+        jhrow[m] = jh - j0 + R;
+        const bool l = (h < 2 * R) && (jh < L.ny + R) && kload;
+        colh[m] = (IT)fd_off(L, 0, l ? jh : 0, l ? k : 0) * S;
+    }
 
-    float q[NJ][2 * R + 1];
+    float q[NJ][Q];                  // pn of plane p at slot (p - i0 + R) % Q
+    float fk[NJ][F], fj[NH][F];      // halo of plane p at slot (p - i0) % F
+    float pm[2][NJ], vv[2][NJ];      // pnm1 / vp of plane p in set (p - i0) % 2
+
+    auto load_halo = [&](int slot, int p) {
+        const IT po = (IT)p * pitch;
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj) fk[jj][slot] = ldo(a.pn, colk[jj] + po);
+#pragma unroll
+        for (int m = 0; m < NH; ++m) fj[m][slot] = ldo(a.pn, colh[m] + po);
+    };
+    auto load_aux = [&](int set, int p) {
+        const IT po = (IT)p * pitch;
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj) {
+            pm[set][jj] = ldo_nt(a.pnm1, colq[jj] + po);
+            vv[set][jj] = ldo_nt(a.vp, colq[jj] + po);
+        }
+    };
+    auto stage = [&](int buf, int qslot, int hslot) {
+        float(*tl)[LW] = tile[buf];
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj) {
+            tl[ty * NJ + jj + R][tx + R] = q[jj][qslot];
+            if (khalo) tl[ty * NJ + jj + R][kcol] = fk[jj][hslot];
+        }
+#pragma unroll
+        for (int m = 0; m < NH; ++m)
+            if (ty + FD_TY * m < 2 * R) tl[jhrow[m]][tx + R] = fj[m][hslot];
+    };
+
+    // prologue: queue planes i0-R .. i0+R (slots 0..2R), halo planes
+    // i0 .. i0+R (slots 0..R), pnm1/vp of plane i0 (set 0); stage plane i0
 #pragma unroll
     for (int jj = 0; jj < NJ; ++jj)
 #pragma unroll
-        for (int l = 0; l < 2 * R; ++l)
-            q[jj][l] = jload[jj] ? a.pn[fd_off(L, i0 - R + l, jr[jj], k)] : 0.f;
+        for (int l = 0; l <= 2 * R; ++l) q[jj][l] = ldo(a.pn, colq[jj] + (IT)(i0 - R + l) * pitch);
+#pragma unroll
+    for (int d = 0; d <= R; ++d) load_halo(d, i0 + d);
+    load_aux(0, i0);
+    stage(0, R, 0);
+    lds_barrier();
 
-    for (int i = i0; i < i1; ++i) {
-        float pm[NJ], vv[NJ];
+    for (int ib = i0; ib < i1; ib += Q) {
 #pragma unroll
-        for (int jj = 0; jj < NJ; ++jj) {
-            q[jj][2 * R] = jload[jj] ? a.pn[fd_off(L, i + R, jr[jj], k)] : 0.f;
-            const int64_t o = fd_off(L, i, jr[jj], k);
-            pm[jj] = jcomp[jj] ? a.pnm1[o] : 0.f;
-            vv[jj] = jcomp[jj] ? a.vp[o] : 0.f;
-        }
-        // stage plane i: centre values, k-halo, j-halo
+        for (int u = 0; u < Q; ++u) {
+            const int i = ib + u;
+            if (i >= i1) break;
+            // Prefetch for plane i+1, issued unconditionally (past the end of
+            // the grid the plane index is clamped; those values are never
+            // used): with a conditional prefetch the wait-count insertion has
+            // to assume no loads were issued and drains every load in flight.
+            {
+                const int pq = min(i + 1 + R, L.nx + R - 1);
+                const IT po = (IT)pq * pitch;
 #pragma unroll
-        for (int jj = 0; jj < NJ; ++jj) tile[ty * NJ + jj + R][tx + R] = q[jj][R];
-        if (tx < 2 * R) {
-            const int kk = (tx < R) ? (k0 - R + tx) : (k0 + FD_TK + tx - R);
-            const int col = kk - k0 + R;
-            if (kk < L.nz + R) {
+                for (int jj = 0; jj < NJ; ++jj) q[jj][(u + 1 + 2 * R) % Q] = ldo(a.pn, colq[jj] + po);
+                load_halo((u + 1 + R) % F, pq);
+                load_aux((u + 1) & 1, min(i + 1, L.nx - 1));
+            }
+            // compute plane i
+            const float(*tl)[LW] = tile[u & 1];
+            const bool inI = (i >= a.sw) && (i < L.nx - a.sw);
+            const float fxv = (a.sw > 0) ? fd_face_damping(a, i, L.nx) : 1.f;
+            const IT pi = (IT)i * pitch;
 #pragma unroll
-                for (int jj = 0; jj < NJ; ++jj) {
-                    const int j = jr[jj];
-                    if (j < L.ny + R) tile[ty * NJ + jj + R][col] = a.pn[fd_off(L, i, j, kk)];
+            for (int jj = 0; jj < NJ; ++jj) {
+                if (!jcomp[jj]) continue;
+                const int r = ty * NJ + jj + R;
+                const int c = tx + R;
+                float s = 1.f;
+                if (a.sw > 0) {
+                    if (inJ[jj] && inK && !inI) s = fxv;
+                    if (inK && inI && !inJ[jj]) s = fyv[jj];
+                    if (inJ[jj] && inI && !inK) s = fzv;
                 }
-            }
-        }
-        for (int h = ty; h < 2 * R; h += FD_TY) {
-            const int jh = (h < R) ? (j0 - R + h) : (j0 + TJ + h - R);
-            if (jh < L.ny + R && kload) tile[jh - j0 + R][tx + R] = a.pn[fd_off(L, i, jh, k)];
-        }
-        __syncthreads();
-
-        const bool inI = (i >= a.sw) && (i < L.nx - a.sw);
-        const float fxv = (a.sw > 0) ? a.fx[i] : 1.f;
-#pragma unroll
-        for (int jj = 0; jj < NJ; ++jj) {
-            if (!jcomp[jj]) continue;
-            const int r = ty * NJ + jj + R;
-            const int c = tx + R;
-            float s = 1.f;
-            if (a.sw > 0) {
-                if (inJ[jj] && inK && !inI) s = fxv;
-                if (inK && inI && !inJ[jj]) s = fyv[jj];
-                if (inJ[jj] && inI && !inK) s = fzv;
-            }
-            float val;
-            if constexpr (ARITH == PHC_FD_ARITH_REF) {
-                float lapx, lapy, lapz;
-                {
+                const float pc = q[jj][(u + R) % Q];
+                const float pmj = pm[u & 1][jj], vvj = vv[u & 1][jj];
+                float val;
+                if constexpr (ARITH == PHC_FD_ARITH_REF) {
+                    float lapx, lapy, lapz;
+                    {
 #pragma clang fp contract(off)
-                    lapx = 0.f;
+                        lapx = 0.f;
 #pragma unroll
-                    for (int l = 1; l <= R; ++l) lapx += a.cx[l] * (q[jj][R + l] + q[jj][R - l]);
-                    lapy = 0.f;
+                        for (int l = 1; l <= R; ++l)
+                            lapx += a.cx[l] * (q[jj][(u + R + l) % Q] + q[jj][(u + R - l) % Q]);
+                        lapy = 0.f;
 #pragma unroll
-                    for (int l = 1; l <= R; ++l) lapy += a.cy[l] * (tile[r + l][c] + tile[r - l][c]);
-                    lapz = 0.f;
+                        for (int l = 1; l <= R; ++l) lapy += a.cy[l] * (tl[r + l][c] + tl[r - l][c]);
+                        lapz = 0.f;
 #pragma unroll
-                    for (int l = 1; l <= R; ++l) lapz += a.cz[l] * (tile[r][c + l] + tile[r][c - l]);
+                        for (int l = 1; l <= R; ++l) lapz += a.cz[l] * (tl[r][c + l] + tl[r][c - l]);
+                    }
+                    const float pmv = DAMP_PREV ? mul_nc(pmj, s) : pmj;
+                    val = fd_update_ref(pc, pmv, vvj, a.coef0, lapx, lapy, lapz, s);
+                } else {
+                    float lap = a.c0f * pc;
+#pragma unroll
+                    for (int l = 1; l <= R; ++l) {
+                        lap = fmaf(a.cx[l], q[jj][(u + R + l) % Q] + q[jj][(u + R - l) % Q], lap);
+                        lap = fmaf(a.cy[l], tl[r + l][c] + tl[r - l][c], lap);
+                        lap = fmaf(a.cz[l], tl[r][c + l] + tl[r][c - l], lap);
+                    }
+                    const float pmv = DAMP_PREV ? pmj * s : pmj;
+                    val = s * fmaf(vvj, lap, 2.f * pc - pmv);
                 }
-                const float pmv = DAMP_PREV ? mul_nc(pm[jj], s) : pm[jj];
-                val = fd_update_ref(q[jj][R], pmv, vv[jj], a.coef0, lapx, lapy, lapz, s);
-            } else {
-                float lap = a.c0f * q[jj][R];
-#pragma unroll
-                for (int l = 1; l <= R; ++l) {
-                    lap = fmaf(a.cx[l], q[jj][R + l] + q[jj][R - l], lap);
-                    lap = fmaf(a.cy[l], tile[r + l][c] + tile[r - l][c], lap);
-                    lap = fmaf(a.cz[l], tile[r][c + l] + tile[r][c - l], lap);
+                const IT o = colq[jj] + pi;
+                if ((int64_t)(o / S) == a.src_off) {
+                    if (a.trace) *a.trace = val;
+                    if (a.inj) val = add_nc(val, mul_nc(vvj, a.rhs_next));
                 }
-                const float pmv = DAMP_PREV ? pm[jj] * s : pm[jj];
-                val = s * fmaf(vv[jj], lap, 2.f * q[jj][R] - pmv);
+                sto_nt(a.pnm1, o, val);
             }
-            const int64_t o = fd_off(L, i, jr[jj], k);
-            if (o == a.src_off) {
-                if (a.trace) *a.trace = val;
-                if (a.inj) val = add_nc(val, mul_nc(vv[jj], a.rhs_next));
-            }
-            a.pnm1[o] = val;
+            stage((u + 1) & 1, (u + 1 + R) % Q, (u + 1) % F);
+            lds_barrier();
         }
-        __syncthreads();
-#pragma unroll
-        for (int jj = 0; jj < NJ; ++jj)
-#pragma unroll
-            for (int l = 0; l < 2 * R; ++l) q[jj][l] = q[jj][l + 1];
     }
 }
 
@@ -233,36 +341,37 @@ __global__ void fd_sample_kernel(const float *p, const int64_t *offs, int nrec, 
 
 using FdKernelFn = void (*)(FdStepArgs);
 
-template <int R, int NJ>
+template <int R, int NJ, typename IT>
 static FdKernelFn pick_arith(int arith, bool damp)
 {
     if (arith == PHC_FD_ARITH_REF)
-        return damp ? fd_step_kernel<R, NJ, PHC_FD_ARITH_REF, true>
-                    : fd_step_kernel<R, NJ, PHC_FD_ARITH_REF, false>;
-    return damp ? fd_step_kernel<R, NJ, PHC_FD_ARITH_FP32, true>
-                : fd_step_kernel<R, NJ, PHC_FD_ARITH_FP32, false>;
+        return damp ? fd_step_kernel<R, NJ, PHC_FD_ARITH_REF, true, IT>
+                    : fd_step_kernel<R, NJ, PHC_FD_ARITH_REF, false, IT>;
+    return damp ? fd_step_kernel<R, NJ, PHC_FD_ARITH_FP32, true, IT>
+                : fd_step_kernel<R, NJ, PHC_FD_ARITH_FP32, false, IT>;
 }
 
 template <int R>
-static FdKernelFn pick_nj(int nj, int arith, bool damp)
+static FdKernelFn pick_nj(int nj, int arith, bool damp, bool idx32)
 {
     switch (nj) {
-    case 2: return pick_arith<R, 2>(arith, damp);
-    case 4: return pick_arith<R, 4>(arith, damp);
-    case 8: return pick_arith<R, 8>(arith, damp);
+    case 2: return idx32 ? pick_arith<R, 2, uint32_t>(arith, damp) : pick_arith<R, 2, int64_t>(arith, damp);
+    case 4: return idx32 ? pick_arith<R, 4, uint32_t>(arith, damp) : pick_arith<R, 4, int64_t>(arith, damp);
     default: return nullptr;
     }
 }
 
-FdKernelFn fd_pick_kernel(int R, int nj, int arith, bool damp)
+// Kernel for half width R, NJ rows per thread, arithmetic mode, deferred
+// damping of the previous level, 32-bit offsets when every array is < 4 GiB.
+FdKernelFn fd_pick_kernel(int R, int nj, int arith, bool damp, bool idx32)
 {
     switch (R) {
-    case 1: return pick_nj<1>(nj, arith, damp);
-    case 2: return pick_nj<2>(nj, arith, damp);
-    case 3: return pick_nj<3>(nj, arith, damp);
-    case 4: return pick_nj<4>(nj, arith, damp);
-    case 5: return pick_nj<5>(nj, arith, damp);
-    case 6: return pick_nj<6>(nj, arith, damp);
+    case 1: return pick_nj<1>(nj, arith, damp, idx32);
+    case 2: return pick_nj<2>(nj, arith, damp, idx32);
+    case 3: return pick_nj<3>(nj, arith, damp, idx32);
+    case 4: return pick_nj<4>(nj, arith, damp, idx32);
+    case 5: return pick_nj<5>(nj, arith, damp, idx32);
+    case 6: return pick_nj<6>(nj, arith, damp, idx32);
     default: return nullptr;
     }
 }
@@ -300,7 +409,7 @@ struct phc_fd {
     std::vector<float> hfx, hfy, hfz, hvp, hrhs;
     int pending[2] = {0, 0};
     hipStream_t stream = nullptr;
-    int nj = 4, chunk_i = 0;
+    int nj = 4, chunk_i = 0, index64 = 0;
     int nrec = 0;
     std::vector<int64_t> rec_offs;
     int last_n = 0, last_nrec = 0;
@@ -328,11 +437,12 @@ static phc_status fd_validate(const phc_fd_params *p)
         return fail(PHC_ERR_INVALID, "coefficient or vp array is NULL");
     if (p->nsamples < 0 || (p->nsamples > 0 && !p->rhs))
         return fail(PHC_ERR_INVALID, "rhs is NULL with nsamples = %d", p->nsamples);
-    if (p->sponge_width < 0) return fail(PHC_ERR_INVALID, "negative sponge width");
+    if (p->sponge_width < 0 || p->sponge_width > FD_MAX_SW)
+        return fail(PHC_ERR_INVALID, "sponge width %d outside 0..%d", p->sponge_width, FD_MAX_SW);
     if (p->arith != PHC_FD_ARITH_REF && p->arith != PHC_FD_ARITH_FP32)
         return fail(PHC_ERR_INVALID, "unknown arithmetic mode %d", p->arith);
-    if (p->tile_j != 0 && p->tile_j != 8 && p->tile_j != 16 && p->tile_j != 32)
-        return fail(PHC_ERR_INVALID, "tile_j must be 8, 16 or 32 (got %d)", p->tile_j);
+    if (p->tile_j != 0 && p->tile_j != 8 && p->tile_j != 16)
+        return fail(PHC_ERR_INVALID, "tile_j must be 8 or 16 (got %d)", p->tile_j);
     if (p->chunk_i < 0) return fail(PHC_ERR_INVALID, "negative chunk_i");
     const double pts = (double)(p->nx + 2 * p->lx) * (p->ny + 2 * p->ly) * (p->nz + 2 * p->lz + 32);
     if (pts > 1.5e10) return fail(PHC_ERR_INVALID, "grid too large (%.3g points)", pts);
@@ -384,6 +494,8 @@ static void fd_fill_args(const phc_fd *h, FdStepArgs &a, int ca, int cb)
     a.fx = h->fx.as<float>();
     a.fy = h->fy.as<float>();
     a.fz = h->fz.as<float>();
+    for (int d = 0; d <= h->sw && d <= FD_MAX_SW; ++d)
+        a.stab[d] = (float)std::exp((double)h->alpha * std::pow((double)d, 2.0));
     a.src_off = fd_off(h->L, h->xs, h->ys, h->zs);
     a.inj = 0;
     a.rhs_next = 0.f;
@@ -415,7 +527,8 @@ static phc_status fd_materialize(phc_fd *h, int level)
 
 static phc_status fd_launch_step(phc_fd *h, const FdStepArgs &a, bool damp)
 {
-    FdKernelFn fn = fd_pick_kernel(h->R, h->nj, h->arith, damp);
+    FdKernelFn fn = fd_pick_kernel(h->R, h->nj, h->arith, damp,
+                                   !h->index64 && h->L.total * 4 < ((int64_t)1 << 32));
     if (!fn) return fail(PHC_ERR_INVALID, "no kernel for R=%d nj=%d", h->R, h->nj);
     const int nb = fd_num_blocks(a, h->nx);
     hipLaunchKernelGGL(fn, dim3(nb), dim3(FD_TK, FD_TY), 0, h->stream, a);
@@ -453,8 +566,9 @@ phc_status phc_fd_create(const phc_fd_params *p, phc_fd **out)
     }
     h->coef0 = p->coef0;
     h->nsamples = p->nsamples;
-    h->nj = p->tile_j ? p->tile_j / FD_TY : 4;
-    h->chunk_i = p->chunk_i ? p->chunk_i : 128;
+    h->nj = p->tile_j ? p->tile_j / FD_TY : 2;
+    h->chunk_i = p->chunk_i ? p->chunk_i : 64;
+    h->index64 = p->index64 ? 1 : 0;
     h->L = make_fd_layout(h->nx, h->ny, h->nz, h->R);
     phc_status st = PHC_OK;
     auto bail = [&](phc_status s) { phc_fd_destroy(h); return s; };

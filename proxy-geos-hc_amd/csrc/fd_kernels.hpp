@@ -11,8 +11,8 @@ namespace phc {
 // Field layout in HBM.  Point (i,j,k), i,j,k in [-R, n+R), lives at
 //   origin + i*pitch_j + j*pitch_k + k.
 // k is the contiguous axis (as in the reference's IDX3_l, FDTDmacros.hpp:14);
-// each k-row is padded so that interior k = 0 starts on a 64-byte boundary
-// (kofs = 16 floats) and every row starts 64-byte aligned.  Levels, vp^2dt^2
+// each k-row is padded so that interior k = 0 starts on a 128-byte boundary
+// (kofs = 32 floats, one L2 line) and every row starts 128-byte aligned.  Levels, vp^2dt^2
 // and (for batched shots) per-shot arrays all use this layout.
 struct FdLayout {
     int nx, ny, nz, R;
@@ -26,8 +26,8 @@ inline FdLayout make_fd_layout(int nx, int ny, int nz, int R)
 {
     FdLayout L;
     L.nx = nx; L.ny = ny; L.nz = nz; L.R = R;
-    const int64_t kofs = 16;
-    L.pitch_k = ((kofs + nz + R) + 15) / 16 * 16;
+    const int64_t kofs = 32;
+    L.pitch_k = ((kofs + nz + R) + 31) / 32 * 32;
     L.pitch_j = L.pitch_k * (ny + 2 * R);
     L.origin = (int64_t)R * L.pitch_j + (int64_t)R * L.pitch_k + kofs;
     L.total = (int64_t)(nx + 2 * R) * L.pitch_j;
@@ -40,6 +40,7 @@ __host__ __device__ inline int64_t fd_off(const FdLayout &L, int i, int j, int k
 }
 
 constexpr int FD_MAX_R = 6;
+constexpr int FD_MAX_SW = 64;  // widest sponge handled by the step kernels
 constexpr int FD_TK = 64;  // k extent of a tile: one wavefront
 constexpr int FD_TY = 4;   // wavefronts per block
 
@@ -53,6 +54,7 @@ struct FdStepArgs {
     float c0f;
     int sw;             // sponge width (0: no sponge)
     const float *fx, *fy, *fz;  // per-axis face damping values (1 outside the slab)
+    float stab[FD_MAX_SW + 1];  // exp(alpha d^2), d = 0..sw (read with a uniform index)
     int64_t src_off;    // offset of the source point
     int inj;            // add vp[src]*rhs_next at src after the update
     float rhs_next;

@@ -45,7 +45,7 @@ class FdParams(ctypes.Structure):
                 ("vp", c_void_p), ("rhs", c_void_p), ("nsamples", c_int),
                 ("sponge_width", c_int), ("sponge_alpha", c_float),
                 ("init_value", c_float), ("arith", c_int),
-                ("tile_j", c_int), ("chunk_i", c_int)]
+                ("tile_j", c_int), ("chunk_i", c_int), ("index64", c_int)]
 
 
 # name -> (restype, argtypes) of every exported C symbol declared in include/*.h
@@ -54,6 +54,7 @@ SIGNATURES = {
     "phc_version": (ctypes.c_char_p, []),
     "phc_device_count": (c_int, []),
     "phc_device_synchronize": (c_int, []),
+    "phc_copy_bandwidth": (c_int, [ctypes.c_size_t, c_int, ctypes.POINTER(c_double)]),
     # setup (include/phc_setup.h)
     "phc_fd_init_coef": (c_int, [c_int, c_float, f32p]),
     "phc_fd_coef0": (c_double, [f32p, c_int, f32p, c_int, f32p, c_int]),
@@ -176,7 +177,8 @@ class FdSolver:
     """Device-resident FD state: two levels (pnGlobal(:,0), pnGlobal(:,1)) plus
     vp^2 dt^2, sponge and source, with the FDTDKernel step surface."""
 
-    def __init__(self, setup, arith=ARITH_REF, init_value=1e-6, tile_j=0, chunk_i=0):
+    def __init__(self, setup, arith=ARITH_REF, init_value=1e-6, tile_j=0, chunk_i=0,
+                 index64=False):
         s = setup
         self.setup = s
         self.nx, self.ny, self.nz, self.L = s.nx, s.ny, s.nz, s.L
@@ -196,6 +198,7 @@ class FdSolver:
         p.arith = arith
         p.tile_j = tile_j
         p.chunk_i = chunk_i
+        p.index64 = 1 if index64 else 0
         h = c_void_p()
         check(lib().phc_fd_create(ctypes.byref(p), ctypes.byref(h)))
         self.h = h

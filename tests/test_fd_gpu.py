@@ -67,13 +67,14 @@ def test_radius_bit_exact(gpu, oracle, L):
     assert np.array_equal(sol.previous(), P.previous())
 
 
-@pytest.mark.parametrize("tile_j,chunk_i", [(8, 7), (16, 128), (32, 16)])
-def test_fused_run_bit_exact_with_receivers(gpu, oracle, tile_j, chunk_i):
-    banner("fused run tile_j=%d chunk_i=%d" % (tile_j, chunk_i))
+@pytest.mark.parametrize("tile_j,chunk_i,index64", [(8, 7, False), (16, 128, False),
+                                                    (16, 5, True), (8, 64, True)])
+def test_fused_run_bit_exact_with_receivers(gpu, oracle, tile_j, chunk_i, index64):
+    banner("fused run tile_j=%d chunk_i=%d index64=%d" % (tile_j, chunk_i, index64))
     s, P = make_pair(gpu, oracle, 45, 50, 90, sponge=10, nsamples=80,
                      xs=20, ys=22, zs=41)
     rec = np.array([[20, 22, 41], [0, 0, 0], [44, 49, 89], [10, 30, 64], [20, 22, 40]], np.int32)
-    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF, tile_j=tile_j, chunk_i=chunk_i)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF, tile_j=tile_j, chunk_i=chunk_i, index64=index64)
     sol.set_receivers(rec)
     src, rv = sol.run(0, 60)
     tr, orv = P.run(60, receivers=rec)
