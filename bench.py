@@ -26,7 +26,18 @@ sys.path.insert(0, os.path.join(ROOT, "proxy-geos-hc_amd"))
 METRIC = "Mpoint-updates/s per MI355X (SEM & FD); learned vs sponge at ≤1% trace err"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE = 16
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fd_step_r4_512_traffic.json")
+# HBM bytes per launch of the stencil kernel, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE scaled by the same counters on a copy kernel of known size
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fd_step_r4_512_traffic.txt")
+
+
+def read_traffic():
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("bytes_per_launch:"):
+            return int(float(line.split(":")[1]))
+    return None
 
 
 def cpu_baseline(setup, nsteps=3):
@@ -72,6 +83,7 @@ def main():
                         nsamples=W + K + 1)
     arith = phc.ARITH_FP32 if args.arith == "fp32" else phc.ARITH_REF
     sol = phc.FdSolver(setup, arith=arith)
+    tile_j, waves, chunk_i = sol.autotune(steps_per_trial=10)
     # warmup (untimed), then the timed region with per-launch events
     sol.set_timing(True)
     if W:
@@ -89,16 +101,33 @@ def main():
     achieved = BYTES_PER_UPDATE * pts / (kernel_ms * 1e-3) / 1e9
 
     # the same K steps replayed as one captured hipGraph (no per-launch events)
-    sol2 = phc.FdSolver(setup, arith=arith)
+    sol2 = phc.FdSolver(setup, arith=arith, tile_j=tile_j, block_waves=waves, chunk_i=chunk_i)
     if W:
         sol2.run(0, W)
     sol2.run(W, K)  # capture + first replay
     sol2.synchronize()
+    t0 = time.perf_counter()
+    sol2.run(W, K)  # replay of the cached graph (same steps, fresh state irrelevant to timing)
+    sol2.synchronize()
+    graph_ms = (time.perf_counter() - t0) / K * 1e3
+    sol2.close()
     fin = sol.check_finite(sol.i2)
+    # reference-rounding arithmetic (FP64 combine) on the same workload
+    ref = phc.FdSolver(setup, arith=phc.ARITH_REF)
+    ref.autotune(steps_per_trial=10)
+    ref.set_timing(True)
+    ref.run(0, min(W, 20) or 1)
+    ref.run(0, min(K, 200))
+    ref_kernel_ms = ref.last_kernel_ms()
+    ref.close()
 
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        traffic = json.load(open(TRAFFIC_FILE)).get("bytes_per_launch")
+    traffic = read_traffic()
+    # a plain 1 GiB copy: the counter calibration kernel and the practical
+    # streaming bandwidth of this device
+    import ctypes
+    cms = ctypes.c_double()
+    phc.check(phc.lib().phc_copy_bandwidth(1 << 30, 5, ctypes.byref(cms)))
+    copy_gbs = 2 * (1 << 30) / (cms.value * 1e-3) / 1e9
 
     out = {
         "metric": METRIC,
@@ -115,11 +144,16 @@ def main():
         "data": "synthetic: seeded 6-layer vp model, Ricker source",
         "config": {"workload": "FD %d^3 radius-4 leapfrog, 20-pt sponge, 6 layers 1500-4500 m/s, "
                                "Ricker 10 Hz, dt CFL 0.4" % n,
-                   "grid": [n, n, n], "radius": 4, "arith": args.arith},
+                   "grid": [n, n, n], "radius": 4, "arith": args.arith,
+                   "tile_j": tile_j, "block_waves": waves, "chunk_i": chunk_i},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
         "kernel_ms": round(kernel_ms, 5),
+        "graph_ms_per_step": round(graph_ms, 5),
+        "ref_arith_kernel_ms": round(ref_kernel_ms, 5),
+        "ref_arith_mpts": round(pts / (ref_kernel_ms * 1e-3) / 1e6, 1),
+        "copy_gbs": round(copy_gbs, 1),
         "finite": fin == 0,
     }
     if not args.no_cpu:

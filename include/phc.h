@@ -75,7 +75,8 @@ typedef struct {
     float init_value;  /* initial value of both levels and of the halo (1e-6) */
     int arith;         /* PHC_FD_ARITH_* */
     /* kernel tiling; 0 selects the built-in default for the radius */
-    int tile_j;        /* output rows per block along j (multiple of 4) */
+    int tile_j;        /* output rows per block along j (block_waves x 1, 2 or 4) */
+    int block_waves;   /* wavefronts per block: 4, 8 or 16 */
     int chunk_i;       /* planes per block along the marching axis i */
     int index64;       /* 1: force 64-bit element offsets (default: 32-bit when
                         * every array is below 4 GiB) */
@@ -126,6 +127,14 @@ phc_status phc_fd_last_run_kernel_ms(phc_fd *h, double *ms_per_step);
 phc_status phc_fd_set_timing(phc_fd *h, int enable);
 /* Wait for all work queued on the handle's stream. */
 phc_status phc_fd_synchronize(phc_fd *h);
+
+/* Autotuner (no reference counterpart; the reference has fixed launch
+ * shapes, FDTDmacros.hpp:57-61): times each block shape and chunk length on
+ * scratch copies of the levels (the state is not advanced), keeps the fastest
+ * and reports it.  cache_path (may be NULL) is a text cache keyed by grid,
+ * radius and arithmetic; a cached entry is used without timing. */
+phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char *cache_path, int *tile_j,
+                           int *block_waves, int *chunk_i);
 
 #ifdef __cplusplus
 }

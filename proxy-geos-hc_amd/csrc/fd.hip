@@ -17,7 +17,7 @@
 //
 // The reference damps the current level in place after the stencil
 // (applySponge's second product).  Here that product is applied when the
-// level is next read as "previous" (DAMP_PREV), which gives the same float
+// level is next read as "previous" (damp_prev), which gives the same float
 // values without a second pass over the grid.
 #include <algorithm>
 #include <cmath>
@@ -109,14 +109,14 @@ __device__ __forceinline__ void sto_nt(float *base, IT off, float v)
 // its interior, so halo reads hit L2.  The current plane plus its halo is
 // staged in LDS (double buffer, one LDS-only barrier per plane).
 // IT is the offset type: 32-bit byte offsets when every array is < 4 GiB.
-template <int R, int NJ, int ARITH, bool DAMP_PREV, typename IT>
-__global__ void __launch_bounds__(FD_TK *FD_TY)
+template <int R, int TY, int NJ, int ARITH, typename IT>
+__global__ void __launch_bounds__(FD_TK *TY)
 fd_step_kernel(FdStepArgs a)
 {
-    constexpr int TJ = FD_TY * NJ;
+    constexpr int TJ = TY * NJ;
     constexpr int LW = FD_TK + 2 * R;
     constexpr int LH = TJ + 2 * R;
-    constexpr int NH = (2 * R + FD_TY - 1) / FD_TY;  // j-halo rows per thread
+    constexpr int NH = (2 * R + TY - 1) / TY;  // j-halo rows per thread
     constexpr int Q = 2 * R + 2;                     // queue ring = unroll factor
     constexpr int F = R + 1;                         // halo ring (Q == 2F)
     constexpr IT S = sizeof(IT) == 4 ? 4 : 1;        // offset units per element
@@ -173,7 +173,7 @@ fd_step_kernel(FdStepArgs a)
     IT colh[NH];
 #pragma unroll
     for (int m = 0; m < NH; ++m) {
-        const int h = ty + FD_TY * m;
+        const int h = ty + TY * m;
         const int jh = (h < R) ? (j0 - R + h) : (j0 + TJ + h - R);
         jhrow[m] = jh - j0 + R;
         const bool l = (h < 2 * R) && (jh < L.ny + R) && kload;
@@ -208,7 +208,7 @@ fd_step_kernel(FdStepArgs a)
         }
 #pragma unroll
         for (int m = 0; m < NH; ++m)
-            if (ty + FD_TY * m < 2 * R) tl[jhrow[m]][tx + R] = fj[m][hslot];
+            if (ty + TY * m < 2 * R) tl[jhrow[m]][tx + R] = fj[m][hslot];
     };
 
     // prologue: queue planes i0-R .. i0+R (slots 0..2R), halo planes
@@ -274,7 +274,8 @@ fd_step_kernel(FdStepArgs a)
 #pragma unroll
                         for (int l = 1; l <= R; ++l) lapz += a.cz[l] * (tl[r][c + l] + tl[r][c - l]);
                     }
-                    const float pmv = DAMP_PREV ? mul_nc(pmj, s) : pmj;
+                    // deferred damping of the previous level (x1 is exact)
+                    const float pmv = mul_nc(pmj, a.damp_prev ? s : 1.f);
                     val = fd_update_ref(pc, pmv, vvj, a.coef0, lapx, lapy, lapz, s);
                 } else {
                     float lap = a.c0f * pc;
@@ -284,7 +285,7 @@ fd_step_kernel(FdStepArgs a)
                         lap = fmaf(a.cy[l], tl[r + l][c] + tl[r - l][c], lap);
                         lap = fmaf(a.cz[l], tl[r][c + l] + tl[r][c - l], lap);
                     }
-                    const float pmv = DAMP_PREV ? pmj * s : pmj;
+                    const float pmv = pmj * (a.damp_prev ? s : 1.f);
                     val = s * fmaf(vvj, lap, 2.f * pc - pmv);
                 }
                 const IT o = colq[jj] + pi;
@@ -341,37 +342,44 @@ __global__ void fd_sample_kernel(const float *p, const int64_t *offs, int nrec, 
 
 using FdKernelFn = void (*)(FdStepArgs);
 
-template <int R, int NJ, typename IT>
-static FdKernelFn pick_arith(int arith, bool damp)
+template <int R, int TY, int NJ>
+static FdKernelFn pick_arith(int arith, bool idx32)
 {
     if (arith == PHC_FD_ARITH_REF)
-        return damp ? fd_step_kernel<R, NJ, PHC_FD_ARITH_REF, true, IT>
-                    : fd_step_kernel<R, NJ, PHC_FD_ARITH_REF, false, IT>;
-    return damp ? fd_step_kernel<R, NJ, PHC_FD_ARITH_FP32, true, IT>
-                : fd_step_kernel<R, NJ, PHC_FD_ARITH_FP32, false, IT>;
+        return idx32 ? fd_step_kernel<R, TY, NJ, PHC_FD_ARITH_REF, uint32_t>
+                     : fd_step_kernel<R, TY, NJ, PHC_FD_ARITH_REF, int64_t>;
+    return idx32 ? fd_step_kernel<R, TY, NJ, PHC_FD_ARITH_FP32, uint32_t>
+                 : fd_step_kernel<R, TY, NJ, PHC_FD_ARITH_FP32, int64_t>;
 }
 
+// Block shapes: TY wavefronts per block, NJ rows per thread (tile_j = TY*NJ).
+// Radius 4 (the 8th-order operator and order-4 SEM) gets the full set; the
+// other radii one shape.
 template <int R>
-static FdKernelFn pick_nj(int nj, int arith, bool damp, bool idx32)
+static FdKernelFn pick_shape(int ty, int nj, int arith, bool idx32)
 {
-    switch (nj) {
-    case 2: return idx32 ? pick_arith<R, 2, uint32_t>(arith, damp) : pick_arith<R, 2, int64_t>(arith, damp);
-    case 4: return idx32 ? pick_arith<R, 4, uint32_t>(arith, damp) : pick_arith<R, 4, int64_t>(arith, damp);
-    default: return nullptr;
+    if (ty == 4 && nj == 2) return pick_arith<R, 4, 2>(arith, idx32);
+    if constexpr (R == 4) {
+        if (ty == 4 && nj == 4) return pick_arith<R, 4, 4>(arith, idx32);
+        if (ty == 8 && nj == 1) return pick_arith<R, 8, 1>(arith, idx32);
+        if (ty == 8 && nj == 2) return pick_arith<R, 8, 2>(arith, idx32);
+        if (ty == 16 && nj == 1) return pick_arith<R, 16, 1>(arith, idx32);
+        if (ty == 16 && nj == 2) return pick_arith<R, 16, 2>(arith, idx32);
     }
+    return nullptr;
 }
 
-// Kernel for half width R, NJ rows per thread, arithmetic mode, deferred
-// damping of the previous level, 32-bit offsets when every array is < 4 GiB.
-FdKernelFn fd_pick_kernel(int R, int nj, int arith, bool damp, bool idx32)
+// Kernel for half width R and block shape, arithmetic mode, 32-bit offsets
+// when every array is < 4 GiB.
+FdKernelFn fd_pick_kernel(int R, int ty, int nj, int arith, bool idx32)
 {
     switch (R) {
-    case 1: return pick_nj<1>(nj, arith, damp, idx32);
-    case 2: return pick_nj<2>(nj, arith, damp, idx32);
-    case 3: return pick_nj<3>(nj, arith, damp, idx32);
-    case 4: return pick_nj<4>(nj, arith, damp, idx32);
-    case 5: return pick_nj<5>(nj, arith, damp, idx32);
-    case 6: return pick_nj<6>(nj, arith, damp, idx32);
+    case 1: return pick_shape<1>(ty, nj, arith, idx32);
+    case 2: return pick_shape<2>(ty, nj, arith, idx32);
+    case 3: return pick_shape<3>(ty, nj, arith, idx32);
+    case 4: return pick_shape<4>(ty, nj, arith, idx32);
+    case 5: return pick_shape<5>(ty, nj, arith, idx32);
+    case 6: return pick_shape<6>(ty, nj, arith, idx32);
     default: return nullptr;
     }
 }
@@ -409,7 +417,7 @@ struct phc_fd {
     std::vector<float> hfx, hfy, hfz, hvp, hrhs;
     int pending[2] = {0, 0};
     hipStream_t stream = nullptr;
-    int nj = 4, chunk_i = 0, index64 = 0;
+    int ty = 4, nj = 2, chunk_i = 0, index64 = 0;
     int nrec = 0;
     std::vector<int64_t> rec_offs;
     int last_n = 0, last_nrec = 0;
@@ -441,8 +449,15 @@ static phc_status fd_validate(const phc_fd_params *p)
         return fail(PHC_ERR_INVALID, "sponge width %d outside 0..%d", p->sponge_width, FD_MAX_SW);
     if (p->arith != PHC_FD_ARITH_REF && p->arith != PHC_FD_ARITH_FP32)
         return fail(PHC_ERR_INVALID, "unknown arithmetic mode %d", p->arith);
-    if (p->tile_j != 0 && p->tile_j != 8 && p->tile_j != 16)
-        return fail(PHC_ERR_INVALID, "tile_j must be 8 or 16 (got %d)", p->tile_j);
+    if (p->block_waves != 0 && p->block_waves != 4 && p->block_waves != 8 && p->block_waves != 16)
+        return fail(PHC_ERR_INVALID, "block_waves must be 4, 8 or 16 (got %d)", p->block_waves);
+    {
+        const int ty = p->block_waves ? p->block_waves : ((p->lx == 4) ? 16 : 4);
+        const int tj = p->tile_j ? p->tile_j : (p->block_waves ? ty * 2 : ((p->lx == 4) ? 16 : 8));
+        if (tj % ty != 0 || !fd_pick_kernel(p->lx, ty, tj / ty, p->arith, true))
+            return fail(PHC_ERR_INVALID, "no kernel for half width %d, tile_j %d, %d waves per block",
+                        p->lx, tj, ty);
+    }
     if (p->chunk_i < 0) return fail(PHC_ERR_INVALID, "negative chunk_i");
     const double pts = (double)(p->nx + 2 * p->lx) * (p->ny + 2 * p->ly) * (p->nz + 2 * p->lz + 32);
     if (pts > 1.5e10) return fail(PHC_ERR_INVALID, "grid too large (%.3g points)", pts);
@@ -500,7 +515,7 @@ static void fd_fill_args(const phc_fd *h, FdStepArgs &a, int ca, int cb)
     a.inj = 0;
     a.rhs_next = 0.f;
     a.trace = nullptr;
-    const int TJ = FD_TY * h->nj;
+    const int TJ = h->ty * h->nj;
     a.tiles_k = (h->nz + FD_TK - 1) / FD_TK;
     a.tiles_j = (h->ny + TJ - 1) / TJ;
     a.chunk_i = h->chunk_i;
@@ -525,13 +540,14 @@ static phc_status fd_materialize(phc_fd *h, int level)
     return PHC_OK;
 }
 
-static phc_status fd_launch_step(phc_fd *h, const FdStepArgs &a, bool damp)
+static phc_status fd_launch_step(phc_fd *h, FdStepArgs a, bool damp)
 {
-    FdKernelFn fn = fd_pick_kernel(h->R, h->nj, h->arith, damp,
+    FdKernelFn fn = fd_pick_kernel(h->R, h->ty, h->nj, h->arith,
                                    !h->index64 && h->L.total * 4 < ((int64_t)1 << 32));
-    if (!fn) return fail(PHC_ERR_INVALID, "no kernel for R=%d nj=%d", h->R, h->nj);
+    if (!fn) return fail(PHC_ERR_INVALID, "no kernel for R=%d block_waves=%d rows/thread=%d", h->R, h->ty, h->nj);
+    a.damp_prev = damp ? 1 : 0;
     const int nb = fd_num_blocks(a, h->nx);
-    hipLaunchKernelGGL(fn, dim3(nb), dim3(FD_TK, FD_TY), 0, h->stream, a);
+    hipLaunchKernelGGL(fn, dim3(nb), dim3(FD_TK, h->ty), 0, h->stream, a);
     PHC_HIP(hipGetLastError());
     return PHC_OK;
 }
@@ -566,8 +582,12 @@ phc_status phc_fd_create(const phc_fd_params *p, phc_fd **out)
     }
     h->coef0 = p->coef0;
     h->nsamples = p->nsamples;
-    h->nj = p->tile_j ? p->tile_j / FD_TY : 2;
-    h->chunk_i = p->chunk_i ? p->chunk_i : 64;
+    // default shape: the fastest measured at 512^3 for radius 4 (16 waves,
+    // 16 rows, whole i-extent per block); other radii have one shape
+    const int dty = (h->R == 4) ? 16 : 4, dtj = (h->R == 4) ? 16 : 8;
+    h->ty = p->block_waves ? p->block_waves : dty;
+    h->nj = (p->tile_j ? p->tile_j : (p->block_waves ? 2 * h->ty : dtj)) / h->ty;
+    h->chunk_i = p->chunk_i ? p->chunk_i : (h->R == 4 ? 512 : 256);
     h->index64 = p->index64 ? 1 : 0;
     h->L = make_fd_layout(h->nx, h->ny, h->nz, h->R);
     phc_status st = PHC_OK;
@@ -895,3 +915,100 @@ phc_status phc_fd_synchronize(phc_fd *h)
 }
 
 } // extern "C"
+
+// ================================================================ autotuner
+// Times every block shape / chunk length the library has a kernel for on the
+// handle's grid (on scratch copies of the two levels, so the simulation state
+// is untouched) and keeps the fastest.  Results are cached in a text file,
+// one line per (grid, radius, arithmetic): "fd nx ny nz R arith waves tile_j chunk_i ms".
+namespace phc {
+
+static bool fd_tune_lookup(const char *path, const phc_fd *h, int *ty, int *tj, int *ci)
+{
+    if (!path) return false;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return false;
+    char tag[8];
+    int nx, ny, nz, R, ar, a, b, c;
+    double ms;
+    bool hit = false;
+    while (std::fscanf(f, "%7s %d %d %d %d %d %d %d %d %lf", tag, &nx, &ny, &nz, &R, &ar, &a, &b, &c, &ms) == 10)
+        if (!std::strcmp(tag, "fd") && nx == h->nx && ny == h->ny && nz == h->nz && R == h->R && ar == h->arith) {
+            *ty = a; *tj = b; *ci = c;
+            hit = true;
+        }
+    std::fclose(f);
+    return hit && fd_pick_kernel(h->R, *ty, *tj / *ty, h->arith, true) && *ci > 0;
+}
+
+} // namespace phc
+
+extern "C" phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char *cache_path,
+                                      int *tile_j, int *block_waves, int *chunk_i)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    if (steps_per_trial < 1) return fail(PHC_ERR_INVALID, "steps_per_trial must be >= 1");
+    int ty = 0, tj = 0, ci = 0;
+    if (fd_tune_lookup(cache_path, h, &ty, &tj, &ci)) {
+        h->ty = ty; h->nj = tj / ty; h->chunk_i = ci;
+    } else {
+        const size_t bytes = sizeof(float) * (size_t)h->L.total;
+        DevBuf s0, s1;
+        PHC_TRY(s0.alloc(bytes));
+        PHC_TRY(s1.alloc(bytes));
+        PHC_HIP(hipMemcpyAsync(s0.ptr, h->lev[0].ptr, bytes, hipMemcpyDeviceToDevice, h->stream));
+        PHC_HIP(hipMemcpyAsync(s1.ptr, h->lev[1].ptr, bytes, hipMemcpyDeviceToDevice, h->stream));
+        hipEvent_t e0, e1;
+        PHC_HIP(hipEventCreate(&e0));
+        PHC_HIP(hipEventCreate(&e1));
+        const int shapes[][2] = {{4, 8}, {4, 16}, {8, 8}, {8, 16}, {16, 16}, {16, 32}};
+        const int chunks[] = {64, 128, 256, 512, 1024};
+        double best = 1e30;
+        int bty = h->ty, bnj = h->nj, bci = h->chunk_i;
+        phc_status st = PHC_OK;
+        for (auto &sh : shapes) {
+            if (!fd_pick_kernel(h->R, sh[0], sh[1] / sh[0], h->arith, true)) continue;
+            for (int c : chunks) {
+                if (c > h->nx && c != chunks[0]) continue;
+                h->ty = sh[0]; h->nj = sh[1] / sh[0]; h->chunk_i = std::min(c, h->nx);
+                float ms = 0.f;
+                for (int rep = 0; rep < 2 && st == PHC_OK; ++rep) {  // first pass warms up
+                    if (hipEventRecord(e0, h->stream) != hipSuccess) st = fail(PHC_ERR_HIP, "event record failed");
+                    for (int s = 0; s < steps_per_trial && st == PHC_OK; ++s) {
+                        FdStepArgs a;
+                        fd_fill_args(h, a, 0, 1);
+                        a.pn = s1.as<float>();
+                        a.pnm1 = s0.as<float>();
+                        if (s & 1) { a.pn = s0.as<float>(); a.pnm1 = s1.as<float>(); }
+                        st = fd_launch_step(h, a, false);
+                    }
+                    if (st == PHC_OK && (hipEventRecord(e1, h->stream) != hipSuccess ||
+                                         hipEventSynchronize(e1) != hipSuccess ||
+                                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                        st = fail(PHC_ERR_HIP, "trial timing failed");
+                }
+                if (st != PHC_OK) break;
+                if (ms < best) { best = ms; bty = h->ty; bnj = h->nj; bci = h->chunk_i; }
+            }
+            if (st != PHC_OK) break;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        h->ty = bty; h->nj = bnj; h->chunk_i = bci;
+        fd_drop_graph(h);
+        PHC_TRY(st);
+        if (cache_path) {
+            FILE *f = std::fopen(cache_path, "a");
+            if (f) {
+                std::fprintf(f, "fd %d %d %d %d %d %d %d %d %.5f\n", h->nx, h->ny, h->nz, h->R,
+                             h->arith, bty, bty * bnj, bci, best / steps_per_trial);
+                std::fclose(f);
+            }
+        }
+    }
+    fd_drop_graph(h);
+    if (tile_j) *tile_j = h->ty * h->nj;
+    if (block_waves) *block_waves = h->ty;
+    if (chunk_i) *chunk_i = h->chunk_i;
+    return PHC_OK;
+}

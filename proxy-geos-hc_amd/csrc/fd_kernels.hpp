@@ -42,7 +42,6 @@ __host__ __device__ inline int64_t fd_off(const FdLayout &L, int i, int j, int k
 constexpr int FD_MAX_R = 6;
 constexpr int FD_MAX_SW = 64;  // widest sponge handled by the step kernels
 constexpr int FD_TK = 64;  // k extent of a tile: one wavefront
-constexpr int FD_TY = 4;   // wavefronts per block
 
 struct FdStepArgs {
     const float *pn;    // current level (read, with neighbours)
@@ -56,6 +55,7 @@ struct FdStepArgs {
     const float *fx, *fy, *fz;  // per-axis face damping values (1 outside the slab)
     float stab[FD_MAX_SW + 1];  // exp(alpha d^2), d = 0..sw (read with a uniform index)
     int64_t src_off;    // offset of the source point
+    int damp_prev;      // previous level is read damped (deferred applySponge)
     int inj;            // add vp[src]*rhs_next at src after the update
     float rhs_next;
     float *trace;       // if set: *trace = new value at src before injection

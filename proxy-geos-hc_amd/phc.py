@@ -45,7 +45,8 @@ class FdParams(ctypes.Structure):
                 ("vp", c_void_p), ("rhs", c_void_p), ("nsamples", c_int),
                 ("sponge_width", c_int), ("sponge_alpha", c_float),
                 ("init_value", c_float), ("arith", c_int),
-                ("tile_j", c_int), ("chunk_i", c_int), ("index64", c_int)]
+                ("tile_j", c_int), ("block_waves", c_int), ("chunk_i", c_int),
+                ("index64", c_int)]
 
 
 # name -> (restype, argtypes) of every exported C symbol declared in include/*.h
@@ -80,6 +81,8 @@ SIGNATURES = {
     "phc_fd_last_run_kernel_ms": (c_int, [c_void_p, ctypes.POINTER(c_double)]),
     "phc_fd_set_timing": (c_int, [c_void_p, c_int]),
     "phc_fd_synchronize": (c_int, [c_void_p]),
+    "phc_fd_autotune": (c_int, [c_void_p, c_int, ctypes.c_char_p, ctypes.POINTER(c_int),
+                                ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
 }
 
 
@@ -178,7 +181,7 @@ class FdSolver:
     vp^2 dt^2, sponge and source, with the FDTDKernel step surface."""
 
     def __init__(self, setup, arith=ARITH_REF, init_value=1e-6, tile_j=0, chunk_i=0,
-                 index64=False):
+                 index64=False, block_waves=0):
         s = setup
         self.setup = s
         self.nx, self.ny, self.nz, self.L = s.nx, s.ny, s.nz, s.L
@@ -197,6 +200,7 @@ class FdSolver:
         p.init_value = init_value
         p.arith = arith
         p.tile_j = tile_j
+        p.block_waves = block_waves
         p.chunk_i = chunk_i
         p.index64 = 1 if index64 else 0
         h = c_void_p()
@@ -283,3 +287,11 @@ class FdSolver:
 
     def synchronize(self):
         check(lib().phc_fd_synchronize(self.h))
+
+    def autotune(self, steps_per_trial=10, cache_path=None):
+        """Pick the fastest block shape for this grid; returns (tile_j, waves, chunk_i)."""
+        tj, ty, ci = c_int(), c_int(), c_int()
+        cp = None if cache_path is None else cache_path.encode()
+        check(lib().phc_fd_autotune(self.h, steps_per_trial, cp, ctypes.byref(tj), ctypes.byref(ty),
+                                    ctypes.byref(ci)))
+        return tj.value, ty.value, ci.value

@@ -67,14 +67,16 @@ def test_radius_bit_exact(gpu, oracle, L):
     assert np.array_equal(sol.previous(), P.previous())
 
 
-@pytest.mark.parametrize("tile_j,chunk_i,index64", [(8, 7, False), (16, 128, False),
-                                                    (16, 5, True), (8, 64, True)])
-def test_fused_run_bit_exact_with_receivers(gpu, oracle, tile_j, chunk_i, index64):
-    banner("fused run tile_j=%d chunk_i=%d index64=%d" % (tile_j, chunk_i, index64))
+@pytest.mark.parametrize("tile_j,waves,chunk_i,index64", [
+    (8, 4, 7, False), (16, 4, 128, False), (16, 4, 5, True), (8, 4, 64, True),
+    (8, 8, 13, False), (16, 8, 256, False), (16, 16, 9, False), (32, 16, 33, True)])
+def test_fused_run_bit_exact_with_receivers(gpu, oracle, tile_j, waves, chunk_i, index64):
+    banner("fused run tile_j=%d waves=%d chunk_i=%d index64=%d" % (tile_j, waves, chunk_i, index64))
     s, P = make_pair(gpu, oracle, 45, 50, 90, sponge=10, nsamples=80,
                      xs=20, ys=22, zs=41)
     rec = np.array([[20, 22, 41], [0, 0, 0], [44, 49, 89], [10, 30, 64], [20, 22, 40]], np.int32)
-    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF, tile_j=tile_j, chunk_i=chunk_i, index64=index64)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF, tile_j=tile_j, block_waves=waves, chunk_i=chunk_i,
+                       index64=index64)
     sol.set_receivers(rec)
     src, rv = sol.run(0, 60)
     tr, orv = P.run(60, receivers=rec)
@@ -89,6 +91,24 @@ def test_fused_run_bit_exact_with_receivers(gpu, oracle, tile_j, chunk_i, index6
     for it in range(70, 75):
         sol.step(it)
     P.run(5)
+    assert np.array_equal(sol.current(), P.current())
+
+
+def test_autotune_keeps_state_and_parity(gpu, oracle, tmp_path):
+    banner("autotune")
+    s, P = make_pair(gpu, oracle, 48, 40, 100, sponge=10, nsamples=40)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF)
+    sol.run(0, 10)
+    before = sol.current().copy()
+    cache = str(tmp_path / "tune.txt")
+    cfg = sol.autotune(steps_per_trial=3, cache_path=cache)
+    assert np.array_equal(sol.current(), before)
+    assert cfg[0] % cfg[1] == 0 and cfg[2] >= 1
+    assert cfg == sol.autotune(steps_per_trial=3, cache_path=cache)  # cache hit
+    assert open(cache).read().startswith("fd 48 40 100 4 0 ")
+    src, _ = sol.run(10, 20)
+    tr, _ = P.run(30)
+    assert np.array_equal(src, tr[10:])
     assert np.array_equal(sol.current(), P.current())
 
 
