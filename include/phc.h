@@ -136,6 +136,69 @@ phc_status phc_fd_synchronize(phc_fd *h);
 phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char *cache_path, int *tile_j,
                            int *block_waves, int *chunk_i);
 
+/* ===================================================================== SEM
+ * Replaces SEMsolver (src/sem/solver/SEMsolver.hpp:18-91, SEMsolver.cpp:12-212)
+ * on the structured hexahedral box mesh of SEMmesh (src/sem/mesh/SEMmesh.cpp).
+ * Node numbering is the reference's: node = ix + iz*nx + iy*nx*nz
+ * (SEMmesh.cpp:214-223); element e = ex_i + ez_i*ex + ey_i*ex*ez.
+ */
+typedef struct phc_sem phc_sem;
+
+/* Step kernel (SEMsolver::computeOneStep, SEMsolver.cpp:21-125). */
+#define PHC_SEM_SEPARABLE 0 /* assembled operator as a radius-p stencil (axis-aligned box) */
+#define PHC_SEM_ELEMENT 1   /* element-by-element sum factorisation + atomic scatter */
+
+typedef struct {
+    int ex, ey, ez;        /* elements per axis (SEMproxy.cpp:12-14) */
+    float lx, ly, lz;      /* domain lengths (SEMproxy.cpp:16-18) */
+    int order;             /* GLL order p, 1..5 (SEMdata.hpp:14) */
+    const float *model;    /* wave speed per element (SEMmesh::getModel) */
+    float dt;              /* time step (SEMdata.hpp:15) */
+    const float *rhs;      /* source wavelet, nsamples values (SEMproxy.cpp:108-115) */
+    int nsamples;
+    int src_element;       /* element holding the source (SEMproxy.cpp:101-106) */
+    int kernel;            /* PHC_SEM_* */
+    int weights_ref;       /* 1: the reference's 8-decimal GLL weights; 0: exact */
+    int tile_j, block_waves, chunk_i; /* separable kernel tiling (0: default) */
+} phc_sem_params;
+
+phc_status phc_sem_create(const phc_sem_params *p, phc_sem **out);
+phc_status phc_sem_destroy(phc_sem *h);
+/* Number of nodes (nx*ny*nz) and of nodes per axis. */
+phc_status phc_sem_sizes(phc_sem *h, int *num_nodes, int *nx, int *ny, int *nz);
+/* SEMsolver::computeOneStep (SEMsolver.cpp:21-125): inject the source sample
+ * timeSample into level i2 at the first node of the source element, then
+ * level i1 <- 2 level i2 - level i1 - dt^2 (K level i2)/M on interior nodes. */
+phc_status phc_sem_compute_one_step(phc_sem *h, int time_sample, int i1, int i2);
+/* SEMproxy::run (SEMproxy.cpp:41-64) for steps it0..it0+nsteps-1, injection
+ * fused into the preceding step, captured in one hipGraph.  *i1 / *i2 are the
+ * reference's level indices and are swapped as the reference swaps them. */
+phc_status phc_sem_run(phc_sem *h, int it0, int nsteps, int *i1, int *i2);
+/* Receivers at node ids; traces of the last phc_sem_run (receiver-major) and
+ * the source-node trace (what SEMsolver::outputPnValues prints). */
+phc_status phc_sem_set_receivers(phc_sem *h, int nrec, const int *nodes);
+phc_status phc_sem_get_traces(phc_sem *h, float *src, float *rec);
+/* Level as the reference holds it: pnGlobal(:, level) in node order. */
+phc_status phc_sem_get_field(phc_sem *h, int level, float *host);
+phc_status phc_sem_set_field(phc_sem *h, int level, const float *host);
+/* Assembled lumped mass M (sum over elements of w w w |J| / c^2, FP64 then
+ * rounded) in node order. */
+phc_status phc_sem_get_mass(phc_sem *h, float *host);
+/* y = K p for a host vector p (node order), through the handle's kernel. */
+phc_status phc_sem_apply_stiffness(phc_sem *h, const float *p, float *y);
+phc_status phc_sem_check_finite(phc_sem *h, int level);
+phc_status phc_sem_set_timing(phc_sem *h, int enable);
+phc_status phc_sem_last_run_kernel_ms(phc_sem *h, double *ms_per_step);
+phc_status phc_sem_synchronize(phc_sem *h);
+/* Autotuner of the separable kernel (no reference counterpart): block shape
+ * and chunk length timed on scratch copies of the levels; optional text cache.
+ * A no-op returning zeros for the element kernel. */
+phc_status phc_sem_autotune(phc_sem *h, int steps_per_trial, const char *cache_path, int *tile_j,
+                            int *block_waves, int *chunk_i);
+/* SEMmesh::getElementNumberFromPoints (SEMmesh.cpp:266-285). */
+int phc_sem_element_from_point(int ex, int ey, int ez, float lx, float ly, float lz, float x,
+                               float y, float z);
+
 #ifdef __cplusplus
 }
 #endif

@@ -220,3 +220,126 @@ def fd_exact_coefs(L, dx):
            4: [F(-205, 72), F(8, 5), F(-1, 5), F(8, 315), F(-1, 560)],
            5: [F(-5269, 1800), F(5, 3), F(-5, 21), F(5, 126), F(-5, 1008), F(1, 3150)]}
     return np.array([float(c) for c in tab[L]], np.float64) / (dx * dx)
+
+
+# ----------------------------------------------------------------------- SEM
+def gll(p, ref_rounding=True):
+    """(points, weights, dphi[j, i] = l_j'(xi_i)) of order p."""
+    xi = np.zeros(p + 1)
+    w = np.zeros(p + 1)
+    d = np.zeros((p + 1) * (p + 1))
+    lib().orc_gll_points(p, xi)
+    lib().orc_gll_weights(p, w, 1 if ref_rounding else 0)
+    lib().orc_gll_deriv(p, xi, d)
+    return xi, w, d.reshape(p + 1, p + 1)
+
+
+class SemProblem:
+    """The reference SEM proxy (SEMproxy.cpp:10-119, SEMsolver.cpp:21-212)
+    restated: structured box mesh of ex^3 elements of order p, per-element
+    velocity, Ricker source in the element containing `src_point`, leapfrog
+    with Dirichlet boundary nodes.  `fp64=True` uses the FP64 oracle."""
+
+    def __init__(self, ex=10, lx=2000.0, order=2, dt=0.001, f0=10.0, model=None, nsamples=100,
+                 src_point=(1001.0, 1001.0, 1001.0), ey=None, ez=None, ly=None, lz=None,
+                 fp64=False, source_order=1):
+        ey = ex if ey is None else ey
+        ez = ex if ez is None else ez
+        ly = lx if ly is None else ly
+        lz = lx if lz is None else lz
+        m = SemMesh()
+        lib().orc_sem_mesh_init(ctypes.byref(m), ex, ey, ez, lx, ly, lz, order)
+        self.mesh = m
+        self.p = order
+        self.nE = ex * ey * ez
+        self.nN = m.nx * m.ny * m.nz
+        self.dt = np.float32(dt)
+        if model is None:
+            model = np.full(self.nE, 1500.0, np.float32)
+        self.model = np.ascontiguousarray(model, np.float32)
+        self.src_element = lib().orc_sem_element_from_point(ctypes.byref(m), *src_point)
+        self.rhs = np.zeros(nsamples, np.float32)
+        # SEM translation unit: std::exp(float)
+        lib().orc_source_term(nsamples, self.dt, f0, source_order, 0, self.rhs)
+        self.fp64 = fp64
+        if fp64:
+            self.h = lib().orc_sem64_create(ctypes.byref(m), self.model.astype(np.float64), float(self.dt))
+            self.p0 = np.zeros(self.nN)
+            self.p1 = np.zeros(self.nN)
+        else:
+            self.h = lib().orc_sem_create(ctypes.byref(m), self.model, self.dt)
+            self.p0 = np.zeros(self.nN, np.float32)
+            self.p1 = np.zeros(self.nN, np.float32)
+        self.nodes = np.zeros(self.nE * (order + 1) ** 3, np.int32)
+        lib().orc_sem_nodes_list(ctypes.byref(m), self.nodes)
+        self.nodes = self.nodes.reshape(self.nE, (order + 1) ** 3)
+        self.src_node = int(self.nodes[self.src_element, 0])
+        self.i1, self.i2 = 0, 1
+        self.it = 0
+
+    def __del__(self):
+        try:
+            if self.fp64:
+                lib().orc_sem64_destroy(self.h)
+            else:
+                lib().orc_sem_destroy(self.h)
+        except Exception:
+            pass
+
+    def level(self, i):
+        return self.p0 if i == 0 else self.p1
+
+    def run(self, nsteps, receivers=None):
+        """SEMproxy::run: computeOneStep(i1, i2) -> output(i1) -> swap.  Returns
+        the source-node trace (value after each step, outputPnValues) and
+        receiver traces (node ids)."""
+        tr = np.zeros(nsteps, self.p0.dtype)
+        rec = None if receivers is None else np.zeros((len(receivers), nsteps), self.p0.dtype)
+        for s in range(nsteps):
+            it = self.it + s
+            a, b = self.level(self.i1), self.level(self.i2)
+            if self.fp64:
+                lib().orc_sem64_step(self.h, a, b, self.src_element, float(self.rhs[it]))
+            else:
+                lib().orc_sem_step(self.h, a, b, self.src_element, self.rhs[it])
+            tr[s] = a[self.src_node]
+            if rec is not None:
+                rec[:, s] = a[np.asarray(receivers)]
+            self.i1, self.i2 = self.i2, self.i1
+        self.it += nsteps
+        return tr, rec
+
+    def current(self):
+        return self.level(self.i2)
+
+    def previous(self):
+        return self.level(self.i1)
+
+    def apply_k(self, p):
+        if self.fp64:
+            y = np.zeros(self.nN)
+            M = np.zeros(self.nN)
+            lib().orc_sem64_apply_k(self.h, np.ascontiguousarray(p, np.float64), y, M)
+        else:
+            y = np.zeros(self.nN, np.float32)
+            M = np.zeros(self.nN, np.float32)
+            lib().orc_sem_apply_k(self.h, np.ascontiguousarray(p, np.float32), y, M)
+        return y, M
+
+    def node_coords(self):
+        """Physical coordinates of every node (x, y, z), node id = ix + iz nx + iy nx nz."""
+        m = self.mesh
+        xi = gll(self.p)[0]
+        def axis(ne, h):
+            c = np.zeros(ne * self.p + 1)
+            for e in range(ne):
+                c[e * self.p:(e + 1) * self.p + 1] = e * h + (xi + 1) * h / 2
+            return c
+        cx = axis(m.ex, m.lx / m.ex)
+        cy = axis(m.ey, m.ly / m.ey)
+        cz = axis(m.ez, m.lz / m.ez)
+        n = np.arange(self.nN)
+        ix = n % m.nx
+        iz = (n // m.nx) % m.nz
+        iy = n // (m.nx * m.nz)
+        return cx[ix], cy[iy], cz[iz]

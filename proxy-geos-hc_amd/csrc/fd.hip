@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "fd_kernels.hpp"
+#include "march.hpp"
 #include "phc_common.hpp"
 
 namespace phc {
@@ -45,54 +46,12 @@ __device__ __forceinline__ float fd_update_ref(float pc, float pm, float v, doub
     return r * s;
 }
 
-// Workgroup barrier that orders LDS only: global loads issued before it stay
-// in flight (a plain __syncthreads() also waits for them).
-__device__ __forceinline__ void lds_barrier()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // Face damping of one axis at index i of n (FDTDinit.hpp:343-395: the high
 // face loop runs last, so it wins where the faces overlap).
 __device__ __forceinline__ float fd_face_damping(const FdStepArgs &a, int i, int n)
 {
     const int d = (i >= n - a.sw) ? a.sw - (n - i) : ((i < a.sw) ? a.sw - i : -1);
     return d >= 0 ? a.stab[d] : 1.f;
-}
-
-__device__ __forceinline__ float mul_nc(float a, float b)
-{
-#pragma clang fp contract(off)
-    return a * b;
-}
-
-__device__ __forceinline__ float add_nc(float a, float b)
-{
-#pragma clang fp contract(off)
-    return a + b;
-}
-
-// Load through a kernel-argument base pointer.  With 32-bit offsets the
-// offset is in bytes, so the access is base (scalar) + 32-bit vector offset.
-template <typename IT>
-__device__ __forceinline__ float ldo(const float *base, IT off)
-{
-    if constexpr (sizeof(IT) == 4) return *(const float *)((const char *)base + off);
-    else return base[off];
-}
-template <typename IT>
-__device__ __forceinline__ float ldo_nt(const float *base, IT off)
-{
-    if constexpr (sizeof(IT) == 4) return __builtin_nontemporal_load((const float *)((const char *)base + off));
-    else return __builtin_nontemporal_load(base + off);
-}
-template <typename IT>
-__device__ __forceinline__ void sto_nt(float *base, IT off, float v)
-{
-    if constexpr (sizeof(IT) == 4) __builtin_nontemporal_store(v, (float *)((char *)base + off));
-    else __builtin_nontemporal_store(v, base + off);
 }
 
 // One launch = inner3D + applySponge over the grid.
@@ -124,12 +83,7 @@ fd_step_kernel(FdStepArgs a)
 
     const FdLayout &L = a.L;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    int b = blockIdx.x;
-    const int nb = gridDim.x;
-    // XCD-aware remap: hardware deals consecutive block ids round-robin over
-    // the 8 XCDs; give each XCD a contiguous range of tiles so halo re-reads
-    // hit its own L2.
-    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    const int b = xcd_block(blockIdx.x, gridDim.x);
     const int kt = b % a.tiles_k;
     const int rest = b / a.tiles_k;
     const int jt = rest % a.tiles_j;

@@ -295,3 +295,163 @@ class FdSolver:
         check(lib().phc_fd_autotune(self.h, steps_per_trial, cp, ctypes.byref(tj), ctypes.byref(ty),
                                     ctypes.byref(ci)))
         return tj.value, ty.value, ci.value
+
+
+# ----------------------------------------------------------------------- SEM
+SEM_SEPARABLE = 0
+SEM_ELEMENT = 1
+
+
+class SemParams(ctypes.Structure):
+    _fields_ = [("ex", c_int), ("ey", c_int), ("ez", c_int),
+                ("lx", c_float), ("ly", c_float), ("lz", c_float),
+                ("order", c_int), ("model", c_void_p), ("dt", c_float),
+                ("rhs", c_void_p), ("nsamples", c_int), ("src_element", c_int),
+                ("kernel", c_int), ("weights_ref", c_int),
+                ("tile_j", c_int), ("block_waves", c_int), ("chunk_i", c_int)]
+
+
+SIGNATURES.update({
+    "phc_sem_create": (c_int, [ctypes.POINTER(SemParams), ctypes.POINTER(c_void_p)]),
+    "phc_sem_destroy": (c_int, [c_void_p]),
+    "phc_sem_sizes": (c_int, [c_void_p] + [ctypes.POINTER(c_int)] * 4),
+    "phc_sem_compute_one_step": (c_int, [c_void_p, c_int, c_int, c_int]),
+    "phc_sem_run": (c_int, [c_void_p, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
+    "phc_sem_set_receivers": (c_int, [c_void_p, c_int, c_void_p]),
+    "phc_sem_get_traces": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "phc_sem_get_field": (c_int, [c_void_p, c_int, f32p]),
+    "phc_sem_set_field": (c_int, [c_void_p, c_int, f32p]),
+    "phc_sem_get_mass": (c_int, [c_void_p, f32p]),
+    "phc_sem_apply_stiffness": (c_int, [c_void_p, f32p, f32p]),
+    "phc_sem_check_finite": (c_int, [c_void_p, c_int]),
+    "phc_sem_set_timing": (c_int, [c_void_p, c_int]),
+    "phc_sem_last_run_kernel_ms": (c_int, [c_void_p, ctypes.POINTER(c_double)]),
+    "phc_sem_synchronize": (c_int, [c_void_p]),
+    "phc_sem_autotune": (c_int, [c_void_p, c_int, ctypes.c_char_p, ctypes.POINTER(c_int),
+                                 ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
+    "phc_sem_element_from_point": (c_int, [c_int, c_int, c_int, c_float, c_float, c_float,
+                                           c_float, c_float, c_float]),
+})
+
+
+class SemSolver:
+    """SEMsolver (src/sem/solver/SEMsolver.hpp:18-91) on the device: the
+    structured box of SEMmesh, per-element wave speed, Ricker source in the
+    element containing `src_point`, levels i1/i2 as in SEMproxy."""
+
+    def __init__(self, ex=10, lx=2000.0, order=2, dt=0.001, f0=10.0, model=None, nsamples=100,
+                 src_point=(1001.0, 1001.0, 1001.0), ey=None, ez=None, ly=None, lz=None,
+                 kernel=SEM_SEPARABLE, weights_ref=True, source_order=1, rhs=None,
+                 tile_j=0, block_waves=0, chunk_i=0):
+        ey = ex if ey is None else ey
+        ez = ex if ez is None else ez
+        ly = lx if ly is None else ly
+        lz = lx if lz is None else lz
+        nE = ex * ey * ez
+        self.model = (np.full(nE, 1500.0, np.float32) if model is None
+                      else np.ascontiguousarray(model, np.float32))
+        self.dt = np.float32(dt)
+        self.rhs = (source_term(nsamples, self.dt, f0, source_order, EXP_FLOAT) if rhs is None
+                    else np.ascontiguousarray(rhs, np.float32))
+        self.src_element = lib().phc_sem_element_from_point(ex, ey, ez, lx, ly, lz, *src_point)
+        p = SemParams()
+        p.ex, p.ey, p.ez = ex, ey, ez
+        p.lx, p.ly, p.lz = lx, ly, lz
+        p.order = order
+        p.model = _ptr(self.model)
+        p.dt = self.dt
+        p.rhs = _ptr(self.rhs)
+        p.nsamples = len(self.rhs)
+        p.src_element = self.src_element
+        p.kernel = kernel
+        p.weights_ref = 1 if weights_ref else 0
+        p.tile_j, p.block_waves, p.chunk_i = tile_j, block_waves, chunk_i
+        h = c_void_p()
+        check(lib().phc_sem_create(ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        nn, nx, ny, nz = c_int(), c_int(), c_int(), c_int()
+        check(lib().phc_sem_sizes(h, ctypes.byref(nn), ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz)))
+        self.nn, self.nx, self.ny, self.nz = nn.value, nx.value, ny.value, nz.value
+        self.i1, self.i2 = 0, 1
+        self.nrec = 0
+        self.it = 0
+
+    def close(self):
+        if self.h:
+            check(lib().phc_sem_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def compute_one_step(self, time_sample, i1, i2):
+        check(lib().phc_sem_compute_one_step(self.h, time_sample, i1, i2))
+
+    def step(self):
+        """One iteration of SEMproxy::run (SEMproxy.cpp:45-55)."""
+        self.compute_one_step(self.it, self.i1, self.i2)
+        self.i1, self.i2 = self.i2, self.i1
+        self.it += 1
+
+    def run(self, nsteps):
+        i1, i2 = c_int(self.i1), c_int(self.i2)
+        check(lib().phc_sem_run(self.h, self.it, nsteps, ctypes.byref(i1), ctypes.byref(i2)))
+        self.i1, self.i2 = i1.value, i2.value
+        self.it += nsteps
+        src = np.zeros(nsteps, np.float32)
+        rec = np.zeros(max(self.nrec, 1) * nsteps, np.float32)
+        check(lib().phc_sem_get_traces(self.h, _ptr(src), _ptr(rec) if self.nrec else None))
+        return src, (rec.reshape(self.nrec, nsteps) if self.nrec else None)
+
+    def set_receivers(self, nodes):
+        a = np.ascontiguousarray(nodes, np.int32)
+        self.nrec = len(a)
+        check(lib().phc_sem_set_receivers(self.h, self.nrec, _ptr(a)))
+
+    def field(self, level):
+        out = np.zeros(self.nn, np.float32)
+        check(lib().phc_sem_get_field(self.h, level, out))
+        return out
+
+    def set_field(self, level, arr):
+        check(lib().phc_sem_set_field(self.h, level, np.ascontiguousarray(arr, np.float32)))
+
+    def current(self):
+        return self.field(self.i2)
+
+    def previous(self):
+        return self.field(self.i1)
+
+    def mass(self):
+        out = np.zeros(self.nn, np.float32)
+        check(lib().phc_sem_get_mass(self.h, out))
+        return out
+
+    def apply_stiffness(self, p):
+        y = np.zeros(self.nn, np.float32)
+        check(lib().phc_sem_apply_stiffness(self.h, np.ascontiguousarray(p, np.float32), y))
+        return y
+
+    def set_timing(self, on):
+        check(lib().phc_sem_set_timing(self.h, 1 if on else 0))
+
+    def last_kernel_ms(self):
+        v = c_double()
+        check(lib().phc_sem_last_run_kernel_ms(self.h, ctypes.byref(v)))
+        return v.value
+
+    def synchronize(self):
+        check(lib().phc_sem_synchronize(self.h))
+
+    def check_finite(self, level):
+        return lib().phc_sem_check_finite(self.h, level)
+
+    def autotune(self, steps_per_trial=10, cache_path=None):
+        tj, ty, ci = c_int(), c_int(), c_int()
+        cp = None if cache_path is None else cache_path.encode()
+        check(lib().phc_sem_autotune(self.h, steps_per_trial, cp, ctypes.byref(tj), ctypes.byref(ty),
+                                     ctypes.byref(ci)))
+        return tj.value, ty.value, ci.value
