@@ -43,6 +43,15 @@ phc_status phc_fd_model_two_layer(int nx, int ny, int nz, float vmin, float dt, 
 phc_status phc_fd_model_layered(int nx, int ny, int nz, int nlayers, float vtop, float vbottom,
                                 uint64_t seed, float dt, int *interfaces, float *vp);
 
+/* Seeded smooth random model per SEM element (no reference counterpart; the
+ * model of the order-4 precision study): standard-normal white noise from
+ * splitmix64(seed) (Box-Muller) on the ex*ey*ez element grid, smoothed by a
+ * separable Gaussian of std sigma elements (truncated at 3 sigma, clamped
+ * edges), min-max scaled to [vmin, vmax].  Element order e = ix + iz*ex +
+ * iy*ex*ez. */
+phc_status phc_sem_model_gaussian(int ex, int ey, int ez, double sigma, float vmin, float vmax,
+                                  uint64_t seed, float *model);
+
 #ifdef __cplusplus
 }
 #endif

@@ -307,6 +307,11 @@ __global__ void sem_inject_kernel(float *p, int64_t off, float coef, const float
     if (threadIdx.x == 0 && blockIdx.x == 0) p[off] = add_nc(p[off], mul_nc(coef, rhs[it]));
 }
 
+__global__ void sem_copy_point_kernel(const float *p, int64_t off, float *dst)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = p[off];
+}
+
 __global__ void sem_sample_kernel(const float *p, const int64_t *offs, int nrec, float *out,
                                   int stride, int step, int64_t src_off, const float *src_trace)
 {
@@ -373,6 +378,7 @@ struct phc_sem {
     int nsamples;
     std::vector<float> hrhs, model;
     int src_element, src_node;
+    bool src_interior = true;  // boundary nodes are never updated: no fused injection
     float inj_coef;
     hipStream_t stream = nullptr;
     // separable path
@@ -573,6 +579,8 @@ phc_status phc_sem_create(const phc_sem_params *p, phc_sem **out)
         h->src_node = exi * h->p + (ezi * h->p) * h->nx + (eyi * h->p) * h->nx * h->nz;
         const float c = h->model[e];
         h->inj_coef = ((h->dt * h->dt) * c) * c;  // SEMsolver.cpp:41-42
+        const int sx = exi * h->p, sz = ezi * h->p, sy = eyi * h->p;
+        h->src_interior = sx >= 1 && sx < h->nx - 1 && sz >= 1 && sz < h->nz - 1 && sy >= 1 && sy < h->ny - 1;
     }
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(PHC_ERR_HIP, "hipStreamCreate failed"));
@@ -716,9 +724,20 @@ static phc_status sem_enqueue_run(phc_sem *h, int it0, int nsteps, int i1, int i
     for (int s = 0; s < nsteps; ++s) {
         const int it = it0 + s;
         const bool more = s + 1 < nsteps;
-        PHC_TRY(sem_launch(h, i1, i2, more ? 1 : 0, more ? h->hrhs[it + 1] : 0.f,
-                           h->trace.as<float>() + s, ev ? (*ev)[2 * s] : nullptr,
-                           ev ? (*ev)[2 * s + 1] : nullptr));
+        if (h->src_interior) {
+            PHC_TRY(sem_launch(h, i1, i2, more ? 1 : 0, more ? h->hrhs[it + 1] : 0.f,
+                               h->trace.as<float>() + s, ev ? (*ev)[2 * s] : nullptr,
+                               ev ? (*ev)[2 * s + 1] : nullptr));
+        } else {
+            // source on a (never updated) boundary node: inject in place each
+            // step, as SEMsolver.cpp:39-43 does, and read the node back
+            if (s > 0) PHC_TRY(sem_inject(h, i2, it));
+            PHC_TRY(sem_launch(h, i1, i2, 0, 0.f, nullptr, ev ? (*ev)[2 * s] : nullptr,
+                               ev ? (*ev)[2 * s + 1] : nullptr));
+            hipLaunchKernelGGL(sem_copy_point_kernel, dim3(1), dim3(64), 0, h->stream, h->lev[i1].as<float>(),
+                               sem_off(h, h->src_node), h->trace.as<float>() + s);
+            PHC_HIP(hipGetLastError());
+        }
         if (h->nrec > 0) {
             hipLaunchKernelGGL(sem_sample_kernel, dim3((h->nrec + 255) / 256), dim3(256), 0, h->stream,
                                h->lev[i1].as<float>(), h->recoff.as<int64_t>(), h->nrec,

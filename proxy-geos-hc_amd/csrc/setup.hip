@@ -9,6 +9,7 @@
 //   init_models      src/fd/utilsFDTD/FDTDinit.hpp:249-317
 #pragma clang fp contract(off)
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -159,6 +160,52 @@ phc_status phc_fd_model_layered(int nx, int ny, int nz, int nlayers, float vtop,
     for (int64_t i = 0; i < nx; ++i)
         for (int64_t j = 0; j < ny; ++j)
             for (int64_t k = 0; k < nz; ++k) vp[(i * ny + j) * nz + k] = v2[k];
+    return PHC_OK;
+}
+
+phc_status phc_sem_model_gaussian(int ex, int ey, int ez, double sigma, float vmin, float vmax,
+                                  uint64_t seed, float *model)
+{
+    if (!model || ex < 1 || ey < 1 || ez < 1 || !(sigma >= 0) || !(vmax >= vmin))
+        return phc::fail(PHC_ERR_INVALID, "bad Gaussian-model arguments");
+    const int64_t n = (int64_t)ex * ey * ez;
+    std::vector<double> a(n), b(n);
+    uint64_t s = seed;
+    for (int64_t i = 0; i < n; i += 2) {
+        // Box-Muller on two uniforms in (0, 1]
+        const double u1 = ((splitmix64(s) >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+        const double u2 = ((splitmix64(s) >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+        const double r = std::sqrt(-2.0 * std::log(u1));
+        a[i] = r * std::cos(2.0 * M_PI * u2);
+        if (i + 1 < n) a[i + 1] = r * std::sin(2.0 * M_PI * u2);
+    }
+    if (sigma > 0) {
+        const int rad = (int)std::ceil(3.0 * sigma);
+        std::vector<double> k(2 * rad + 1);
+        double ks = 0;
+        for (int d = -rad; d <= rad; ++d) ks += (k[d + rad] = std::exp(-0.5 * d * d / (sigma * sigma)));
+        for (double &v : k) v /= ks;
+        const int dims[3] = {ex, ez, ey};
+        const int64_t strides[3] = {1, ex, (int64_t)ex * ez};
+        for (int ax = 0; ax < 3; ++ax) {
+            const int m = dims[ax];
+            const int64_t st = strides[ax];
+            for (int64_t i = 0; i < n; ++i) {
+                const int c = (int)((i / st) % m);
+                double acc = 0;
+                for (int d = -rad; d <= rad; ++d) {
+                    const int cc = std::min(m - 1, std::max(0, c + d));
+                    acc += k[d + rad] * a[i + (int64_t)(cc - c) * st];
+                }
+                b[i] = acc;
+            }
+            std::swap(a, b);
+        }
+    }
+    double lo = a[0], hi = a[0];
+    for (double v : a) { lo = std::min(lo, v); hi = std::max(hi, v); }
+    for (int64_t i = 0; i < n; ++i)
+        model[i] = (hi > lo) ? (float)(vmin + (vmax - vmin) * (a[i] - lo) / (hi - lo)) : vmin;
     return PHC_OK;
 }
 

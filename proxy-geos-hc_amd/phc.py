@@ -65,6 +65,8 @@ SIGNATURES = {
     "phc_fd_model_two_layer": (c_int, [c_int, c_int, c_int, c_float, c_float, f32p]),
     "phc_fd_model_layered": (c_int, [c_int, c_int, c_int, c_int, c_float, c_float,
                                      ctypes.c_uint64, c_float, c_void_p, f32p]),
+    "phc_sem_model_gaussian": (c_int, [c_int, c_int, c_int, c_double, c_float, c_float,
+                                       ctypes.c_uint64, f32p]),
     # FD
     "phc_fd_create": (c_int, [ctypes.POINTER(FdParams), ctypes.POINTER(c_void_p)]),
     "phc_fd_destroy": (c_int, [c_void_p]),
@@ -332,6 +334,12 @@ SIGNATURES.update({
     "phc_sem_element_from_point": (c_int, [c_int, c_int, c_int, c_float, c_float, c_float,
                                            c_float, c_float, c_float]),
 })
+
+
+def sem_model_gaussian(ex, ey, ez, sigma=5.0, vmin=2000.0, vmax=4000.0, seed=1):
+    out = np.zeros(ex * ey * ez, np.float32)
+    check(lib().phc_sem_model_gaussian(ex, ey, ez, sigma, vmin, vmax, seed, out))
+    return out
 
 
 class SemSolver:

@@ -184,3 +184,22 @@ def test_autotune_keeps_state(gpu, oracle, tmp_path):
     tr, _ = O.run(50)
     assert rel_l2(s.current(), O.current()) < 1e-5
     assert rel_l2(src, tr[20:]) < 1e-5
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_source_on_boundary_node(gpu, oracle, kernel):
+    """The reference's source point (1001, 1001, 1001) outside a small domain
+    selects element 0, whose first node is a Dirichlet boundary node: the
+    injection then accumulates in place (SEMsolver.cpp:39-43)."""
+    kw = dict(ex=4, lx=160.0, order=2, dt=0.002, f0=20.0, nsamples=60, src_point=(1001.0, 1001.0, 1001.0))
+    s = gpu.SemSolver(kernel=kernel, **kw)
+    O = oracle.SemProblem(**kw)
+    assert O.src_element == 0 and O.src_node == 0
+    a = gpu.SemSolver(kernel=kernel, **kw)
+    src, _ = s.run(50)
+    for _ in range(50):
+        a.step()
+    tr, _ = O.run(50)
+    assert rel_l2(src, tr) < 1e-5
+    assert rel_l2(s.current(), O.current()) < 1e-5
+    assert rel_l2(a.current(), O.current()) < 1e-5
