@@ -199,6 +199,60 @@ phc_status phc_sem_autotune(phc_sem *h, int steps_per_trial, const char *cache_p
 int phc_sem_element_from_point(int ex, int ey, int ez, float lx, float ly, float lz, float x,
                                float y, float z);
 
+/* ================================================================= BATCH
+ * Many FD shots advanced together: one persistent launch per time step does
+ * the FD step of every shot.  Each shot is the FD problem of the reference's
+ * driver (src/fd/FDTDmain.cpp:9-108: FDTDKernel::addRHS + computeOneStepSB per
+ * sample, FDTDkernels.hpp:75-128) on its own grid size, source position,
+ * vp^2 dt^2 model and source samples; the stencil coefficients, sponge and
+ * number of samples are shared.  The reference runs one shot per process;
+ * these calls replace a loop of such processes.  Half width 4 only.
+ */
+typedef struct phc_batch phc_batch;
+
+typedef struct {
+    int nshots;
+    const int *dims;          /* nshots x (nx, ny, nz) */
+    const int *src;           /* nshots x (xs, ys, zs) */
+    const float *const *vp;   /* per shot: nx*ny*nz values of vp^2 dt^2, IDX3 order */
+    const float *const *rhs;  /* per shot: nsamples source samples */
+    int nsamples;
+    int half_width;           /* must be 4 */
+    const float *coef;        /* half_width+1 stencil coefficients, all three axes */
+    double coef0;             /* centre coefficient (FDTDinit.hpp:204-208) */
+    int sponge_width;         /* 0..64 */
+    float sponge_alpha;
+    float init_value;         /* initial value of both levels */
+    int arith;                /* PHC_FD_ARITH_* */
+    int tile_j, block_waves;  /* 0 = default (16 x 16) */
+    int chunk_i;              /* 0 = default (64) */
+    int blocks_per_cu;        /* 0 = default (1) */
+} phc_batch_params;
+
+phc_status phc_batch_create(const phc_batch_params *p, phc_batch **out);
+phc_status phc_batch_destroy(phc_batch *h);
+/* nrec receivers per shot at grid points ijk[(shot*nrec + r)*3 + 0..2]. */
+phc_status phc_batch_set_receivers(phc_batch *h, int nrec, const int *ijk);
+/* Samples it0 .. it0+nsteps-1 of every shot (FDTDmain.cpp:55-81 per shot),
+ * captured in one hipGraph.  Shots continue from where the last run ended. */
+phc_status phc_batch_run(phc_batch *h, int it0, int nsteps);
+/* Traces of the last run: src[shot*nsteps + s] = new value at the shot's
+ * source after step s (before the next sample is injected, FDTDutils.hpp:
+ * 193-203); rec[(shot*nrec + r)*nsteps + s] likewise at each receiver.
+ * Either pointer may be NULL. */
+phc_status phc_batch_get_traces(phc_batch *h, float *src, float *rec);
+/* One shot's current (current != 0) or previous level, halo included, in the
+ * host layout of phc_fd_get_field. */
+phc_status phc_batch_get_field(phc_batch *h, int shot, int current, float *host);
+phc_status phc_batch_set_timing(phc_batch *h, int enable);
+/* Mean duration in ms of one batched step launch of the last timed run. */
+phc_status phc_batch_last_run_kernel_ms(phc_batch *h, double *ms_per_step);
+phc_status phc_batch_synchronize(phc_batch *h);
+/* Autotuner (no reference counterpart): block shape, chunk length and
+ * resident blocks per CU timed on a copy of the shots (state not advanced). */
+phc_status phc_batch_autotune(phc_batch *h, int steps_per_trial, int *tile_j, int *block_waves,
+                              int *chunk_i, int *blocks_per_cu);
+
 #ifdef __cplusplus
 }
 #endif

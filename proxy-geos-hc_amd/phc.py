@@ -299,6 +299,145 @@ class FdSolver:
         return tj.value, ty.value, ci.value
 
 
+# ------------------------------------------------------------- batched shots
+class BatchParams(ctypes.Structure):
+    _fields_ = [("nshots", c_int), ("dims", c_void_p), ("src", c_void_p),
+                ("vp", c_void_p), ("rhs", c_void_p), ("nsamples", c_int),
+                ("half_width", c_int), ("coef", c_void_p), ("coef0", c_double),
+                ("sponge_width", c_int), ("sponge_alpha", c_float), ("init_value", c_float),
+                ("arith", c_int), ("tile_j", c_int), ("block_waves", c_int),
+                ("chunk_i", c_int), ("blocks_per_cu", c_int)]
+
+
+SIGNATURES.update({
+    "phc_batch_create": (c_int, [ctypes.POINTER(BatchParams), ctypes.POINTER(c_void_p)]),
+    "phc_batch_destroy": (c_int, [c_void_p]),
+    "phc_batch_set_receivers": (c_int, [c_void_p, c_int, c_void_p]),
+    "phc_batch_run": (c_int, [c_void_p, c_int, c_int]),
+    "phc_batch_get_traces": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "phc_batch_get_field": (c_int, [c_void_p, c_int, c_int, f32p]),
+    "phc_batch_set_timing": (c_int, [c_void_p, c_int]),
+    "phc_batch_last_run_kernel_ms": (c_int, [c_void_p, ctypes.POINTER(c_double)]),
+    "phc_batch_synchronize": (c_int, [c_void_p]),
+    "phc_batch_autotune": (c_int, [c_void_p, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                                   ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
+})
+
+
+def multi_shot_setups(nshots=64, nmin=128, nmax=384, f0_range=(5.0, 15.0), nsteps=800, seed=2024,
+                      nlayers=6, sponge_width=20, cfl=0.4, dx=10.0):
+    """The batched survey: per shot, each axis drawn uniformly from [nmin, nmax],
+    a centred source, a layered vp model of its own seed and a Ricker f0 drawn
+    from f0_range; shared h, dt (from vmax and the CFL number) and step count."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(nshots):
+        nx, ny, nz = (int(v) for v in rng.integers(nmin, nmax + 1, size=3))
+        f0 = float(rng.uniform(*f0_range))
+        out.append(FdSetup(nx, ny, nz, L=4, dx=dx, f0=f0, cfl=cfl, sponge_width=sponge_width,
+                           model=("layered", nlayers, seed * 1000 + s), nsamples=nsteps))
+    return out
+
+
+class FdBatch:
+    """Many FD shots (FdSetup each, same half width, coefficients, sponge and
+    sample count) advanced together by the batched persistent kernel."""
+
+    def __init__(self, setups, arith=ARITH_FP32, init_value=1e-6, tile_j=0, block_waves=0, chunk_i=0,
+                 blocks_per_cu=0):
+        s0 = setups[0]
+        for s in setups:
+            if (s.L != s0.L or s.nsamples != s0.nsamples or s.sponge_width != s0.sponge_width
+                    or not np.array_equal(s.cx, s0.cx) or not np.array_equal(s.cy, s0.cx)
+                    or not np.array_equal(s.cz, s0.cx)):
+                raise ValueError("batched shots must share half width, coefficients, sponge and nsamples")
+        self.setups = setups
+        self.n = len(setups)
+        self.dims = np.array([[s.nx, s.ny, s.nz] for s in setups], np.int32)
+        src = np.array([[s.xs, s.ys, s.zs] for s in setups], np.int32)
+        vps = (c_void_p * self.n)(*[_ptr(s.vp) for s in setups])
+        rhs = (c_void_p * self.n)(*[_ptr(s.rhs) for s in setups])
+        p = BatchParams()
+        p.nshots = self.n
+        p.dims = _ptr(self.dims)
+        p.src = _ptr(src)
+        p.vp = ctypes.cast(vps, c_void_p)
+        p.rhs = ctypes.cast(rhs, c_void_p)
+        p.nsamples = s0.nsamples
+        p.half_width = s0.L
+        p.coef = _ptr(s0.cx)
+        p.coef0 = s0.coef0
+        p.sponge_width = s0.sponge_width
+        p.sponge_alpha = s0.sponge_alpha
+        p.init_value = init_value
+        p.arith = arith
+        p.tile_j, p.block_waves, p.chunk_i, p.blocks_per_cu = tile_j, block_waves, chunk_i, blocks_per_cu
+        h = c_void_p()
+        check(lib().phc_batch_create(ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        self.it = 0
+        self.nrec = 0
+
+    def close(self):
+        if self.h:
+            check(lib().phc_batch_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_receivers(self, ijk):
+        """ijk: (nshots, nrec, 3) grid points."""
+        a = np.ascontiguousarray(ijk, np.int32)
+        self.nrec = a.shape[1]
+        check(lib().phc_batch_set_receivers(self.h, self.nrec, _ptr(a.reshape(-1))))
+
+    def run(self, nsteps, traces=True, it0=None):
+        """Advance every shot by nsteps samples (from sample it0, default: where
+        the last run ended); returns (source traces (nshots, nsteps), receiver
+        traces (nshots, nrec, nsteps) or None)."""
+        it0 = self.it if it0 is None else it0
+        check(lib().phc_batch_run(self.h, it0, nsteps))
+        self.it = it0 + nsteps
+        if not traces:
+            return None, None
+        src = np.zeros(self.n * nsteps, np.float32)
+        rec = np.zeros(max(self.nrec, 1) * self.n * nsteps, np.float32)
+        check(lib().phc_batch_get_traces(self.h, _ptr(src), _ptr(rec) if self.nrec else None))
+        return (src.reshape(self.n, nsteps),
+                rec.reshape(self.n, self.nrec, nsteps) if self.nrec else None)
+
+    def field(self, shot, current=True):
+        s = self.setups[shot]
+        L = s.L
+        out = np.zeros((s.nx + 2 * L) * (s.ny + 2 * L) * (s.nz + 2 * L), np.float32)
+        check(lib().phc_batch_get_field(self.h, shot, 1 if current else 0, out))
+        return out.reshape(s.nx + 2 * L, s.ny + 2 * L, s.nz + 2 * L)
+
+    def set_timing(self, on):
+        check(lib().phc_batch_set_timing(self.h, 1 if on else 0))
+
+    def last_kernel_ms(self):
+        v = c_double()
+        check(lib().phc_batch_last_run_kernel_ms(self.h, ctypes.byref(v)))
+        return v.value
+
+    def synchronize(self):
+        check(lib().phc_batch_synchronize(self.h))
+
+    def autotune(self, steps_per_trial=3):
+        """Returns (tile_j, block_waves, chunk_i, blocks_per_cu) of the fastest trial."""
+        r = [c_int() for _ in range(4)]
+        check(lib().phc_batch_autotune(self.h, steps_per_trial, *[ctypes.byref(x) for x in r]))
+        return tuple(x.value for x in r)
+
+    def points(self):
+        return int(np.prod(self.dims, axis=1, dtype=np.int64).sum())
+
+
 # ----------------------------------------------------------------------- SEM
 SEM_SEPARABLE = 0
 SEM_ELEMENT = 1
