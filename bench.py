@@ -10,7 +10,11 @@ on the solver's stream; value = interior points x K / wall time.  The
 roofline block uses the mean stencil-kernel duration from those events and
 16 algorithmic bytes per point update (read pn, pnm1, vp^2dt^2; write pnp1).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Beside the headline line, "batch" reports the batched multi-shot workload
+(64 uneven shots, 800 steps, one persistent launch per step) and the same shots
+run one at a time.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
 """
 import argparse
 import json
@@ -31,10 +35,13 @@ BYTES_PER_UPDATE = 16
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fd_step_r4_512_traffic.txt")
 
 
-def read_traffic():
-    if not os.path.exists(TRAFFIC_FILE):
+BATCH_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fd_batch_64shots_traffic.txt")
+
+
+def read_traffic(path=TRAFFIC_FILE):
+    if not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_FILE):
+    for line in open(path):
         if line.startswith("bytes_per_launch:"):
             return int(float(line.split(":")[1]))
     return None
@@ -58,6 +65,68 @@ def cpu_baseline(setup, nsteps=3):
             "sample": "%d steps of the same 512^3 layered workload (oracle/fd_oracle.c, OpenMP)" % nsteps}
 
 
+def bench_batch(phc, nsteps=800, nshots=64, baselines=True, shape=None):
+    """Batched multi-shot FD: 64 shots, each axis drawn from U[128, 384], per-shot
+    6-layer vp and Ricker f0 in U[5, 15] Hz, 20-point sponge, dt from CFL 0.4,
+    nsteps steps; all shots advanced by one persistent launch per step, the run
+    captured in one hipGraph.  Baselines: the same shots run one after the
+    other through the single-grid path (graph-captured each), with its default
+    launch shape and with a shape autotuned per shot."""
+    setups = phc.multi_shot_setups(nshots=nshots, nsteps=nsteps, seed=2024)
+    pts = int(sum(s.nx * s.ny * s.nz for s in setups))
+    if shape:
+        b = phc.FdBatch(setups, arith=phc.ARITH_FP32, tile_j=shape[0], block_waves=shape[1],
+                        chunk_i=shape[2], blocks_per_cu=shape[3])
+        cfg = tuple(shape)
+    else:
+        b = phc.FdBatch(setups, arith=phc.ARITH_FP32)
+        cfg = b.autotune(steps_per_trial=3)
+    b.run(nsteps, it0=0, traces=False)  # capture + warm
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(nsteps, it0=0, traces=False)  # replay of the captured run
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    b.set_timing(True)
+    b.run(min(nsteps, 100), it0=0, traces=False)
+    kernel_ms = b.last_kernel_ms()
+    b.close()
+    out = {"workload": "%d shots, axes U[128,384], 6-layer vp, f0 U[5,15] Hz, 20-pt sponge, "
+                       "%d steps, FP32" % (nshots, nsteps),
+           "points": pts, "steps": nsteps, "wall_s": round(wall, 4),
+           "value": round(pts * nsteps / wall / 1e6, 1), "unit": "Mpoint-updates/s",
+           "kernel_ms_per_step": round(kernel_ms, 4),
+           "tile_j": cfg[0], "block_waves": cfg[1], "chunk_i": cfg[2], "blocks_per_cu": cfg[3]}
+    ach = BYTES_PER_UPDATE * pts / (kernel_ms * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": read_traffic(BATCH_TRAFFIC_FILE)}
+    if baselines:
+        t_def = t_tuned = 0.0
+        for s in setups:
+            for tune in (False, True):
+                sol = phc.FdSolver(s, arith=phc.ARITH_FP32)
+                if tune:
+                    sol.autotune(steps_per_trial=3)
+                sol.run(0, nsteps)  # capture + warm
+                sol.synchronize()
+                t0 = time.perf_counter()
+                sol.run(0, nsteps)
+                sol.synchronize()
+                dt = time.perf_counter() - t0
+                sol.close()
+                if tune:
+                    t_tuned += dt
+                else:
+                    t_def += dt
+        out["one_at_a_time_default_shape"] = {"wall_s": round(t_def, 4),
+                                              "value": round(pts * nsteps / t_def / 1e6, 1)}
+        out["one_at_a_time_tuned_shape"] = {"wall_s": round(t_tuned, 4),
+                                            "value": round(pts * nsteps / t_tuned / 1e6, 1)}
+        out["speedup_vs_default_shape"] = round(t_def / wall, 3)
+        out["speedup_vs_tuned_shape"] = round(t_tuned / wall, 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +135,15 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--arith", default="fp32", choices=["fp32", "ref"])
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched multi-shot measurement reported beside the headline")
+    ap.add_argument("--workload", default="fd", choices=["fd", "batch"],
+                    help="fd: the 512^3 headline; batch: only the batched multi-shot workload "
+                         "(--steps steps, default 800)")
+    ap.add_argument("--batch-shape", default=None,
+                    help="tile_j,block_waves,chunk_i,blocks_per_cu for the batch (default: autotuned)")
+    ap.add_argument("--no-baselines", action="store_true",
+                    help="batch workload: skip the one-at-a-time runs")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -77,6 +155,18 @@ def main():
     if phc.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "batch":
+        shape = [int(v) for v in args.batch_shape.split(",")] if args.batch_shape else None
+        nst = args.steps if "--steps" in sys.argv else 800
+        r = bench_batch(phc, nsteps=nst, baselines=not args.no_baselines, shape=shape)
+        out = {"metric": METRIC, "value": r["value"], "unit": "Mpoint-updates/s", "n_gpus": 1,
+               "steps": nst, "warmup": nst, "ms_per_step": round(r["wall_s"] / nst * 1e3, 4),
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+               "data": "synthetic: seeded per-shot grids, 6-layer vp models and Ricker sources",
+               "config": {"workload": "batched multi-shot FD: " + r["workload"]},
+               "roofline": r["roofline"], "batch": r}
+        print(json.dumps(out), flush=True)
+        return
     n = args.n
     K, W = args.steps, args.warmup
     setup = phc.FdSetup(nx=n, model=("layered", 6, 20240512), f0=10.0, cfl=0.4,
@@ -156,6 +246,9 @@ def main():
         "copy_gbs": round(copy_gbs, 1),
         "finite": fin == 0,
     }
+    sol.close()
+    if not args.no_extras:
+        out["batch"] = bench_batch(phc)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(setup)
     print(json.dumps(out), flush=True)

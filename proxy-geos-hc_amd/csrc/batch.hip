@@ -50,32 +50,65 @@ struct FdGrid {
     float *trace;
 };
 
+// Work queues of one step: queue q holds items[start[q] .. start[q]+len[q]),
+// a contiguous run of tiles in spatial order of roughly equal cost.
+constexpr int BATCH_QUEUES = 8;  // one per XCD
+struct BatchQueues {
+    int start[BATCH_QUEUES];
+    int len[BATCH_QUEUES];
+};
+
 // One time step of every shot.  `c` holds what the shots share (coefficients,
-// sponge table, chunk length, deferred damping flag); the persistent blocks
-// walk the work list with a grid stride.
+// sponge table, chunk length, deferred damping flag).  The hardware deals
+// consecutive blocks round-robin over the 8 XCDs, so block b starts on queue
+// b % 8: the blocks of one XCD take the tiles of one contiguous region in
+// order, and neighbouring tiles that run side by side share their halo rows
+// in that XCD's L2.  A block whose queue is empty takes items from the other
+// queues.  counter[q] (zero at the start of the step) is the next unclaimed
+// position of queue q.
 template <int R, int TY, int NJ, int ARITH, typename IT>
 __global__ void __launch_bounds__(FD_TK *TY)
 fd_batch_kernel(FdStepArgs c, const ShotDesc *__restrict__ shots, const WorkItem *__restrict__ items,
-                int nitems, int cur, int it, int inj, float *traces, int trace_stride, int step)
+                BatchQueues qs, int cur, int it, int inj, float *traces, int trace_stride, int step,
+                int *counter)
 {
     using SH = FdTileShape<R, TY, NJ>;
     __shared__ float tile[2][SH::LH][SH::LW];
-    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
-        const WorkItem wi = items[w];
-        const ShotDesc &d = shots[wi.shot];
-        FdGrid g;
-        g.L = d.L;
-        g.pn = d.lev[cur];
-        g.pnm1 = d.lev[1 - cur];
-        g.vp = d.vp;
-        g.fy = d.fy;
-        g.fz = d.fz;
-        g.src_off = d.src_off;
-        g.inj = inj;
-        g.rhs_next = inj ? d.rhs[it + 1] : 0.f;
-        g.trace = traces + (int64_t)wi.shot * trace_stride + step;
-        fd_tile<R, TY, NJ, ARITH, IT>(c, g, wi.kt, wi.jt, wi.ic, tile);
+    __shared__ int claimed[2];  // alternating slots: a slot is rewritten two claims later,
+    int slot = 0;               // after a barrier every thread passes once it has read it
+    const bool leader = threadIdx.x == 0 && threadIdx.y == 0;
+    const int q0 = blockIdx.x % BATCH_QUEUES;
+    for (int d = 0; d < BATCH_QUEUES; ++d) {
+        const int q = (q0 + d) % BATCH_QUEUES;
+        while (true) {
+            if (leader) claimed[slot] = atomicAdd(counter + q, 1);
+            lds_barrier();
+            const int pos = claimed[slot];
+            slot ^= 1;
+            if (pos >= qs.len[q]) break;
+            const WorkItem wi = items[qs.start[q] + pos];
+            const ShotDesc &sd = shots[wi.shot];
+            FdGrid g;
+            g.L = sd.L;
+            g.pn = sd.lev[cur];
+            g.pnm1 = sd.lev[1 - cur];
+            g.vp = sd.vp;
+            g.fy = sd.fy;
+            g.fz = sd.fz;
+            g.src_off = sd.src_off;
+            g.inj = inj;
+            g.rhs_next = inj ? sd.rhs[it + 1] : 0.f;
+            g.trace = traces + (int64_t)wi.shot * trace_stride + step;
+            fd_tile<R, TY, NJ, ARITH, IT>(c, g, wi.kt, wi.jt, wi.ic, tile);
+        }
     }
+}
+
+// Zero the per-step work counters (a kernel, so that captured graphs replay it).
+__global__ void fd_batch_zero_kernel(int *counter, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) counter[t] = 0;
 }
 
 // rhs[it] into every shot's current level at its source (FDTDkernels.hpp:121-128)
@@ -102,7 +135,8 @@ __global__ void fd_batch_sample_kernel(const ShotDesc *shots, int nshots, const 
     out[(int64_t)t * stride + step] = (o == d.src_off) ? traces[(int64_t)s * stride + step] : d.lev[level][o];
 }
 
-using BatchFn = void (*)(FdStepArgs, const ShotDesc *, const WorkItem *, int, int, int, int, float *, int, int);
+using BatchFn = void (*)(FdStepArgs, const ShotDesc *, const WorkItem *, BatchQueues, int, int, int, float *,
+                        int, int, int *);
 
 template <int R, int TY, int NJ>
 static BatchFn batch_arith(int arith)
@@ -135,12 +169,13 @@ struct phc_batch {
     std::vector<FdLayout> L;
     std::vector<int> src;
     std::vector<ShotDesc> hdesc;
-    DevBuf pool, shots, items, traces, rec, recoff;
+    DevBuf pool, shots, items, traces, rec, recoff, counters;
     int cur = 1;      // level holding the current field (the reference's i2)
     int pending = 0;  // the previous level still owes its sponge product
     hipStream_t stream = nullptr;
     int ty = 16, nj = 1, chunk_i = 64, blocks_per_cu = 1, cus = 256;
     int nitems = 0;
+    BatchQueues queues;
     int nrec = 0;  // receivers per shot
     int timing = 0;
     double last_ms = 0.0;
@@ -156,9 +191,9 @@ static void batch_drop_graph(phc_batch *h)
     h->g_it0 = h->g_n = h->g_cur = h->g_pending = h->g_nrec = -1;
 }
 
-// Work items of every shot for the current shape, sorted by decreasing cost
-// (planes incl. the 2R lead-in x rows x lanes) so that the grid-stride walk
-// deals the long items first and the short ones fill the tail.
+// Work items of every shot for the current shape in spatial order (shot,
+// i-chunk, j-tile, k-tile), cut into BATCH_QUEUES contiguous runs of about
+// equal cost (planes incl. the 2R lead-in x rows).
 static phc_status batch_build_items(phc_batch *h)
 {
     std::vector<WorkItem> items;
@@ -177,14 +212,20 @@ static phc_status batch_build_items(phc_batch *h)
                     cost.push_back((double)(planes + 2 * h->R) * rows);
                 }
     }
-    std::vector<int> order(items.size());
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::vector<WorkItem> sorted(items.size());
-    for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
-    h->nitems = (int)sorted.size();
-    PHC_TRY(h->items.alloc(sizeof(WorkItem) * sorted.size()));
-    PHC_HIP(hipMemcpy(h->items.ptr, sorted.data(), sizeof(WorkItem) * sorted.size(), hipMemcpyHostToDevice));
+    const double total = std::accumulate(cost.begin(), cost.end(), 0.0);
+    int q = 0;
+    double acc = 0.0;
+    h->queues.start[0] = 0;
+    for (size_t i = 0; i < items.size(); ++i) {
+        acc += cost[i];
+        while (q + 1 < BATCH_QUEUES && acc >= total * (q + 1) / BATCH_QUEUES) h->queues.start[++q] = (int)i + 1;
+    }
+    while (q + 1 < BATCH_QUEUES) h->queues.start[++q] = (int)items.size();
+    for (int k = 0; k < BATCH_QUEUES; ++k)
+        h->queues.len[k] = (k + 1 < BATCH_QUEUES ? h->queues.start[k + 1] : (int)items.size()) - h->queues.start[k];
+    h->nitems = (int)items.size();
+    PHC_TRY(h->items.alloc(sizeof(WorkItem) * items.size()));
+    PHC_HIP(hipMemcpy(h->items.ptr, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice));
     return PHC_OK;
 }
 
@@ -205,15 +246,16 @@ static void batch_common(const phc_batch *h, FdStepArgs &c, int damp)
 // (previous, overwritten by the new level).  traces[shot*stride + step] gets
 // each shot's new source value before the fused injection of rhs[it+1].
 static phc_status batch_launch(phc_batch *h, int cur, int it, int inj, int damp, float *traces, int stride,
-                               int step)
+                               int step, int *counter)
 {
     BatchFn fn = batch_pick(h->R, h->ty, h->nj, h->arith);
     if (!fn) return fail(PHC_ERR_INVALID, "no batched kernel for %d waves x %d rows", h->ty, h->nj);
     FdStepArgs c;
     batch_common(h, c, damp);
-    const int grid = std::max(1, std::min(h->nitems, h->cus * h->blocks_per_cu));
+    // resident blocks, a multiple of the queue count (no more than there are items)
+    const int grid = std::max(BATCH_QUEUES, std::min(h->nitems, h->cus * h->blocks_per_cu) / BATCH_QUEUES * BATCH_QUEUES);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(FD_TK, h->ty), 0, h->stream, c, h->shots.as<ShotDesc>(),
-                       h->items.as<WorkItem>(), h->nitems, cur, it, inj, traces, stride, step);
+                       h->items.as<WorkItem>(), h->queues, cur, it, inj, traces, stride, step, counter);
     PHC_HIP(hipGetLastError());
     return PHC_OK;
 }
@@ -221,8 +263,17 @@ static phc_status batch_launch(phc_batch *h, int cur, int it, int inj, int damp,
 // The whole run on h->stream: injection of rhs[it0] into every current level,
 // then one batched step per sample (plus receiver sampling).  With `ev`,
 // events bracket every batched launch.
+static phc_status batch_zero_counters(phc_batch *h, int n)
+{
+    hipLaunchKernelGGL(fd_batch_zero_kernel, dim3((n + 255) / 256), dim3(256), 0, h->stream,
+                       h->counters.as<int>(), n);
+    PHC_HIP(hipGetLastError());
+    return PHC_OK;
+}
+
 static phc_status batch_enqueue(phc_batch *h, int it0, int nsteps, hipEvent_t *ev)
 {
+    PHC_TRY(batch_zero_counters(h, nsteps * BATCH_QUEUES));
     hipLaunchKernelGGL(fd_batch_inject_kernel, dim3((h->nshots + 63) / 64), dim3(64), 0, h->stream,
                        h->shots.as<ShotDesc>(), h->nshots, h->cur, it0);
     PHC_HIP(hipGetLastError());
@@ -230,7 +281,8 @@ static phc_status batch_enqueue(phc_batch *h, int it0, int nsteps, hipEvent_t *e
     for (int s = 0; s < nsteps; ++s) {
         const int damp = (s == 0) ? h->pending : (h->sw > 0 ? 1 : 0);
         if (ev) PHC_HIP(hipEventRecord(ev[2 * s], h->stream));
-        PHC_TRY(batch_launch(h, cur, it0 + s, s + 1 < nsteps ? 1 : 0, damp, h->traces.as<float>(), nsteps, s));
+        PHC_TRY(batch_launch(h, cur, it0 + s, s + 1 < nsteps ? 1 : 0, damp, h->traces.as<float>(), nsteps, s,
+                             h->counters.as<int>() + s * BATCH_QUEUES));
         if (ev) PHC_HIP(hipEventRecord(ev[2 * s + 1], h->stream));
         if (h->nrec > 0) {
             const int n = h->nshots * h->nrec;
@@ -364,7 +416,8 @@ phc_status phc_batch_destroy(phc_batch *h)
     if (!h) return PHC_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     batch_drop_graph(h);
-    for (DevBuf *b : {&h->pool, &h->shots, &h->items, &h->traces, &h->rec, &h->recoff}) b->release();
+    for (DevBuf *b : {&h->pool, &h->shots, &h->items, &h->traces, &h->rec, &h->recoff, &h->counters})
+        b->release();
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return PHC_OK;
@@ -404,11 +457,13 @@ phc_status phc_batch_run(phc_batch *h, int it0, int nsteps)
                     h->nsamples);
     const size_t tbytes = sizeof(float) * (size_t)nsteps * h->nshots;
     const size_t rbytes = tbytes * h->nrec;
-    if (h->traces.bytes < tbytes || (h->nrec > 0 && h->rec.bytes < rbytes)) {
+    const size_t cbytes = sizeof(int) * (size_t)nsteps * BATCH_QUEUES;
+    if (h->traces.bytes < tbytes || (h->nrec > 0 && h->rec.bytes < rbytes) || h->counters.bytes < cbytes) {
         PHC_HIP(hipStreamSynchronize(h->stream));
         batch_drop_graph(h);
         if (h->traces.bytes < tbytes) PHC_TRY(h->traces.alloc(tbytes));
         if (h->nrec > 0 && h->rec.bytes < rbytes) PHC_TRY(h->rec.alloc(rbytes));
+        if (h->counters.bytes < cbytes) PHC_TRY(h->counters.alloc(cbytes));
     }
     if (h->timing) {
         std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
@@ -530,15 +585,17 @@ phc_status phc_batch_autotune(phc_batch *h, int steps_per_trial, int *tile_j, in
     PHC_HIP(hipStreamSynchronize(h->stream));
     batch_drop_graph(h);
     // trials run on a copy of the pool; the shots' state is restored after
-    DevBuf save, tr;
+    DevBuf save, tr, ctr;
     PHC_TRY(save.alloc(h->pool.bytes));
     PHC_TRY(tr.alloc(sizeof(float) * h->nshots));
+    const int nctr = steps_per_trial * BATCH_QUEUES;
+    PHC_TRY(ctr.alloc(sizeof(int) * nctr));
     PHC_HIP(hipMemcpy(save.ptr, h->pool.ptr, h->pool.bytes, hipMemcpyDeviceToDevice));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     PHC_HIP(hipEventCreate(&e0));
     PHC_HIP(hipEventCreate(&e1));
     static const int shapes[][2] = {{16, 1}, {8, 2}, {4, 2}, {4, 4}};
-    static const int chunks[] = {32, 64, 128, 256};
+    static const int chunks[] = {32, 64, 128, 256, 512};
     static const int bpcs[] = {1, 2, 4};
     double best = 1e30;
     int bty = h->ty, bnj = h->nj, bci = h->chunk_i, bbp = h->blocks_per_cu;
@@ -555,9 +612,12 @@ phc_status phc_batch_autotune(phc_batch *h, int steps_per_trial, int *tile_j, in
                 if ((st = batch_build_items(h))) break;
                 float ms = 0.f;
                 for (int rep = 0; rep < 2 && st == PHC_OK; ++rep) {  // first pass warms up
+                    hipLaunchKernelGGL(fd_batch_zero_kernel, dim3((nctr + 255) / 256), dim3(256), 0, h->stream,
+                                       ctr.as<int>(), nctr);
                     if (hipEventRecord(e0, h->stream) != hipSuccess) st = fail(PHC_ERR_HIP, "event record failed");
                     for (int s = 0; s < steps_per_trial && st == PHC_OK; ++s)
-                        st = batch_launch(h, (h->cur + s) & 1, 0, 0, h->sw > 0, tr.as<float>(), 1, 0);
+                        st = batch_launch(h, (h->cur + s) & 1, 0, 0, h->sw > 0, tr.as<float>(), 1, 0,
+                                          ctr.as<int>() + s * BATCH_QUEUES);
                     if (st == PHC_OK && (hipEventRecord(e1, h->stream) != hipSuccess ||
                                          hipEventSynchronize(e1) != hipSuccess ||
                                          hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
@@ -575,6 +635,7 @@ phc_status phc_batch_autotune(phc_batch *h, int steps_per_trial, int *tile_j, in
         st = fail(PHC_ERR_HIP, "restoring the shots after tuning failed");
     save.release();
     tr.release();
+    ctr.release();
     h->ty = bty; h->nj = bnj; h->chunk_i = bci; h->blocks_per_cu = bbp;
     PHC_TRY(st);
     PHC_TRY(batch_build_items(h));
