@@ -253,6 +253,71 @@ phc_status phc_batch_synchronize(phc_batch *h);
 phc_status phc_batch_autotune(phc_batch *h, int steps_per_trial, int *tile_j, int *block_waves,
                               int *chunk_i, int *blocks_per_cu);
 
+/* ======================================================= LEARNED BOUNDARY
+ * A thin learned absorbing layer for the FD step, in place of the sponge
+ * (FDTDinit.hpp:319-396, FDTDkernels.hpp:108-118) or the PML (FDTDkernels.hpp:
+ * 36-104).  No reference counterpart.  The grid ends PHC_LB_WIDTH points
+ * outside the region of interest, with no damping; at each of those points,
+ * face f's network predicts the term of the radius-4 update that would read
+ * beyond the grid edge:
+ *   x = ((pn - h)/s at offsets -4..4 along the inward normal,
+ *        (pnm1 - h)/s at the same offsets, sqrt(vp^2dt^2 c_1)),
+ *   s = max |x_field - h| (h: the value the halo holds),
+ *   new value += vp^2dt^2 c_1 s y,
+ *   y = w3 . relu(W2 relu(W1 x + b1[depth]) + b2) + b3
+ * with 19 inputs, two hidden layers of 32, bf16 weights and FP32 accumulation
+ * (MFMA).  Parameters per face, FP32, in this order: W1[32][19], b1[4][32]
+ * (one per layer depth 0..3 from the edge), W2[32][32], b2[32], w3[32], b3;
+ * faces 2*axis+side (axis 0 = x/i, 1 = y/j, 2 = z/k; side 0 = low).
+ * Training samples are PHC_LB_SAMPLE_FLOATS floats: x[19], target, depth,
+ * index of the point in its face's layer (compact order below, as a float);
+ * the target is the missing term over c_1 s, taken from a wide-sponge run.
+ */
+typedef struct phc_lb phc_lb;
+#define PHC_LB_WIDTH 4
+#define PHC_LB_FACES 6
+#define PHC_LB_PARAMS_PER_FACE 1825
+#define PHC_LB_SAMPLE_FLOATS 22
+
+/* A model with He-uniform random weights (seeded). */
+phc_status phc_lb_create(uint64_t seed, phc_lb **out);
+phc_status phc_lb_destroy(phc_lb *h);
+/* FP32 parameters, PHC_LB_FACES x PHC_LB_PARAMS_PER_FACE.  set resets Adam. */
+phc_status phc_lb_get_params(phc_lb *h, float *host);
+phc_status phc_lb_set_params(phc_lb *h, const float *host);
+/* Training-sample store: capacity per face (clears it), clear, counts[6]. */
+phc_status phc_lb_set_capacity(phc_lb *h, int64_t per_face);
+phc_status phc_lb_clear_samples(phc_lb *h);
+phc_status phc_lb_sample_counts(phc_lb *h, int64_t *counts);
+phc_status phc_lb_get_samples(phc_lb *h, int face, int64_t first, int64_t n, float *host);
+phc_status phc_lb_add_samples(phc_lb *h, int face, const float *host, int64_t n);
+/* Append samples from an FD run with a wide sponge: the layer points of the
+ * region origin[3] .. origin+dims of its grid, inputs from levels cur
+ * (current) and 1-cur (previous), the targets from the grid beyond the
+ * region.  halo: the value the thin grid's halo will hold.  Each point is kept
+ * with probability keep, and only if its s >= s_min.  The region plus 4
+ * points must lie outside the wide run's sponge. */
+phc_status phc_lb_collect(phc_lb *h, phc_fd *wide, int cur, const int *origin, const int *dims, float halo,
+                          float keep, float s_min, uint64_t seed);
+/* Mean squared error over every stored sample and, if grad != NULL, its
+ * gradient with respect to the FP32 parameters (at their bf16 values). */
+phc_status phc_lb_eval(phc_lb *h, float *loss, float *grad);
+/* Adam (lr, 0.9, 0.999, 1e-8) on minibatches of batch_per_face random samples
+ * of every face; loss_history[s] (may be NULL) = minibatch loss at step s. */
+phc_status phc_lb_train(phc_lb *h, int steps, int batch_per_face, float lr, uint64_t seed, float *loss_history);
+/* Network output y of face `face` for n given samples (MFMA inference path). */
+phc_status phc_lb_infer(phc_lb *h, int face, const float *samples, int64_t n, float *y);
+/* Attach a copy of the model's bf16 weights to an FD handle (lb NULL detaches).
+ * Needs half width 4, equal coefficients on the three axes, sponge_width 0,
+ * sides >= 16 and the source outside the layer; halo is the value the halo
+ * holds (the handle's init_value).  Every step then runs the correction pass,
+ * the stencil and the apply pass. */
+phc_status phc_fd_set_learned_boundary(phc_fd *h, const phc_lb *lb, float halo);
+/* The corrections the next step would add (vp^2dt^2 c_1 s y per layer point)
+ * from levels cur / 1-cur, compact per face: face f, depth d (from its edge),
+ * u, v = the other two axes in increasing order, index ((d*n_u)+u)*n_v+v. */
+phc_status phc_fd_lb_corrections(phc_fd *h, int cur, float *host);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,6 +26,7 @@
 
 #include "fd_kernels.hpp"
 #include "fd_tile.hpp"
+#include "lb.hpp"
 #include "march.hpp"
 #include "phc_common.hpp"
 
@@ -167,6 +168,10 @@ struct phc_fd {
     int last_n = 0, last_nrec = 0;
     int timing = 0;
     double last_ms = 0.0;
+    // learned thin boundary (phc_fd_set_learned_boundary)
+    int lb_on = 0;
+    LbInfer lbi;
+    DevBuf lb_w, lb_corr;
     // cached graph of the last phc_fd_run
     hipGraphExec_t gexec = nullptr;
     int g_it0 = -1, g_n = -1, g_cur = -1, g_damp = -1, g_nrec = -1;
@@ -303,6 +308,38 @@ static void fd_drop_graph(phc_fd *h)
     h->g_it0 = h->g_n = h->g_cur = h->g_damp = h->g_nrec = -1;
 }
 
+// One full step from levels cb (current) and ca (previous, overwritten by the
+// new level): with the learned boundary, its correction pass before and its
+// apply pass after the stencil launch.
+static phc_status fd_launch_full_step(phc_fd *h, const FdStepArgs &a, bool damp, int ca, int cb)
+{
+    if (h->lb_on)
+        PHC_HIP(lb_enqueue_correct(h->lbi, h->L, h->lev[cb].as<float>(), h->lev[ca].as<float>(),
+                                   h->vp.as<float>(), h->stream));
+    PHC_TRY(fd_launch_step(h, a, damp));
+    if (h->lb_on) PHC_HIP(lb_enqueue_apply(h->lbi, h->L, h->lev[ca].as<float>(), h->stream));
+    return PHC_OK;
+}
+
+phc_status phc::fd_field_view(phc_fd *h, FdFieldView *v)
+{
+    if (!h || !v) return fail(PHC_ERR_INVALID, "NULL argument");
+    if (h->R != LB_R) return fail(PHC_ERR_INVALID, "the learned boundary needs half width %d", LB_R);
+    for (int l = 0; l <= h->R; ++l)
+        if (h->cx[l] != h->cy[l] || h->cx[l] != h->cz[l])
+            return fail(PHC_ERR_INVALID, "the learned boundary needs equal coefficients on the three axes");
+    if (!(h->cx[1] > 0.f)) return fail(PHC_ERR_INVALID, "c_1 must be positive");
+    v->L = h->L;
+    v->lev[0] = h->lev[0].as<float>();
+    v->lev[1] = h->lev[1].as<float>();
+    v->vp = h->vp.as<float>();
+    v->sw = h->sw;
+    v->c1 = h->cx[1];
+    for (int l = 0; l <= LB_R; ++l) v->chat[l] = h->cx[l] / h->cx[1];
+    v->stream = h->stream;
+    return PHC_OK;
+}
+
 extern "C" {
 
 phc_status phc_fd_create(const phc_fd_params *p, phc_fd **out)
@@ -376,7 +413,7 @@ phc_status phc_fd_destroy(phc_fd *h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     fd_drop_graph(h);
     for (DevBuf *b : {&h->lev[0], &h->lev[1], &h->vp, &h->fx, &h->fy, &h->fz, &h->rhs, &h->trace,
-                      &h->rec, &h->recoff})
+                      &h->rec, &h->recoff, &h->lb_w, &h->lb_corr})
         b->release();
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -404,7 +441,7 @@ phc_status phc_fd_compute_one_step_sb(phc_fd *h, int ca, int cb)
     PHC_TRY(fd_materialize(h, cb));
     FdStepArgs a;
     fd_fill_args(h, a, ca, cb);
-    PHC_TRY(fd_launch_step(h, a, h->pending[ca] != 0));
+    PHC_TRY(fd_launch_full_step(h, a, h->pending[ca] != 0, ca, cb));
     h->pending[ca] = 0;
     h->pending[cb] = (h->sw > 0) ? 1 : 0;
     return PHC_OK;
@@ -457,7 +494,7 @@ static phc_status fd_enqueue_run(phc_fd *h, int it0, int nsteps, int c0, bool da
             a.rhs_next = h->hrhs[it + 1];
         }
         if (ev) PHC_HIP(hipEventRecord(ev[2 * s], h->stream));
-        PHC_TRY(fd_launch_step(h, a, s == 0 ? damp0 : (h->sw > 0)));
+        PHC_TRY(fd_launch_full_step(h, a, s == 0 ? damp0 : (h->sw > 0), ca, cb));
         if (ev) PHC_HIP(hipEventRecord(ev[2 * s + 1], h->stream));
         if (h->nrec > 0) {
             hipLaunchKernelGGL(fd_sample_kernel, dim3((h->nrec + 255) / 256), dim3(256), 0,
@@ -648,6 +685,52 @@ phc_status phc_fd_set_timing(phc_fd *h, int enable)
 {
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
     h->timing = enable ? 1 : 0;
+    return PHC_OK;
+}
+
+phc_status phc_fd_set_learned_boundary(phc_fd *h, const phc_lb *lb, float halo)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    fd_drop_graph(h);
+    if (!lb) {
+        h->lb_on = 0;
+        h->lb_w.release();
+        h->lb_corr.release();
+        return PHC_OK;
+    }
+    FdFieldView fv;
+    PHC_TRY(fd_field_view(h, &fv));
+    if (h->sw != 0) return fail(PHC_ERR_INVALID, "the learned boundary replaces the sponge: sponge_width must be 0");
+    const int n[3] = {h->nx, h->ny, h->nz}, src[3] = {h->xs, h->ys, h->zs};
+    for (int a = 0; a < 3; ++a) {
+        if (n[a] < 4 * LB_W) return fail(PHC_ERR_INVALID, "grid side %d below %d", n[a], 4 * LB_W);
+        if (src[a] < LB_W || src[a] >= n[a] - LB_W)
+            return fail(PHC_ERR_INVALID, "the source must lie inside the %d-point learned layer", LB_W);
+    }
+    if (!std::isfinite(halo)) return fail(PHC_ERR_INVALID, "halo value is not finite");
+    const int64_t ncorr = lb_corr_layout(h->L, h->lbi.corr_off);
+    PHC_TRY(h->lb_w.alloc(sizeof(uint16_t) * LB_FACES * LB_NP));
+    PHC_TRY(h->lb_corr.alloc(sizeof(float) * (size_t)ncorr));
+    PHC_TRY(lb_export_bf16(lb, h->lb_w.as<uint16_t>(), h->stream));
+    h->lbi.w = h->lb_w.as<uint16_t>();
+    h->lbi.corr = h->lb_corr.as<float>();
+    h->lbi.halo = halo;
+    h->lbi.c1 = fv.c1;
+    for (int l = 0; l <= LB_R; ++l) h->lbi.chat[l] = fv.chat[l];
+    h->lb_on = 1;
+    return PHC_OK;
+}
+
+phc_status phc_fd_lb_corrections(phc_fd *h, int cur, float *host)
+{
+    if (!h || !host) return fail(PHC_ERR_INVALID, "handle or host buffer is NULL");
+    if (!h->lb_on) return fail(PHC_ERR_STATE, "no learned boundary attached");
+    if (cur < 0 || cur > 1) return fail(PHC_ERR_INVALID, "cur level %d is not 0 or 1", cur);
+    PHC_HIP(lb_enqueue_correct(h->lbi, h->L, h->lev[cur].as<float>(), h->lev[1 - cur].as<float>(),
+                               h->vp.as<float>(), h->stream));
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    PHC_HIP(hipMemcpy(host, h->lb_corr.ptr, sizeof(float) * h->lbi.corr_off[LB_FACES], hipMemcpyDeviceToHost));
     return PHC_OK;
 }
 

@@ -1,0 +1,876 @@
+// mlp.hip -- learned thin absorbing boundary: training-data collection from
+// wide-sponge runs, training (forward, backward, Adam) and MFMA inference
+// fused into the FD time loop (phc_lb_* C ABI).  The model is described in
+// lb.hpp.  No reference counterpart (the reference's boundaries are the
+// sponge, src/fd/utilsFDTD/FDTDinit.hpp:319-396, and the PML,
+// FDTDkernels.hpp:36-104).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "fd_kernels.hpp"
+#include "lb.hpp"
+#include "march.hpp"
+#include "phc_common.hpp"
+
+namespace phc {
+
+// ------------------------------------------------------------ bf16 helpers
+__device__ inline uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// ------------------------------------------------------------ layer geometry
+// Face f = 2*axis + side.  Its layer is LB_W x n_u x n_v points (u, v the
+// other two axes in increasing order); compact index ((d*n_u)+u)*n_v+v.
+struct LbFaceGeom {
+    int axis, side, nu, nv;
+};
+
+__host__ __device__ inline LbFaceGeom lb_face(const int n[3], int f)
+{
+    LbFaceGeom g;
+    g.axis = f >> 1;
+    g.side = f & 1;
+    g.nu = (g.axis == 0) ? n[1] : n[0];
+    g.nv = (g.axis == 2) ? n[1] : n[2];
+    return g;
+}
+
+// grid coordinates of compact index (d, u, v) of face g
+__host__ __device__ inline void lb_point(const int n[3], const LbFaceGeom &g, int d, int u, int v, int c[3])
+{
+    const int a = g.axis;
+    const int ua = (a == 0) ? 1 : 0, va = (a == 2) ? 1 : 2;
+    c[a] = g.side ? n[a] - 1 - d : d;
+    c[ua] = u;
+    c[va] = v;
+}
+
+int64_t lb_corr_layout(const FdLayout &L, int64_t *off)
+{
+    const int n[3] = {L.nx, L.ny, L.nz};
+    off[0] = 0;
+    for (int f = 0; f < LB_FACES; ++f) {
+        const LbFaceGeom g = lb_face(n, f);
+        off[f + 1] = off[f] + (int64_t)LB_W * g.nu * g.nv;
+    }
+    return off[LB_FACES];
+}
+
+__device__ inline uint32_t lb_hash(uint64_t x)
+{
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return (uint32_t)x;
+}
+
+// Inputs of one layer point from a field pair: x[0..8] = (pn - h)/s at
+// offsets -4..4 along the inward normal, x[9..17] the same for pnm1, x[18] =
+// sqrt(vp2 c1).  Offsets beyond the edge of the region [0, n) read as h (as
+// the grid's halo holds h).  Returns s (0 if every input equals h).
+struct LbGridView {
+    const float *pn, *pnm1, *vp;
+    FdLayout L;
+    int o[3];  // region origin in the grid
+    int n[3];  // region size
+};
+
+__device__ inline float lb_inputs(const LbGridView &G, const LbFaceGeom &g, int d, int u, int v, float h, float c1,
+                                  float x[LB_NIN], int64_t *center)
+{
+    int c[3];
+    lb_point(G.n, g, d, u, v, c);
+    const int dir = g.side ? -1 : 1;  // inward
+    int64_t step;
+    if (g.axis == 0) step = G.L.pitch_j;
+    else if (g.axis == 1) step = G.L.pitch_k;
+    else step = 1;
+    const int64_t base = fd_off(G.L, G.o[0] + c[0], G.o[1] + c[1], G.o[2] + c[2]);
+    *center = base;
+    float s = 0.f;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+        const int out = 4 - m;  // outward offset (negative: inward)
+        float a = 0.f, b = 0.f;
+        if (out <= d) {
+            const int64_t o = base + (int64_t)(m - 4) * dir * step;
+            a = G.pn[o] - h;
+            b = G.pnm1[o] - h;
+        }
+        x[m] = a;
+        x[9 + m] = b;
+        s = fmaxf(s, fmaxf(fabsf(a), fabsf(b)));
+    }
+    // below LB_S_TINY the point is treated as quiet (1/s would overflow)
+    const float inv = s > LB_S_TINY ? 1.f / s : 0.f;
+#pragma unroll
+    for (int m = 0; m < 18; ++m) x[m] *= inv;
+    x[18] = sqrtf(G.vp[base] * c1);
+    return s;
+}
+
+// ------------------------------------------------------------ collection
+struct LbCollectArgs {
+    LbGridView G;
+    float h, c1, chat[LB_R + 1];
+    float keep;   // probability of keeping a point (hash-based)
+    float s_min;  // points whose s is below this are skipped
+    uint64_t seed;
+    float *out[LB_FACES];
+    unsigned long long *count;  // per face
+    int64_t cap;
+    int64_t face_off[LB_FACES + 1];  // compact offsets of the region's faces
+};
+
+// One thread per layer point of the region.  Target: the term the region's
+// update misses, t = sum_{l>d} (c_l/c_1) (p(P + l n_out) - h) / s, read from
+// the wide grid around the region.
+__global__ void lb_collect_kernel(LbCollectArgs A)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.face_off[LB_FACES]) return;
+    int f = 0;
+    while (f + 1 < LB_FACES && t >= A.face_off[f + 1]) ++f;
+    const LbFaceGeom g = lb_face(A.G.n, f);
+    int64_t r = t - A.face_off[f];
+    const int v = (int)(r % g.nv);
+    r /= g.nv;
+    const int u = (int)(r % g.nu);
+    const int d = (int)(r / g.nu);
+    const uint32_t hv = lb_hash(A.seed * 0x9e3779b97f4a7c15ull + (uint64_t)t);
+    if ((float)(hv >> 8) * (1.f / 16777216.f) >= A.keep) return;
+    float x[LB_NIN];
+    int64_t center;
+    const float s = lb_inputs(A.G, g, d, u, v, A.h, A.c1, x, &center);
+    if (!(s >= A.s_min) || !(s > LB_S_TINY)) return;
+    const int dir = g.side ? -1 : 1;
+    const int64_t step = g.axis == 0 ? A.G.L.pitch_j : (g.axis == 1 ? A.G.L.pitch_k : 1);
+    float tgt = 0.f;
+    for (int l = d + 1; l <= LB_R; ++l) tgt += A.chat[l] * (A.G.pn[center - (int64_t)l * dir * step] - A.h);
+    tgt /= s;
+    const unsigned long long slot = atomicAdd(A.count + f, 1ull);
+    if ((int64_t)slot >= A.cap) return;
+    float *o = A.out[f] + slot * LB_NSTRIDE;
+    for (int m = 0; m < LB_NIN; ++m) o[m] = x[m];
+    o[LB_NIN] = tgt;
+    o[LB_NIN + 1] = (float)d;
+    o[LB_NIN + 2] = (float)(t - A.face_off[f]);  // compact index in the face's layer
+}
+
+// ------------------------------------------------------------ training
+constexpr int LB_TB = 64;  // samples (threads) per training block
+
+struct LbBatch {
+    const float *samples[LB_FACES];
+    const int *idx[LB_FACES];  // positions in samples (nullptr: 0..count-1)
+    int count[LB_FACES];
+    int block_start[LB_FACES + 1];
+    float inv_total;  // 1 / total samples in the batch
+};
+
+// Forward, loss and backward of one batch: every block takes LB_TB samples of
+// one face; the weights are used rounded to bf16 (the stored inference
+// precision), arithmetic in FP32.  Per-sample activations and deltas go to
+// LDS; each thread then sums the gradient of a strided subset of the face's
+// parameters over the block's samples and adds it to grad (FP32 atomics).
+__global__ void __launch_bounds__(LB_TB) lb_train_kernel(const float *__restrict__ theta, float *grad, float *loss,
+                                                         LbBatch B)
+{
+    __shared__ float w[LB_NP];
+    __shared__ float sx[LB_TB][LB_NIN + 1];
+    __shared__ float sh1[LB_TB][LB_H + 1], sh2[LB_TB][LB_H + 1], sd1[LB_TB][LB_H + 1], sd2[LB_TB][LB_H + 1];
+    __shared__ float sdy[LB_TB];
+    __shared__ int sdep[LB_TB];
+    __shared__ float sloss[LB_TB];
+    int f = 0;
+    while (f + 1 < LB_FACES && (int)blockIdx.x >= B.block_start[f + 1]) ++f;
+    const int chunk = blockIdx.x - B.block_start[f];
+    const int tid = threadIdx.x;
+    for (int p = tid; p < LB_NP; p += LB_TB) w[p] = bf2f(f2bf(theta[f * LB_NP + p]));
+    __syncthreads();
+    const int pos = chunk * LB_TB + tid;
+    const bool valid = pos < B.count[f];
+    float t = 0.f;
+    int d = 0;
+    if (valid) {
+        const int si = B.idx[f] ? B.idx[f][pos] : pos;
+        const float *s = B.samples[f] + (int64_t)si * LB_NSTRIDE;
+        for (int m = 0; m < LB_NIN; ++m) sx[tid][m] = s[m];
+        t = s[LB_NIN];
+        d = min(max((int)s[LB_NIN + 1], 0), LB_W - 1);
+    } else {
+        for (int m = 0; m < LB_NIN; ++m) sx[tid][m] = 0.f;
+    }
+    // forward; activations stay in this thread's LDS rows
+    for (int i = 0; i < LB_H; ++i) {
+        float a = w[LB_OFF_B1 + d * LB_H + i];
+#pragma unroll
+        for (int m = 0; m < LB_NIN; ++m) a = fmaf(w[LB_OFF_W1 + i * LB_NIN + m], sx[tid][m], a);
+        sh1[tid][i] = fmaxf(a, 0.f);
+    }
+    float y = w[LB_OFF_B3];
+    for (int i = 0; i < LB_H; ++i) {
+        float a = w[LB_OFF_B2 + i];
+#pragma unroll 8
+        for (int j = 0; j < LB_H; ++j) a = fmaf(w[LB_OFF_W2 + i * LB_H + j], sh1[tid][j], a);
+        const float h2 = fmaxf(a, 0.f);
+        sh2[tid][i] = h2;
+        y = fmaf(w[LB_OFF_W3 + i], h2, y);
+    }
+    // backward
+    const float e = valid ? y - t : 0.f;
+    const float dy = 2.f * B.inv_total * e;
+    sloss[tid] = e * e;
+    sdy[tid] = dy;
+    sdep[tid] = d;
+    for (int i = 0; i < LB_H; ++i) sd2[tid][i] = sh2[tid][i] > 0.f ? dy * w[LB_OFF_W3 + i] : 0.f;
+    for (int j = 0; j < LB_H; ++j) {  // W2^T d2, masked by relu'
+        float a = 0.f;
+#pragma unroll 8
+        for (int i = 0; i < LB_H; ++i) a = fmaf(w[LB_OFF_W2 + i * LB_H + j], sd2[tid][i], a);
+        sd1[tid][j] = sh1[tid][j] > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float sum = 0.f;
+        for (int k = 0; k < LB_TB; ++k) sum += sloss[k];
+        atomicAdd(loss, sum * B.inv_total);
+    }
+    float *gf = grad + f * LB_NP;
+    for (int p = tid; p < LB_NP; p += LB_TB) {
+        float g = 0.f;
+        if (p < LB_OFF_B1) {
+            const int i = p / LB_NIN, m = p % LB_NIN;
+            for (int k = 0; k < LB_TB; ++k) g = fmaf(sd1[k][i], sx[k][m], g);
+        } else if (p < LB_OFF_W2) {
+            const int dd = (p - LB_OFF_B1) / LB_H, i = (p - LB_OFF_B1) % LB_H;
+            for (int k = 0; k < LB_TB; ++k) g += (sdep[k] == dd) ? sd1[k][i] : 0.f;
+        } else if (p < LB_OFF_B2) {
+            const int i = (p - LB_OFF_W2) / LB_H, j = (p - LB_OFF_W2) % LB_H;
+            for (int k = 0; k < LB_TB; ++k) g = fmaf(sd2[k][i], sh1[k][j], g);
+        } else if (p < LB_OFF_W3) {
+            const int i = p - LB_OFF_B2;
+            for (int k = 0; k < LB_TB; ++k) g += sd2[k][i];
+        } else if (p < LB_OFF_B3) {
+            const int i = p - LB_OFF_W3;
+            for (int k = 0; k < LB_TB; ++k) g = fmaf(sdy[k], sh2[k][i], g);
+        } else {
+            for (int k = 0; k < LB_TB; ++k) g += sdy[k];
+        }
+        atomicAdd(gf + p, g);
+    }
+}
+
+// Random minibatch positions: idx[f][n] uniform in [0, count_f).
+__global__ void lb_random_idx_kernel(int *idx, int n, int count, uint64_t seed)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) idx[t] = (int)(lb_hash(seed + 0x632be59bd9b4e019ull * (uint64_t)(t + 1)) % (uint32_t)count);
+}
+
+// Adam step on every parameter; clears the gradient.
+__global__ void lb_adam_kernel(float *theta, float *grad, float *m, float *v, int n, float lr, float b1, float b2,
+                               float c1, float c2, float eps)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float g = grad[t];
+    const float mt = b1 * m[t] + (1.f - b1) * g;
+    const float vt = b2 * v[t] + (1.f - b2) * g * g;
+    m[t] = mt;
+    v[t] = vt;
+    theta[t] -= lr * (mt * c1) / (sqrtf(vt * c2) + eps);
+    grad[t] = 0.f;
+}
+
+__global__ void lb_fill_kernel(float *p, int n, float val)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = val;
+}
+
+__global__ void lb_to_bf16_kernel(const float *theta, uint16_t *w, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) w[t] = f2bf(theta[t]);
+}
+
+// ------------------------------------------------------------ inference
+// MFMA 32x32x8 (bf16 in, FP32 accumulate), one wavefront per 32 points of one
+// face.  The network is evaluated transposed, H^T = W x^T, so that lane l
+// always holds point l%32: operand A = a weight matrix (lane l: row l%32,
+// k = 4*(l/32)+0..3 of the k-chunk), operand B = activations (lane l: column
+// l%32, the same k), result (lane l: column l%32, rows 8*(v/4) + 4*(l/32) +
+// v%4 for v = 0..15).  The rows a lane receives for chunk c of the next layer
+// are exactly the result rows it holds with v/4 = c, so activations never
+// leave the lane.  Activations (inputs and hidden) are split into a bf16 high
+// part and a bf16 remainder, two products each, so they carry ~16 bits.
+typedef short lb_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float lb_f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ inline void lb_split(const float *a, lb_bf16x4 &hi, lb_bf16x4 &lo)
+{
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint16_t h = f2bf(a[e]);
+        hi[e] = (short)h;
+        lo[e] = (short)f2bf(a[e] - bf2f(h));
+    }
+}
+
+// y for the wave's 32 points.  x: this lane's 12 inputs (k = 8c + 4*half + e,
+// c = 0..2, e = 0..3); d: depth of this lane's point; w: the face's bf16
+// parameters.  Returns y (valid in both halves).
+__device__ inline float lb_mlp_wave(const uint16_t *__restrict__ w, const float xk[12], int d)
+{
+    const int lane = threadIdx.x & 63;
+    const int row = lane & 31, half = lane >> 5;
+    lb_f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    // layer 1: H1^T[32 x 32] = W1[32 x 24] x^T[24 x 32]
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        lb_bf16x4 a, bh, bl;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int k = 8 * c + 4 * half + e;
+            a[e] = (short)(k < LB_NIN ? w[LB_OFF_W1 + row * LB_NIN + k] : (uint16_t)0);
+        }
+        lb_split(xk + 4 * c, bh, bl);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bl, acc, 0, 0, 0);
+    }
+    float h1[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int i = 8 * (v >> 2) + 4 * half + (v & 3);
+        h1[v] = fmaxf(acc[v] + bf2f(w[LB_OFF_B1 + d * LB_H + i]), 0.f);
+        acc[v] = 0.f;
+    }
+    // layer 2: H2^T = W2 H1^T
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        lb_bf16x4 a, bh, bl;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) a[e] = (short)w[LB_OFF_W2 + row * LB_H + 8 * c + 4 * half + e];
+        lb_split(h1 + 4 * c, bh, bl);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bl, acc, 0, 0, 0);
+    }
+    float y = 0.f;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int i = 8 * (v >> 2) + 4 * half + (v & 3);
+        const float h2 = fmaxf(acc[v] + bf2f(w[LB_OFF_B2 + i]), 0.f);
+        y = fmaf(bf2f(w[LB_OFF_W3 + i]), h2, y);
+    }
+    y += __shfl_xor(y, 32);
+    return y + bf2f(w[LB_OFF_B3]);
+}
+
+// this lane's 12 of the 24 padded inputs: k = 8c + e (lanes 0-31) or
+// 8c + 4 + e (lanes 32-63)
+__device__ inline void lb_lane_inputs(const float x[LB_NIN], float xk[12])
+{
+    const bool upper = (threadIdx.x & 32) != 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int k0 = 8 * c + e, k1 = 8 * c + 4 + e;
+            const float v0 = k0 < LB_NIN ? x[k0] : 0.f;
+            const float v1 = k1 < LB_NIN ? x[k1] : 0.f;
+            xk[4 * c + e] = upper ? v1 : v0;
+        }
+}
+
+struct LbCorrectArgs {
+    LbGridView G;
+    const uint16_t *w;
+    float *corr;
+    int64_t off[LB_FACES + 1];   // compact offsets
+    int64_t wave0[LB_FACES + 1]; // first wave of each face
+    float h, c1;
+};
+
+// Correction pass: corr[face][d][u][v] = vp2 * c1 * s * y for every layer point.
+__global__ void __launch_bounds__(256) lb_correct_kernel(LbCorrectArgs A)
+{
+    const int64_t wv = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= A.wave0[LB_FACES]) return;  // uniform per wave
+    int f = 0;
+    while (f + 1 < LB_FACES && wv >= A.wave0[f + 1]) ++f;
+    const LbFaceGeom g = lb_face(A.G.n, f);
+    const int64_t cnt = A.off[f + 1] - A.off[f];
+    const int64_t idx = (wv - A.wave0[f]) * 32 + (threadIdx.x & 31);
+    const int64_t ci = idx < cnt ? idx : cnt - 1;
+    const int v = (int)(ci % g.nv);
+    const int u = (int)((ci / g.nv) % g.nu);
+    const int d = (int)(ci / ((int64_t)g.nv * g.nu));
+    float x[LB_NIN], xk[12];
+    int64_t center;
+    const float s = lb_inputs(A.G, g, d, u, v, A.h, A.c1, x, &center);
+    lb_lane_inputs(x, xk);
+    const float y = lb_mlp_wave(A.w + f * LB_NP, xk, d);
+    if ((threadIdx.x & 32) == 0 && idx < cnt) A.corr[A.off[f] + idx] = A.G.vp[center] * A.c1 * s * y;
+}
+
+// MFMA inference on given inputs (testing): y[n] for samples of one face.
+__global__ void __launch_bounds__(256) lb_infer_kernel(const uint16_t *w, const float *samples, int64_t n, float *y)
+{
+    const int64_t wv = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv * 32 >= n) return;
+    const int64_t idx = wv * 32 + (threadIdx.x & 31);
+    const int64_t ci = idx < n ? idx : n - 1;
+    float x[LB_NIN], xk[12];
+    for (int m = 0; m < LB_NIN; ++m) x[m] = samples[ci * LB_NSTRIDE + m];
+    const int d = min(max((int)samples[ci * LB_NSTRIDE + LB_NIN + 1], 0), LB_W - 1);
+    lb_lane_inputs(x, xk);
+    const float r = lb_mlp_wave(w, xk, d);
+    if ((threadIdx.x & 32) == 0 && idx < n) y[idx] = r;
+}
+
+// Apply pass: one thread per point of the union of the layers; adds the
+// corrections of every face whose layer holds the point.
+struct LbApplyArgs {
+    float *p;
+    FdLayout L;
+    const float *corr;
+    int64_t off[LB_FACES + 1];
+    int64_t nA, nB, nC;  // points in the i-layers, j-layers (minus i), k-layers (minus i, j)
+};
+
+__global__ void lb_apply_kernel(LbApplyArgs A)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = A.L.nx, ny = A.L.ny, nz = A.L.nz;
+    int i, j, k;
+    if (t < A.nA) {  // i in layers, all j, k
+        k = (int)(t % nz);
+        j = (int)((t / nz) % ny);
+        const int q = (int)(t / ((int64_t)nz * ny));
+        i = q < LB_W ? q : nx - 2 * LB_W + q;
+    } else if (t < A.nA + A.nB) {  // j in layers, i interior, all k
+        const int64_t r = t - A.nA;
+        k = (int)(r % nz);
+        const int q = (int)((r / nz) % (2 * LB_W));
+        i = LB_W + (int)(r / ((int64_t)nz * 2 * LB_W));
+        j = q < LB_W ? q : ny - 2 * LB_W + q;
+    } else if (t < A.nA + A.nB + A.nC) {  // k in layers, i and j interior
+        const int64_t r = t - A.nA - A.nB;
+        const int q = (int)(r % (2 * LB_W));
+        j = LB_W + (int)((r / (2 * LB_W)) % (ny - 2 * LB_W));
+        i = LB_W + (int)(r / ((int64_t)2 * LB_W * (ny - 2 * LB_W)));
+        k = q < LB_W ? q : nz - 2 * LB_W + q;
+    } else {
+        return;
+    }
+    const int n[3] = {nx, ny, nz};
+    const int c[3] = {i, j, k};
+    float add = 0.f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int side = -1, d = 0;
+        if (c[a] < LB_W) { side = 0; d = c[a]; }
+        else if (c[a] >= n[a] - LB_W) { side = 1; d = n[a] - 1 - c[a]; }
+        if (side < 0) continue;
+        const LbFaceGeom g = lb_face(n, 2 * a + side);
+        const int u = (a == 0) ? j : i;
+        const int v = (a == 2) ? j : k;
+        add += A.corr[A.off[2 * a + side] + ((int64_t)d * g.nu + u) * g.nv + v];
+    }
+    const int64_t o = fd_off(A.L, i, j, k);
+    A.p[o] = add_nc(A.p[o], add);
+}
+
+hipError_t lb_enqueue_correct(const LbInfer &lb, const FdLayout &L, const float *pn, const float *pnm1,
+                              const float *vp, hipStream_t stream)
+{
+    LbCorrectArgs A;
+    A.G.pn = pn;
+    A.G.pnm1 = pnm1;
+    A.G.vp = vp;
+    A.G.L = L;
+    A.G.o[0] = A.G.o[1] = A.G.o[2] = 0;
+    A.G.n[0] = L.nx;
+    A.G.n[1] = L.ny;
+    A.G.n[2] = L.nz;
+    A.w = lb.w;
+    A.corr = lb.corr;
+    A.h = lb.halo;
+    A.c1 = lb.c1;
+    A.wave0[0] = 0;
+    for (int f = 0; f <= LB_FACES; ++f) A.off[f] = lb.corr_off[f];
+    for (int f = 0; f < LB_FACES; ++f) A.wave0[f + 1] = A.wave0[f] + (A.off[f + 1] - A.off[f] + 31) / 32;
+    const int64_t blocks = (A.wave0[LB_FACES] + 3) / 4;
+    hipLaunchKernelGGL(lb_correct_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, A);
+    return hipGetLastError();
+}
+
+hipError_t lb_enqueue_apply(const LbInfer &lb, const FdLayout &L, float *pnew, hipStream_t stream)
+{
+    LbApplyArgs A;
+    A.p = pnew;
+    A.L = L;
+    A.corr = lb.corr;
+    for (int f = 0; f <= LB_FACES; ++f) A.off[f] = lb.corr_off[f];
+    A.nA = (int64_t)2 * LB_W * L.ny * L.nz;
+    A.nB = (int64_t)2 * LB_W * (L.nx - 2 * LB_W) * L.nz;
+    A.nC = (int64_t)2 * LB_W * (L.nx - 2 * LB_W) * (L.ny - 2 * LB_W);
+    const int64_t n = A.nA + A.nB + A.nC;
+    hipLaunchKernelGGL(lb_apply_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, A);
+    return hipGetLastError();
+}
+
+} // namespace phc
+
+using namespace phc;
+
+struct phc_lb {
+    DevBuf theta, grad, m, v, loss, wbf, idx;
+    DevBuf data[LB_FACES];
+    DevBuf count;  // unsigned long long per face
+    int64_t cap = 0;
+    int64_t adam_t = 0;
+    hipStream_t stream = nullptr;
+};
+
+static phc_status lb_upload_bf16(phc_lb *h)
+{
+    const int n = LB_FACES * LB_NP;
+    hipLaunchKernelGGL(lb_to_bf16_kernel, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->theta.as<float>(),
+                       h->wbf.as<uint16_t>(), n);
+    PHC_HIP(hipGetLastError());
+    return PHC_OK;
+}
+
+static phc_status lb_counts(phc_lb *h, int64_t c[LB_FACES])
+{
+    unsigned long long hc[LB_FACES];
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    PHC_HIP(hipMemcpy(hc, h->count.ptr, sizeof(hc), hipMemcpyDeviceToHost));
+    for (int f = 0; f < LB_FACES; ++f) c[f] = std::min<int64_t>((int64_t)hc[f], h->cap);
+    return PHC_OK;
+}
+
+// one forward/backward pass of a batch; loss is accumulated into h->loss
+static phc_status lb_launch_train(phc_lb *h, const LbBatch &B)
+{
+    if (B.block_start[LB_FACES] == 0) return PHC_OK;
+    hipLaunchKernelGGL(lb_train_kernel, dim3(B.block_start[LB_FACES]), dim3(LB_TB), 0, h->stream,
+                       h->theta.as<float>(), h->grad.as<float>(), h->loss.as<float>(), B);
+    PHC_HIP(hipGetLastError());
+    return PHC_OK;
+}
+
+static void lb_fill(phc_lb *h, float *p, int n, float val)
+{
+    hipLaunchKernelGGL(lb_fill_kernel, dim3((n + 255) / 256), dim3(256), 0, h->stream, p, n, val);
+}
+
+extern "C" {
+
+phc_status phc_lb_create(uint64_t seed, phc_lb **out)
+{
+    if (!out) return fail(PHC_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    phc_lb *h = new phc_lb();
+    auto bail = [&](phc_status s) { phc_lb_destroy(h); return s; };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(PHC_ERR_HIP, "hipStreamCreate failed"));
+    const size_t n = LB_FACES * LB_NP;
+    phc_status st;
+    if ((st = h->theta.alloc(4 * n)) || (st = h->grad.alloc(4 * n)) || (st = h->m.alloc(4 * n)) ||
+        (st = h->v.alloc(4 * n)) || (st = h->loss.alloc(4)) || (st = h->wbf.alloc(2 * n)) ||
+        (st = h->count.alloc(8 * LB_FACES)))
+        return bail(st);
+    // He-uniform initial weights, zero biases, small output layer
+    std::mt19937_64 rng(seed);
+    std::vector<float> th(n, 0.f);
+    for (int f = 0; f < LB_FACES; ++f) {
+        float *p = th.data() + f * LB_NP;
+        std::uniform_real_distribution<float> u1(-std::sqrt(6.f / LB_NIN), std::sqrt(6.f / LB_NIN));
+        std::uniform_real_distribution<float> u2(-std::sqrt(6.f / LB_H), std::sqrt(6.f / LB_H));
+        std::uniform_real_distribution<float> u3(-0.1f / std::sqrt((float)LB_H), 0.1f / std::sqrt((float)LB_H));
+        for (int q = 0; q < LB_H * LB_NIN; ++q) p[LB_OFF_W1 + q] = u1(rng);
+        for (int q = 0; q < LB_H * LB_H; ++q) p[LB_OFF_W2 + q] = u2(rng);
+        for (int q = 0; q < LB_H; ++q) p[LB_OFF_W3 + q] = u3(rng);
+    }
+    if (hipMemcpy(h->theta.ptr, th.data(), 4 * n, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(PHC_ERR_HIP, "upload failed"));
+    std::vector<float> z(n, 0.f);
+    if (hipMemcpy(h->grad.ptr, z.data(), 4 * n, hipMemcpyHostToDevice) ||
+        hipMemcpy(h->m.ptr, z.data(), 4 * n, hipMemcpyHostToDevice) ||
+        hipMemcpy(h->v.ptr, z.data(), 4 * n, hipMemcpyHostToDevice) ||
+        hipMemset(h->count.ptr, 0, 8 * LB_FACES) != hipSuccess)
+        return bail(fail(PHC_ERR_HIP, "upload failed"));
+    if ((st = lb_upload_bf16(h))) return bail(st);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fail(PHC_ERR_HIP, "sync failed"));
+    *out = h;
+    return PHC_OK;
+}
+
+phc_status phc_lb_destroy(phc_lb *h)
+{
+    if (!h) return PHC_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (DevBuf *b : {&h->theta, &h->grad, &h->m, &h->v, &h->loss, &h->wbf, &h->idx, &h->count}) b->release();
+    for (auto &b : h->data) b.release();
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return PHC_OK;
+}
+
+phc_status phc_lb_get_params(phc_lb *h, float *host)
+{
+    if (!h || !host) return fail(PHC_ERR_INVALID, "NULL argument");
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    PHC_HIP(hipMemcpy(host, h->theta.ptr, 4 * LB_FACES * LB_NP, hipMemcpyDeviceToHost));
+    return PHC_OK;
+}
+
+phc_status phc_lb_set_params(phc_lb *h, const float *host)
+{
+    if (!h || !host) return fail(PHC_ERR_INVALID, "NULL argument");
+    for (int q = 0; q < LB_FACES * LB_NP; ++q)
+        if (!std::isfinite(host[q])) return fail(PHC_ERR_INVALID, "parameter %d is not finite", q);
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    PHC_HIP(hipMemcpy(h->theta.ptr, host, 4 * LB_FACES * LB_NP, hipMemcpyHostToDevice));
+    PHC_TRY(lb_upload_bf16(h));
+    h->adam_t = 0;
+    std::vector<float> z(LB_FACES * LB_NP, 0.f);
+    PHC_HIP(hipMemcpy(h->m.ptr, z.data(), 4 * z.size(), hipMemcpyHostToDevice));
+    PHC_HIP(hipMemcpy(h->v.ptr, z.data(), 4 * z.size(), hipMemcpyHostToDevice));
+    PHC_HIP(hipMemcpy(h->grad.ptr, z.data(), 4 * z.size(), hipMemcpyHostToDevice));
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    return PHC_OK;
+}
+
+phc_status phc_lb_set_capacity(phc_lb *h, int64_t per_face)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    if (per_face < 1 || per_face > ((int64_t)1 << 31) - 1)
+        return fail(PHC_ERR_INVALID, "capacity %lld outside 1..2^31-1", (long long)per_face);
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    for (auto &b : h->data) PHC_TRY(b.alloc(sizeof(float) * LB_NSTRIDE * (size_t)per_face));
+    h->cap = per_face;
+    PHC_HIP(hipMemset(h->count.ptr, 0, 8 * LB_FACES));
+    return PHC_OK;
+}
+
+phc_status phc_lb_clear_samples(phc_lb *h)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    PHC_HIP(hipMemset(h->count.ptr, 0, 8 * LB_FACES));
+    return PHC_OK;
+}
+
+phc_status phc_lb_sample_counts(phc_lb *h, int64_t *counts)
+{
+    if (!h || !counts) return fail(PHC_ERR_INVALID, "NULL argument");
+    return lb_counts(h, counts);
+}
+
+phc_status phc_lb_get_samples(phc_lb *h, int face, int64_t first, int64_t n, float *host)
+{
+    if (!h || !host) return fail(PHC_ERR_INVALID, "NULL argument");
+    if (face < 0 || face >= LB_FACES) return fail(PHC_ERR_INVALID, "face %d outside 0..5", face);
+    int64_t c[LB_FACES];
+    PHC_TRY(lb_counts(h, c));
+    if (first < 0 || n < 0 || first + n > c[face])
+        return fail(PHC_ERR_INVALID, "samples %lld..%lld outside the %lld of face %d", (long long)first,
+                    (long long)(first + n), (long long)c[face], face);
+    if (n > 0)
+        PHC_HIP(hipMemcpy(host, h->data[face].as<float>() + first * LB_NSTRIDE, 4 * LB_NSTRIDE * n,
+                          hipMemcpyDeviceToHost));
+    return PHC_OK;
+}
+
+phc_status phc_lb_add_samples(phc_lb *h, int face, const float *host, int64_t n)
+{
+    if (!h || (!host && n > 0)) return fail(PHC_ERR_INVALID, "NULL argument");
+    if (face < 0 || face >= LB_FACES) return fail(PHC_ERR_INVALID, "face %d outside 0..5", face);
+    int64_t c[LB_FACES];
+    PHC_TRY(lb_counts(h, c));
+    if (n < 0 || c[face] + n > h->cap)
+        return fail(PHC_ERR_INVALID, "%lld samples exceed the capacity %lld", (long long)(c[face] + n),
+                    (long long)h->cap);
+    if (n > 0)
+        PHC_HIP(hipMemcpy(h->data[face].as<float>() + c[face] * LB_NSTRIDE, host, 4 * LB_NSTRIDE * n,
+                          hipMemcpyHostToDevice));
+    const unsigned long long nc = (unsigned long long)(c[face] + n);
+    PHC_HIP(hipMemcpy(h->count.as<unsigned long long>() + face, &nc, 8, hipMemcpyHostToDevice));
+    return PHC_OK;
+}
+
+phc_status phc_lb_collect(phc_lb *h, phc_fd *wide, int cur, const int *origin, const int *dims, float halo,
+                          float keep, float s_min, uint64_t seed)
+{
+    if (!h || !wide || !origin || !dims) return fail(PHC_ERR_INVALID, "NULL argument");
+    if (cur < 0 || cur > 1) return fail(PHC_ERR_INVALID, "cur level %d is not 0 or 1", cur);
+    if (h->cap < 1) return fail(PHC_ERR_STATE, "no sample store: call phc_lb_set_capacity first");
+    FdFieldView fv;
+    PHC_TRY(fd_field_view(wide, &fv));
+    const int N[3] = {fv.L.nx, fv.L.ny, fv.L.nz};
+    for (int a = 0; a < 3; ++a) {
+        if (dims[a] < 2 * LB_W + 1) return fail(PHC_ERR_INVALID, "region side %d too small", dims[a]);
+        // the region, its outward stencil reach and the update's reach lie outside the sponge
+        if (origin[a] - LB_R < fv.sw || origin[a] + dims[a] + LB_R > N[a] - fv.sw)
+            return fail(PHC_ERR_INVALID, "region plus %d points must stay outside the %d-point sponge", LB_R,
+                        fv.sw);
+    }
+    LbCollectArgs A;
+    A.G.pn = fv.lev[cur];
+    A.G.pnm1 = fv.lev[1 - cur];
+    A.G.vp = fv.vp;
+    A.G.L = fv.L;
+    for (int a = 0; a < 3; ++a) {
+        A.G.o[a] = origin[a];
+        A.G.n[a] = dims[a];
+    }
+    A.h = halo;
+    A.c1 = fv.c1;
+    for (int l = 0; l <= LB_R; ++l) A.chat[l] = fv.chat[l];
+    A.keep = keep;
+    A.s_min = s_min;
+    A.seed = seed;
+    for (int f = 0; f < LB_FACES; ++f) A.out[f] = h->data[f].as<float>();
+    A.count = h->count.as<unsigned long long>();
+    A.cap = h->cap;
+    A.face_off[0] = 0;
+    for (int f = 0; f < LB_FACES; ++f) {
+        const LbFaceGeom g = lb_face(A.G.n, f);
+        A.face_off[f + 1] = A.face_off[f] + (int64_t)LB_W * g.nu * g.nv;
+    }
+    PHC_HIP(hipStreamSynchronize(fv.stream));
+    const int64_t n = A.face_off[LB_FACES];
+    hipLaunchKernelGGL(lb_collect_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, A);
+    PHC_HIP(hipGetLastError());
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    return PHC_OK;
+}
+
+// Loss (mean squared error of y against the targets over every stored
+// sample) and, if grad is not NULL, its gradient with respect to the FP32
+// parameters (evaluated at their bf16 values).
+phc_status phc_lb_eval(phc_lb *h, float *loss, float *grad)
+{
+    if (!h || !loss) return fail(PHC_ERR_INVALID, "NULL argument");
+    int64_t c[LB_FACES];
+    PHC_TRY(lb_counts(h, c));
+    int64_t tot = 0;
+    for (int f = 0; f < LB_FACES; ++f) tot += c[f];
+    if (tot == 0) return fail(PHC_ERR_STATE, "no samples");
+    LbBatch B;
+    B.block_start[0] = 0;
+    for (int f = 0; f < LB_FACES; ++f) {
+        B.samples[f] = h->data[f].as<float>();
+        B.idx[f] = nullptr;
+        B.count[f] = (int)c[f];
+        B.block_start[f + 1] = B.block_start[f] + (int)((c[f] + LB_TB - 1) / LB_TB);
+    }
+    B.inv_total = 1.f / (float)tot;
+    const int np = LB_FACES * LB_NP;
+    DevBuf saved;
+    PHC_TRY(saved.alloc(4 * (size_t)np));  // keep the gradient a training run may be accumulating
+    PHC_HIP(hipMemcpyAsync(saved.ptr, h->grad.ptr, 4 * np, hipMemcpyDeviceToDevice, h->stream));
+    lb_fill(h, h->grad.as<float>(), np, 0.f);
+    lb_fill(h, h->loss.as<float>(), 1, 0.f);
+    PHC_TRY(lb_launch_train(h, B));
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    PHC_HIP(hipMemcpy(loss, h->loss.ptr, 4, hipMemcpyDeviceToHost));
+    if (grad) PHC_HIP(hipMemcpy(grad, h->grad.ptr, 4 * np, hipMemcpyDeviceToHost));
+    PHC_HIP(hipMemcpy(h->grad.ptr, saved.ptr, 4 * np, hipMemcpyDeviceToDevice));
+    return PHC_OK;
+}
+
+// Adam on minibatches of batch_per_face random samples of each face;
+// loss_history[s] (may be NULL) = minibatch loss before step s.
+phc_status phc_lb_train(phc_lb *h, int steps, int batch_per_face, float lr, uint64_t seed, float *loss_history)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    if (steps < 0 || batch_per_face < 1 || batch_per_face > (1 << 24) || !(lr > 0.f))
+        return fail(PHC_ERR_INVALID, "bad training parameters");
+    int64_t c[LB_FACES];
+    PHC_TRY(lb_counts(h, c));
+    int nf = 0;
+    for (int f = 0; f < LB_FACES; ++f) nf += c[f] > 0;
+    if (nf == 0) return fail(PHC_ERR_STATE, "no samples");
+    PHC_TRY(h->idx.alloc(sizeof(int) * (size_t)batch_per_face * LB_FACES));
+    LbBatch B;
+    B.block_start[0] = 0;
+    for (int f = 0; f < LB_FACES; ++f) {
+        B.samples[f] = h->data[f].as<float>();
+        B.idx[f] = h->idx.as<int>() + (size_t)f * batch_per_face;
+        B.count[f] = c[f] > 0 ? batch_per_face : 0;
+        B.block_start[f + 1] = B.block_start[f] + (B.count[f] + LB_TB - 1) / LB_TB;
+    }
+    B.inv_total = 1.f / ((float)batch_per_face * nf);
+    const int np = LB_FACES * LB_NP;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    DevBuf lh;
+    PHC_TRY(lh.alloc(sizeof(float) * std::max(steps, 1)));
+    lb_fill(h, h->grad.as<float>(), np, 0.f);
+    for (int s = 0; s < steps; ++s) {
+        for (int f = 0; f < LB_FACES; ++f)
+            if (c[f] > 0)
+                hipLaunchKernelGGL(lb_random_idx_kernel, dim3((batch_per_face + 255) / 256), dim3(256), 0, h->stream,
+                                   h->idx.as<int>() + (size_t)f * batch_per_face, batch_per_face, (int)c[f],
+                                   seed * 1000003ull + (uint64_t)s * LB_FACES + f);
+        lb_fill(h, lh.as<float>() + s, 1, 0.f);
+        hipLaunchKernelGGL(lb_train_kernel, dim3(B.block_start[LB_FACES]), dim3(LB_TB), 0, h->stream,
+                           h->theta.as<float>(), h->grad.as<float>(), lh.as<float>() + s, B);
+        ++h->adam_t;
+        const float c1 = 1.f / (1.f - std::pow(b1, (float)h->adam_t));
+        const float c2 = 1.f / (1.f - std::pow(b2, (float)h->adam_t));
+        hipLaunchKernelGGL(lb_adam_kernel, dim3((np + 255) / 256), dim3(256), 0, h->stream, h->theta.as<float>(),
+                           h->grad.as<float>(), h->m.as<float>(), h->v.as<float>(), np, lr, b1, b2, c1, c2, eps);
+    }
+    PHC_HIP(hipGetLastError());
+    PHC_TRY(lb_upload_bf16(h));
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    if (loss_history && steps > 0)
+        PHC_HIP(hipMemcpy(loss_history, lh.ptr, sizeof(float) * steps, hipMemcpyDeviceToHost));
+    return PHC_OK;
+}
+
+phc_status phc_lb_infer(phc_lb *h, int face, const float *samples, int64_t n, float *y)
+{
+    if (!h || !samples || !y) return fail(PHC_ERR_INVALID, "NULL argument");
+    if (face < 0 || face >= LB_FACES) return fail(PHC_ERR_INVALID, "face %d outside 0..5", face);
+    if (n < 1 || n > ((int64_t)1 << 30)) return fail(PHC_ERR_INVALID, "bad sample count");
+    DevBuf s, out;
+    PHC_TRY(s.alloc(4 * LB_NSTRIDE * (size_t)n));
+    PHC_TRY(out.alloc(4 * (size_t)n));
+    PHC_HIP(hipMemcpy(s.ptr, samples, 4 * LB_NSTRIDE * (size_t)n, hipMemcpyHostToDevice));
+    const int64_t waves = (n + 31) / 32;
+    hipLaunchKernelGGL(lb_infer_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, h->stream,
+                       h->wbf.as<uint16_t>() + face * LB_NP, s.as<float>(), n, out.as<float>());
+    PHC_HIP(hipGetLastError());
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    PHC_HIP(hipMemcpy(y, out.ptr, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    return PHC_OK;
+}
+
+} // extern "C"
+
+// bf16 parameters of a model, for an FD handle (fd.hip)
+phc_status phc::lb_export_bf16(const phc_lb *h, uint16_t *dev_dst, hipStream_t stream)
+{
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    PHC_HIP(hipMemcpyAsync(dev_dst, h->wbf.ptr, 2 * LB_FACES * LB_NP, hipMemcpyDeviceToDevice, stream));
+    PHC_HIP(hipStreamSynchronize(stream));
+    return PHC_OK;
+}

@@ -208,6 +208,7 @@ class FdSolver:
         h = c_void_p()
         check(lib().phc_fd_create(ctypes.byref(p), ctypes.byref(h)))
         self.h = h
+        self.init_value = init_value
         self.i1, self.i2 = 0, 1   # FDTDInit::i1/i2 (FDTDinit.hpp:28-29)
         self.nrec = 0
 
@@ -289,6 +290,18 @@ class FdSolver:
 
     def synchronize(self):
         check(lib().phc_fd_synchronize(self.h))
+
+    def set_learned_boundary(self, lb, halo=None):
+        """Attach a LearnedBoundary (None detaches); halo defaults to init_value."""
+        h = self.init_value if halo is None else halo
+        check(lib().phc_fd_set_learned_boundary(self.h, lb.h if lb is not None else None, h))
+
+    def lb_corrections(self, cur=None):
+        """Per-face corrections the next step would add (list of 6 arrays)."""
+        sizes = lb_corr_sizes(self.nx, self.ny, self.nz)
+        out = np.zeros(sum(sizes), np.float32)
+        check(lib().phc_fd_lb_corrections(self.h, self.i2 if cur is None else cur, out))
+        return np.split(out, np.cumsum(sizes)[:-1])
 
     def autotune(self, steps_per_trial=10, cache_path=None):
         """Pick the fastest block shape for this grid; returns (tile_j, waves, chunk_i)."""
@@ -436,6 +449,117 @@ class FdBatch:
 
     def points(self):
         return int(np.prod(self.dims, axis=1, dtype=np.int64).sum())
+
+
+# --------------------------------------------------------- learned boundary
+LB_WIDTH = 4
+LB_FACES = 6
+LB_NP = 1825
+LB_SAMPLE = 22
+i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+SIGNATURES.update({
+    "phc_lb_create": (c_int, [ctypes.c_uint64, ctypes.POINTER(c_void_p)]),
+    "phc_lb_destroy": (c_int, [c_void_p]),
+    "phc_lb_get_params": (c_int, [c_void_p, f32p]),
+    "phc_lb_set_params": (c_int, [c_void_p, f32p]),
+    "phc_lb_set_capacity": (c_int, [c_void_p, ctypes.c_int64]),
+    "phc_lb_clear_samples": (c_int, [c_void_p]),
+    "phc_lb_sample_counts": (c_int, [c_void_p, i64p]),
+    "phc_lb_get_samples": (c_int, [c_void_p, c_int, ctypes.c_int64, ctypes.c_int64, f32p]),
+    "phc_lb_add_samples": (c_int, [c_void_p, c_int, f32p, ctypes.c_int64]),
+    "phc_lb_collect": (c_int, [c_void_p, c_void_p, c_int, i32p, i32p, c_float, c_float, c_float,
+                               ctypes.c_uint64]),
+    "phc_lb_eval": (c_int, [c_void_p, ctypes.POINTER(c_float), c_void_p]),
+    "phc_lb_train": (c_int, [c_void_p, c_int, c_int, c_float, ctypes.c_uint64, c_void_p]),
+    "phc_lb_infer": (c_int, [c_void_p, c_int, f32p, ctypes.c_int64, f32p]),
+    "phc_fd_set_learned_boundary": (c_int, [c_void_p, c_void_p, c_float]),
+    "phc_fd_lb_corrections": (c_int, [c_void_p, c_int, f32p]),
+})
+
+
+class LearnedBoundary:
+    """Per-face MLP of the thin learned absorbing layer (phc_lb_*): parameters,
+    training-sample store, GPU training and inference."""
+
+    def __init__(self, seed=1):
+        h = c_void_p()
+        check(lib().phc_lb_create(seed, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            check(lib().phc_lb_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(LB_FACES * LB_NP, np.float32)
+        check(lib().phc_lb_get_params(self.h, out))
+        return out.reshape(LB_FACES, LB_NP)
+
+    def set_params(self, theta):
+        check(lib().phc_lb_set_params(self.h, np.ascontiguousarray(theta, np.float32).reshape(-1)))
+
+    def set_capacity(self, per_face):
+        check(lib().phc_lb_set_capacity(self.h, int(per_face)))
+
+    def clear_samples(self):
+        check(lib().phc_lb_clear_samples(self.h))
+
+    def counts(self):
+        c = np.zeros(LB_FACES, np.int64)
+        check(lib().phc_lb_sample_counts(self.h, c))
+        return c
+
+    def samples(self, face):
+        n = int(self.counts()[face])
+        out = np.zeros(n * LB_SAMPLE, np.float32)
+        if n:
+            check(lib().phc_lb_get_samples(self.h, face, 0, n, out))
+        return out.reshape(n, LB_SAMPLE)
+
+    def add_samples(self, face, arr):
+        a = np.ascontiguousarray(arr, np.float32).reshape(-1, LB_SAMPLE)
+        check(lib().phc_lb_add_samples(self.h, face, a.reshape(-1), len(a)))
+
+    def collect(self, wide, cur, origin, dims, halo=0.0, keep=1.0, s_min=0.0, seed=1):
+        """Samples from a wide-sponge FdSolver's levels (cur = current level)."""
+        check(lib().phc_lb_collect(self.h, wide.h, cur, np.asarray(origin, np.int32),
+                                   np.asarray(dims, np.int32), halo, keep, s_min, seed))
+
+    def eval(self, with_grad=True):
+        loss = c_float()
+        g = np.zeros(LB_FACES * LB_NP, np.float32) if with_grad else None
+        check(lib().phc_lb_eval(self.h, ctypes.byref(loss), _ptr(g)))
+        return loss.value, (g.reshape(LB_FACES, LB_NP) if with_grad else None)
+
+    def train(self, steps, batch_per_face=4096, lr=1e-3, seed=1):
+        hist = np.zeros(max(steps, 1), np.float32)
+        check(lib().phc_lb_train(self.h, steps, batch_per_face, lr, seed, _ptr(hist)))
+        return hist[:steps]
+
+    def infer(self, face, samples):
+        a = np.ascontiguousarray(samples, np.float32).reshape(-1, LB_SAMPLE)
+        y = np.zeros(len(a), np.float32)
+        check(lib().phc_lb_infer(self.h, face, a.reshape(-1), len(a), y))
+        return y
+
+
+def lb_corr_sizes(nx, ny, nz):
+    n = (nx, ny, nz)
+    sizes = []
+    for f in range(LB_FACES):
+        a = f >> 1
+        nu = n[1] if a == 0 else n[0]
+        nv = n[1] if a == 2 else n[2]
+        sizes.append(LB_WIDTH * nu * nv)
+    return sizes
 
 
 # ----------------------------------------------------------------------- SEM

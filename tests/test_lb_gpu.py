@@ -1,0 +1,179 @@
+"""Learned thin boundary on the GPU against the FP64 oracle (oracle/lb_oracle.py):
+the training kernels' loss and gradient, the MFMA inference, the collection of
+training samples from a wide-sponge run, the correction pass of a thin run,
+and a few learned-boundary steps against a CPU emulation."""
+import numpy as np
+import pytest
+
+import lb_oracle as lo
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print("\n[lb-gpu] " + msg, flush=True)
+
+
+def random_samples(rng, counts):
+    out = []
+    for n in counts:
+        S = np.zeros((n, lo.NSTRIDE), np.float32)
+        S[:, :18] = rng.uniform(-1, 1, (n, 18))
+        S[:, 18] = rng.uniform(0.1, 0.5, n)
+        S[:, 19] = rng.normal(0, 0.5, n)
+        S[:, 20] = rng.integers(0, lo.W, n)
+        out.append(S)
+    return out
+
+
+def test_loss_and_gradient_match_oracle(gpu):
+    banner("eval loss/grad vs oracle")
+    rng = np.random.default_rng(11)
+    lb = gpu.LearnedBoundary(seed=3)
+    samples = random_samples(rng, [300, 17, 0, 129, 64, 1])
+    lb.set_capacity(512)
+    for f, S in enumerate(samples):
+        if len(S):
+            lb.add_samples(f, S)
+    assert list(lb.counts()) == [len(S) for S in samples]
+    theta = lb.params()
+    loss, g = lb.eval()
+    oloss, og = lo.loss_grad(theta, samples)
+    assert loss == pytest.approx(oloss, rel=1e-5)
+    assert np.linalg.norm(g - og) <= 1e-4 * np.linalg.norm(og)
+    for f in range(lo.FACES):
+        if len(samples[f]) > 16:
+            assert np.linalg.norm(g[f] - og[f]) <= 1e-4 * np.linalg.norm(og[f]), f
+    assert np.all(g[2] == 0)
+
+
+def test_mfma_inference_matches_oracle(gpu):
+    banner("MFMA inference vs oracle")
+    rng = np.random.default_rng(12)
+    lb = gpu.LearnedBoundary(seed=5)
+    theta = rng.normal(0, 0.4, (lo.FACES, lo.NP)).astype(np.float32)
+    lb.set_params(theta)
+    for f, n in ((0, 1), (3, 31), (4, 33), (5, 1000)):
+        S = random_samples(rng, [n])[0]
+        y = lb.infer(f, S)
+        oy, _ = lo.forward(lo.bf16(theta[f]), S[:, :lo.NIN].astype(np.float64),
+                           S[:, lo.NIN + 1].astype(int))
+        scale = max(np.max(np.abs(oy)), 1e-30)
+        assert np.max(np.abs(y - oy)) <= 2e-4 * scale, (f, n, np.max(np.abs(y - oy)) / scale)
+
+
+def wide_run(gpu, n_wide, sponge, steps, src, seed=2, nl=3):
+    s = gpu.FdSetup(n_wide, n_wide, n_wide, L=4, xs=src[0], ys=src[1], zs=src[2], f0=20.0, cfl=0.4,
+                    sponge_width=sponge, model=("layered", nl, seed), nsamples=steps + 1)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_FP32, init_value=0.0)
+    sol.run(0, steps)
+    return s, sol
+
+
+def test_collected_samples_match_oracle(gpu):
+    banner("collection vs oracle")
+    s, sol = wide_run(gpu, 56, 10, 70, (28, 26, 30))
+    o, n = (15, 14, 16), (26, 25, 24)
+    lb = gpu.LearnedBoundary(seed=1)
+    lb.set_capacity(20000)
+    lb.collect(sol, sol.i2, o, n, halo=0.0, keep=1.0, s_min=0.0, seed=7)
+    pn, pm = sol.current().astype(np.float64), sol.previous().astype(np.float64)
+    vp3 = s.vp.reshape(s.nx, s.ny, s.nz)
+    c1 = float(s.cx[1])
+    chat = s.cx.astype(np.float64) / c1
+    total = 0
+    for f in range(lo.FACES):
+        S = lb.samples(f)
+        X, sc, d, v, t = lo.layer_inputs(pn, pm, vp3, o, n, f, 0.0, c1, chat)
+        keep = sc > lo.S_TINY
+        assert len(S) == int(keep.sum()), f
+        idx = S[:, 21].astype(int)
+        assert len(np.unique(idx)) == len(idx)
+        assert np.all(keep[idx])
+        assert np.allclose(S[:, :lo.NIN], X[idx], rtol=1e-5, atol=1e-6)
+        assert np.allclose(S[:, lo.NIN], t[idx], rtol=1e-4, atol=1e-5)
+        assert np.array_equal(S[:, lo.NIN + 1].astype(int), d[idx])
+        total += len(S)
+    assert total > 1000
+    # subsampling keeps about the requested fraction
+    lb.clear_samples()
+    lb.collect(sol, sol.i2, o, n, keep=0.25, seed=9)
+    assert 0.15 * total < lb.counts().sum() < 0.35 * total
+
+
+def thin_solver(gpu, n, steps, seed=4, src=None):
+    src = src or (n // 2, n // 2 - 1, n // 2 + 1)
+    s = gpu.FdSetup(n, n, n, L=4, xs=src[0], ys=src[1], zs=src[2], f0=25.0, cfl=0.4, sponge_width=0,
+                    model=("layered", 3, seed), nsamples=steps + 1)
+    return s
+
+
+def test_correction_pass_matches_oracle(gpu):
+    banner("correction pass vs oracle")
+    rng = np.random.default_rng(13)
+    s = thin_solver(gpu, 40, 60)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_FP32, init_value=0.0)
+    sol.run(0, 45)           # a wavefield that reaches the layer
+    lb = gpu.LearnedBoundary(seed=2)
+    theta = rng.normal(0, 0.3, (lo.FACES, lo.NP)).astype(np.float32)
+    lb.set_params(theta)
+    sol.set_learned_boundary(lb, halo=0.0)
+    got = sol.lb_corrections()
+    pn, pm = sol.current().astype(np.float64), sol.previous().astype(np.float64)
+    vp3 = s.vp.reshape(s.nx, s.ny, s.nz)
+    ref = lo.corrections(theta, pn, pm, vp3, (s.nx, s.ny, s.nz), 0.0, float(s.cx[1]))
+    for f in range(lo.FACES):
+        scale = max(np.max(np.abs(ref[f])), 1e-30)
+        assert np.max(np.abs(got[f] - ref[f])) <= 3e-4 * scale, f
+        assert np.max(np.abs(ref[f])) > 0
+
+
+def test_learned_steps_match_cpu_emulation(gpu, oracle):
+    """REF arithmetic on both sides: the stencil is bit-exact, the corrections
+    agree to the MFMA inference tolerance."""
+    banner("learned steps vs CPU emulation")
+    rng = np.random.default_rng(14)
+    n = 36
+    s = thin_solver(gpu, n, 80, src=(18, 17, 19))
+    theta = (rng.normal(0, 0.3, (lo.FACES, lo.NP)) * 0.5).astype(np.float32)
+    lb = gpu.LearnedBoundary(seed=2)
+    lb.set_params(theta)
+    P = oracle.FdProblem(nx=n, L=4, xs=s.xs, ys=s.ys, zs=s.zs, f0=s.f0, cfl=0.4, model=s.vp.copy(),
+                         nsamples=s.nsamples, sponge_width=0)
+    assert np.array_equal(P.rhs, s.rhs)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF, init_value=1e-6)
+    sol.set_learned_boundary(lb)   # halo = init_value
+    vp3 = s.vp.reshape(n, n, n)
+    c1 = float(s.cx[1])
+    nsteps = 70
+    for it in range(nsteps):
+        corr = lo.corrections(theta, P.current().astype(np.float64), P.previous().astype(np.float64), vp3,
+                              (n, n, n), np.float64(np.float32(1e-6)), c1)
+        P.run(1)
+        cur = P.current()
+        upd = cur.astype(np.float64)
+        lo.apply_corrections(upd, corr, (n, n, n))
+        cur[...] = upd.astype(np.float32)
+        sol.step(it)
+    g, r = sol.current(), P.current()
+    rel = np.linalg.norm((g - r).astype(np.float64)) / np.linalg.norm(r.astype(np.float64) - 1e-6)
+    assert rel < 1e-4, rel
+    # the fused, graph-captured run gives the same as the per-step surface
+    sol2 = gpu.FdSolver(s, arith=gpu.ARITH_REF, init_value=1e-6)
+    sol2.set_learned_boundary(lb)
+    sol2.run(0, nsteps)
+    assert np.array_equal(sol2.current(), g)
+    assert np.array_equal(sol2.previous(), sol.previous())
+
+
+def test_training_reduces_the_loss(gpu):
+    banner("training")
+    s, sol = wide_run(gpu, 56, 10, 90, (30, 27, 26), seed=3)
+    lb = gpu.LearnedBoundary(seed=4)
+    lb.set_capacity(200000)
+    lb.collect(sol, sol.i2, (15, 14, 16), (26, 25, 24), s_min=1e-8)
+    l0, _ = lb.eval(with_grad=False)
+    hist = lb.train(300, batch_per_face=2048, lr=2e-3, seed=1)
+    l1, _ = lb.eval(with_grad=False)
+    assert np.all(np.isfinite(hist))
+    assert l1 < 0.5 * l0, (l0, l1)
