@@ -136,6 +136,25 @@ phc_status phc_fd_synchronize(phc_fd *h);
 phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char *cache_path, int *tile_j,
                            int *block_waves, int *chunk_i);
 
+/* ==================================================================== FD64
+ * The FD time loop of FDTDmain.cpp:55-81 (addRHS + inner3D + applySponge,
+ * FDTDkernels.hpp:8-34, 108-128) in FP64: fields, vp^2 dt^2, coefficients,
+ * source samples and sponge factors (exp(alpha d^2) in double) all double.
+ * The accuracy yardstick of the FP32 paths (half width 4; the arith and tiling
+ * fields of phc_fd_params are ignored).  Synchronous: phc_fd64_run returns
+ * when the steps are done.
+ */
+typedef struct phc_fd64 phc_fd64;
+phc_status phc_fd64_create(const phc_fd_params *p, phc_fd64 **out);
+phc_status phc_fd64_destroy(phc_fd64 *h);
+phc_status phc_fd64_set_receivers(phc_fd64 *h, int nrec, const int *ijk);
+/* Samples it0 .. it0+nsteps-1, continuing from the handle's current state. */
+phc_status phc_fd64_run(phc_fd64 *h, int it0, int nsteps);
+/* Last run: src[nsteps] at the source after each step, rec[nrec*nsteps]. */
+phc_status phc_fd64_get_traces(phc_fd64 *h, double *src, double *rec);
+/* Current (current != 0) or previous level, extended layout as phc_fd_get_field. */
+phc_status phc_fd64_get_field(phc_fd64 *h, int current, double *host);
+
 /* ===================================================================== SEM
  * Replaces SEMsolver (src/sem/solver/SEMsolver.hpp:18-91, SEMsolver.cpp:12-212)
  * on the structured hexahedral box mesh of SEMmesh (src/sem/mesh/SEMmesh.cpp).

@@ -140,7 +140,8 @@ def source_term(nsamples, dt, f0, order=1, exp_variant=EXP_DOUBLE):
 class FdSetup:
     """FDTDInit (src/fd/utilsFDTD/FDTDinit.hpp:36-317) for one grid: coefficients,
     dt, source wavelet and model.  model: "two_layer" (the reference's
-    init_models), "homogeneous" (vmin everywhere) or ("layered", nlayers, seed)."""
+    init_models), "homogeneous" (vmin everywhere), ("layered", nlayers, seed)
+    or an explicit float32 array of vp^2 dt^2 (IDX3 order)."""
 
     def __init__(self, nx=150, ny=None, nz=None, L=4, dx=10.0, xs=None, ys=None, zs=None,
                  f0=15.0, time_max=1.0, vmin=1500.0, vmax=4500.0, cfl=0.8, source_order=1,
@@ -162,7 +163,11 @@ class FdSetup:
         self.rhs = source_term(self.nsamples, self.dt, f0, source_order, EXP_DOUBLE)
         n3 = self.nx * self.ny * self.nz
         self.vp = np.zeros(n3, np.float32)
-        if model == "two_layer":
+        if isinstance(model, np.ndarray):
+            if model.size != n3:
+                raise ValueError("explicit vp^2 dt^2 model has %d values, grid has %d" % (model.size, n3))
+            self.vp = np.ascontiguousarray(model, np.float32).reshape(-1).copy()
+        elif model == "two_layer":
             check(lib().phc_fd_model_two_layer(self.nx, self.ny, self.nz, vmin, self.dt, self.vp))
         elif model == "homogeneous":
             v0 = np.float32(np.float32(np.float32(vmin) * np.float32(vmin)) * self.dt) * self.dt
@@ -310,6 +315,69 @@ class FdSolver:
         check(lib().phc_fd_autotune(self.h, steps_per_trial, cp, ctypes.byref(tj), ctypes.byref(ty),
                                     ctypes.byref(ci)))
         return tj.value, ty.value, ci.value
+
+
+# ------------------------------------------------------------------- FD64
+SIGNATURES.update({
+    "phc_fd64_create": (c_int, [ctypes.POINTER(FdParams), ctypes.POINTER(c_void_p)]),
+    "phc_fd64_destroy": (c_int, [c_void_p]),
+    "phc_fd64_set_receivers": (c_int, [c_void_p, c_int, c_void_p]),
+    "phc_fd64_run": (c_int, [c_void_p, c_int, c_int]),
+    "phc_fd64_get_traces": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "phc_fd64_get_field": (c_int, [c_void_p, c_int, f64p]),
+})
+
+
+class Fd64Solver:
+    """The FD loop in FP64 (accuracy yardstick), from an FdSetup."""
+
+    def __init__(self, setup, init_value=1e-6):
+        s = setup
+        self.setup = s
+        self.nx, self.ny, self.nz, self.L = s.nx, s.ny, s.nz, s.L
+        p = FdParams()
+        p.nx, p.ny, p.nz = s.nx, s.ny, s.nz
+        p.lx = p.ly = p.lz = s.L
+        p.xs, p.ys, p.zs = s.xs, s.ys, s.zs
+        p.coefx, p.coefy, p.coefz = _ptr(s.cx), _ptr(s.cy), _ptr(s.cz)
+        p.coef0 = s.coef0
+        p.vp, p.rhs, p.nsamples = _ptr(s.vp), _ptr(s.rhs), s.nsamples
+        p.sponge_width, p.sponge_alpha, p.init_value = s.sponge_width, s.sponge_alpha, init_value
+        h = c_void_p()
+        check(lib().phc_fd64_create(ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        self.it = 0
+        self.nrec = 0
+
+    def close(self):
+        if self.h:
+            check(lib().phc_fd64_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_receivers(self, ijk):
+        a = np.ascontiguousarray(ijk, np.int32).reshape(-1)
+        self.nrec = len(a) // 3
+        check(lib().phc_fd64_set_receivers(self.h, self.nrec, _ptr(a)))
+
+    def run(self, nsteps):
+        check(lib().phc_fd64_run(self.h, self.it, nsteps))
+        self.it += nsteps
+        src = np.zeros(nsteps)
+        rec = np.zeros(max(self.nrec, 1) * nsteps)
+        check(lib().phc_fd64_get_traces(self.h, _ptr(src), _ptr(rec) if self.nrec else None))
+        return src, (rec.reshape(self.nrec, nsteps) if self.nrec else None)
+
+    def field(self, current=True):
+        L = self.L
+        out = np.zeros((self.nx + 2 * L) * (self.ny + 2 * L) * (self.nz + 2 * L))
+        check(lib().phc_fd64_get_field(self.h, 1 if current else 0, out))
+        return out.reshape(self.nx + 2 * L, self.ny + 2 * L, self.nz + 2 * L)
 
 
 # ------------------------------------------------------------- batched shots
