@@ -318,6 +318,15 @@ phc_status phc_lb_add_samples(phc_lb *h, int face, const float *host, int64_t n)
  * points must lie outside the wide run's sponge. */
 phc_status phc_lb_collect(phc_lb *h, phc_fd *wide, int cur, const int *origin, const int *dims, float halo,
                           float keep, float s_min, uint64_t seed);
+/* Append on-policy samples: `thin` is a run on the region itself (grid =
+ * region, typically with this model attached) and `wide` the same shot on the
+ * region plus margin and sponge (the region at origin of its grid), both at
+ * step n (current levels thin_cur, wide_cur).  Inputs come from the thin
+ * levels (the states the network meets), the target from the wide field
+ * beyond the region, over the thin s: sum_{l>d} (c_l/c_1)(W(P + l n_out) - h)
+ * / s_thin; points whose |target| exceeds 16 are skipped. */
+phc_status phc_lb_collect_onpolicy(phc_lb *h, phc_fd *thin, int thin_cur, phc_fd *wide, int wide_cur,
+                                   const int *origin, float halo, float keep, float s_min, uint64_t seed);
 /* Mean squared error over every stored sample and, if grad != NULL, its
  * gradient with respect to the FP32 parameters (at their bf16 values). */
 phc_status phc_lb_eval(phc_lb *h, float *loss, float *grad);
@@ -328,10 +337,12 @@ phc_status phc_lb_train(phc_lb *h, int steps, int batch_per_face, float lr, uint
 phc_status phc_lb_infer(phc_lb *h, int face, const float *samples, int64_t n, float *y);
 /* Attach a copy of the model's bf16 weights to an FD handle (lb NULL detaches).
  * Needs half width 4, equal coefficients on the three axes, sponge_width 0,
- * sides >= 16 and the source outside the layer; halo is the value the halo
+ * sides 16..4000 and the source outside the layer; halo is the value the halo
  * holds (the handle's init_value).  Every step then runs the correction pass,
- * the stencil and the apply pass. */
-phc_status phc_fd_set_learned_boundary(phc_fd *h, const phc_lb *lb, float halo);
+ * the stencil and the apply pass.  layer_damping (alpha in [-1, 0], 0 = none)
+ * multiplies the new value at depth d of a face's layer by
+ * exp(alpha (4-d)^2), once per face whose layer holds the point. */
+phc_status phc_fd_set_learned_boundary(phc_fd *h, const phc_lb *lb, float halo, float layer_damping);
 /* The corrections the next step would add (vp^2dt^2 c_1 s y per layer point)
  * from levels cur / 1-cur, compact per face: face f, depth d (from its edge),
  * u, v = the other two axes in increasing order, index ((d*n_u)+u)*n_v+v. */

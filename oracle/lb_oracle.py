@@ -206,3 +206,24 @@ def apply_corrections(p_new, corr, n):
         Q = c + R
         np.add.at(p_new, (Q[:, 0], Q[:, 1], Q[:, 2]), corr[f])
     return p_new
+
+
+def onpolicy_targets(tpn, tpm, tvp, wcur, origin, n, f, h, chat, c1, t_max=16.0):
+    """On-policy samples of face f: inputs from the thin levels (extended
+    arrays of the region grid), target sum_{l>d} chat_l (W(P + l n_out) - h) /
+    s_thin from the wide level at the same step; mask of the kept points."""
+    X, s, d, v, _ = layer_inputs(tpn, tpm, tvp, (0, 0, 0), n, f, h, c1)
+    pts, _ = face_points(n, f)
+    a, side, _, _ = face_geom(n, f)
+    dirv = np.zeros(3, int)
+    dirv[a] = -1 if side else 1
+    Pw = pts + np.asarray(origin)[None, :]
+    w = np.asarray(wcur, np.float64)
+    t = np.zeros(len(pts))
+    for l in range(1, R + 1):
+        Q = Pw - l * dirv[None, :] + R
+        t += np.where(l > d, chat[l] * (w[Q[:, 0], Q[:, 1], Q[:, 2]] - h), 0.0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = t / s
+    keep = (s > S_TINY) & (np.abs(t) <= t_max)
+    return X, s, d, t, keep

@@ -336,6 +336,7 @@ phc_status phc::fd_field_view(phc_fd *h, FdFieldView *v)
     v->sw = h->sw;
     v->c1 = h->cx[1];
     for (int l = 0; l <= LB_R; ++l) v->chat[l] = h->cx[l] / h->cx[1];
+    v->coef0 = h->coef0;
     v->stream = h->stream;
     return PHC_OK;
 }
@@ -688,7 +689,7 @@ phc_status phc_fd_set_timing(phc_fd *h, int enable)
     return PHC_OK;
 }
 
-phc_status phc_fd_set_learned_boundary(phc_fd *h, const phc_lb *lb, float halo)
+phc_status phc_fd_set_learned_boundary(phc_fd *h, const phc_lb *lb, float halo, float layer_damping)
 {
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
     PHC_HIP(hipStreamSynchronize(h->stream));
@@ -704,11 +705,14 @@ phc_status phc_fd_set_learned_boundary(phc_fd *h, const phc_lb *lb, float halo)
     if (h->sw != 0) return fail(PHC_ERR_INVALID, "the learned boundary replaces the sponge: sponge_width must be 0");
     const int n[3] = {h->nx, h->ny, h->nz}, src[3] = {h->xs, h->ys, h->zs};
     for (int a = 0; a < 3; ++a) {
-        if (n[a] < 4 * LB_W) return fail(PHC_ERR_INVALID, "grid side %d below %d", n[a], 4 * LB_W);
+        if (n[a] < 4 * LB_W || n[a] > 4000)
+            return fail(PHC_ERR_INVALID, "grid side %d outside %d..4000", n[a], 4 * LB_W);
         if (src[a] < LB_W || src[a] >= n[a] - LB_W)
             return fail(PHC_ERR_INVALID, "the source must lie inside the %d-point learned layer", LB_W);
     }
     if (!std::isfinite(halo)) return fail(PHC_ERR_INVALID, "halo value is not finite");
+    if (!(layer_damping <= 0.f) || layer_damping < -1.f)
+        return fail(PHC_ERR_INVALID, "layer damping %g outside [-1, 0]", layer_damping);
     const int64_t ncorr = lb_corr_layout(h->L, h->lbi.corr_off);
     PHC_TRY(h->lb_w.alloc(sizeof(uint16_t) * LB_FACES * LB_NP));
     PHC_TRY(h->lb_corr.alloc(sizeof(float) * (size_t)ncorr));
@@ -718,6 +722,8 @@ phc_status phc_fd_set_learned_boundary(phc_fd *h, const phc_lb *lb, float halo)
     h->lbi.halo = halo;
     h->lbi.c1 = fv.c1;
     for (int l = 0; l <= LB_R; ++l) h->lbi.chat[l] = fv.chat[l];
+    for (int d = 0; d < LB_W; ++d)
+        h->lbi.ldamp[d] = (float)std::exp((double)layer_damping * (double)(LB_W - d) * (LB_W - d));
     h->lb_on = 1;
     return PHC_OK;
 }

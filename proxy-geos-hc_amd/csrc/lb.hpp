@@ -55,6 +55,7 @@ struct LbInfer {
     float halo;          // value held by the halo (h)
     float c1;            // c_1 of the stencil
     float chat[LB_R + 1];  // c_l / c_1
+    float ldamp[LB_W];   // new value at layer depth d multiplied by ldamp[d] (per face holding it)
 };
 
 // Launch the correction pass (before the step: reads the current and previous
@@ -76,6 +77,7 @@ struct FdFieldView {
     int sw;
     float c1;              // c_1 of the stencil (equal on the three axes)
     float chat[LB_R + 1];  // c_l / c_1
+    double coef0;          // centre coefficient
     hipStream_t stream;
 };
 phc_status fd_field_view(phc_fd *h, FdFieldView *v);

@@ -98,15 +98,19 @@ __device__ inline float lb_inputs(const LbGridView &G, const LbFaceGeom &g, int 
     const int64_t base = fd_off(G.L, G.o[0] + c[0], G.o[1] + c[1], G.o[2] + c[2]);
     *center = base;
     float s = 0.f;
+    // every offset is read (beyond the region's edge the grid still holds
+    // values: its halo, or the wide grid around the region), then masked, so
+    // the loads are not inside lane-divergent branches
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
-        const int out = 4 - m;  // outward offset (negative: inward)
-        float a = 0.f, b = 0.f;
-        if (out <= d) {
-            const int64_t o = base + (int64_t)(m - 4) * dir * step;
-            a = G.pn[o] - h;
-            b = G.pnm1[o] - h;
-        }
+        const int64_t o = base + (int64_t)(m - 4) * dir * step;
+        x[m] = G.pn[o];
+        x[9 + m] = G.pnm1[o];
+    }
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+        const bool inside = 4 - m <= d;  // outward offset 4-m within the region
+        const float a = inside ? x[m] - h : 0.f, b = inside ? x[9 + m] - h : 0.f;
         x[m] = a;
         x[9 + m] = b;
         s = fmaxf(s, fmaxf(fabsf(a), fabsf(b)));
@@ -165,6 +169,63 @@ __global__ void lb_collect_kernel(LbCollectArgs A)
     o[LB_NIN] = tgt;
     o[LB_NIN + 1] = (float)d;
     o[LB_NIN + 2] = (float)(t - A.face_off[f]);  // compact index in the face's layer
+}
+
+// On-policy samples: a thin run (the region itself, with the learned layer)
+// and a wide run (the region plus margin and sponge) of the same shot at the
+// same step n.  Inputs come from the thin run's own levels (the states the
+// network meets when it runs), the target from the wide run's field beyond
+// the region: t = sum_{l>d} (c_l/c_1) (W^n(P + l n_out) - h) / s_thin, the
+// term a transparent edge would add at the thin run's own amplitude scale.
+struct LbOnPolicyArgs {
+    LbGridView T;              // thin levels n (pn) and n-1 (pnm1), o = 0, n = thin size
+    const float *wcur;         // wide level n
+    FdLayout LW;
+    int wo[3];                 // region origin in the wide grid
+    float h, c1;
+    float chat[LB_R + 1];
+    float keep, s_min, t_max;
+    uint64_t seed;
+    float *out[LB_FACES];
+    unsigned long long *count;
+    int64_t cap;
+    int64_t face_off[LB_FACES + 1];
+};
+
+__global__ void lb_onpolicy_kernel(LbOnPolicyArgs A)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.face_off[LB_FACES]) return;
+    int f = 0;
+    while (f + 1 < LB_FACES && t >= A.face_off[f + 1]) ++f;
+    const LbFaceGeom g = lb_face(A.T.n, f);
+    int64_t r = t - A.face_off[f];
+    const int v = (int)(r % g.nv);
+    r /= g.nv;
+    const int u = (int)(r % g.nu);
+    const int d = (int)(r / g.nu);
+    const uint32_t hv = lb_hash(A.seed * 0x9e3779b97f4a7c15ull + (uint64_t)t);
+    if ((float)(hv >> 8) * (1.f / 16777216.f) >= A.keep) return;
+    float x[LB_NIN];
+    int64_t P;
+    const float s = lb_inputs(A.T, g, d, u, v, A.h, A.c1, x, &P);
+    if (!(s >= A.s_min) || !(s > LB_S_TINY)) return;
+    int c[3];
+    lb_point(A.T.n, g, d, u, v, c);
+    const int dir = g.side ? -1 : 1;
+    const int64_t step = g.axis == 0 ? A.LW.pitch_j : (g.axis == 1 ? A.LW.pitch_k : 1);
+    const int64_t Pw = fd_off(A.LW, A.wo[0] + c[0], A.wo[1] + c[1], A.wo[2] + c[2]);
+    float tgt = 0.f;
+    for (int l = d + 1; l <= LB_R; ++l) tgt += A.chat[l] * (A.wcur[Pw - (int64_t)l * dir * step] - A.h);
+    tgt /= s;
+    if (!(fabsf(tgt) <= A.t_max)) return;  // the thin state has lost the wave: no usable label
+    const unsigned long long slot = atomicAdd(A.count + f, 1ull);
+    if ((int64_t)slot >= A.cap) return;
+    float *o = A.out[f] + slot * LB_NSTRIDE;
+    for (int m = 0; m < LB_NIN; ++m) o[m] = x[m];
+    o[LB_NIN] = tgt;
+    o[LB_NIN + 1] = (float)d;
+    o[LB_NIN + 2] = (float)(t - A.face_off[f]);
 }
 
 // ------------------------------------------------------------ training
@@ -328,55 +389,77 @@ __device__ inline void lb_split(const float *a, lb_bf16x4 &hi, lb_bf16x4 &lo)
     }
 }
 
-// y for the wave's 32 points.  x: this lane's 12 inputs (k = 8c + 4*half + e,
-// c = 0..2, e = 0..3); d: depth of this lane's point; w: the face's bf16
-// parameters.  Returns y (valid in both halves).
-__device__ inline float lb_mlp_wave(const uint16_t *__restrict__ w, const float xk[12], int d)
+// One face's weights in the registers of a wavefront, in MFMA operand
+// layout (A operands of layers 1 and 2) or per result row (b2, w3); the
+// per-depth first-layer biases stay in LDS.
+struct LbWaveWeights {
+    lb_bf16x4 a1[3], a2[4];
+    float b2[16], w3[16];
+    float b3;
+};
+
+__device__ inline void lb_load_wave_weights(const uint16_t *__restrict__ w, LbWaveWeights &W)
 {
     const int lane = threadIdx.x & 63;
     const int row = lane & 31, half = lane >> 5;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int k = 8 * c + 4 * half + e;
+            W.a1[c][e] = (short)(k < LB_NIN ? w[LB_OFF_W1 + row * LB_NIN + k] : (uint16_t)0);
+        }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) W.a2[c][e] = (short)w[LB_OFF_W2 + row * LB_H + 8 * c + 4 * half + e];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int i = 8 * (v >> 2) + 4 * half + (v & 3);
+        W.b2[v] = bf2f(w[LB_OFF_B2 + i]);
+        W.w3[v] = bf2f(w[LB_OFF_W3 + i]);
+    }
+    W.b3 = bf2f(w[LB_OFF_B3]);
+}
+
+// y for the wave's 32 points.  xk: this lane's 12 inputs (k = 8c + 4*half + e,
+// c = 0..2, e = 0..3); d: depth of this lane's point; b1: the face's
+// per-depth first-layer biases (LB_W x LB_H floats, LDS).  Returns y (valid
+// in both halves).
+__device__ inline float lb_mlp_wave(const LbWaveWeights &W, const float *b1, const float xk[12], int d)
+{
+    const int half = (threadIdx.x & 63) >> 5;
     lb_f32x16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.f;
     // layer 1: H1^T[32 x 32] = W1[32 x 24] x^T[24 x 32]
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        lb_bf16x4 a, bh, bl;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int k = 8 * c + 4 * half + e;
-            a[e] = (short)(k < LB_NIN ? w[LB_OFF_W1 + row * LB_NIN + k] : (uint16_t)0);
-        }
+        lb_bf16x4 bh, bl;
         lb_split(xk + 4 * c, bh, bl);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a1[c], bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a1[c], bl, acc, 0, 0, 0);
     }
     float h1[16];
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
         const int i = 8 * (v >> 2) + 4 * half + (v & 3);
-        h1[v] = fmaxf(acc[v] + bf2f(w[LB_OFF_B1 + d * LB_H + i]), 0.f);
+        h1[v] = fmaxf(acc[v] + b1[d * LB_H + i], 0.f);
         acc[v] = 0.f;
     }
     // layer 2: H2^T = W2 H1^T
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        lb_bf16x4 a, bh, bl;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) a[e] = (short)w[LB_OFF_W2 + row * LB_H + 8 * c + 4 * half + e];
+        lb_bf16x4 bh, bl;
         lb_split(h1 + 4 * c, bh, bl);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a2[c], bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a2[c], bl, acc, 0, 0, 0);
     }
     float y = 0.f;
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        const int i = 8 * (v >> 2) + 4 * half + (v & 3);
-        const float h2 = fmaxf(acc[v] + bf2f(w[LB_OFF_B2 + i]), 0.f);
-        y = fmaf(bf2f(w[LB_OFF_W3 + i]), h2, y);
-    }
+    for (int v = 0; v < 16; ++v) y = fmaf(W.w3[v], fmaxf(acc[v] + W.b2[v], 0.f), y);
     y += __shfl_xor(y, 32);
-    return y + bf2f(w[LB_OFF_B3]);
+    return y + W.b3;
 }
 
 // this lane's 12 of the 24 padded inputs: k = 8c + e (lanes 0-31) or
@@ -395,103 +478,136 @@ __device__ inline void lb_lane_inputs(const float x[LB_NIN], float xk[12])
         }
 }
 
+constexpr int LB_GPW = 4;  // groups of 32 layer points per wavefront
+
 struct LbCorrectArgs {
     LbGridView G;
     const uint16_t *w;
     float *corr;
-    int64_t off[LB_FACES + 1];   // compact offsets
-    int64_t wave0[LB_FACES + 1]; // first wave of each face
+    int64_t off[LB_FACES + 1];  // compact offsets
+    int64_t blk0[LB_FACES + 1]; // first block of each face
     float h, c1;
 };
 
-// Correction pass: corr[face][d][u][v] = vp2 * c1 * s * y for every layer point.
+// Correction pass: corr[face][d][u][v] = vp2 * c1 * s * y for every layer
+// point.  A block works on one face: its biases b1 go to LDS, each wavefront
+// keeps the face's other weights in registers and evaluates LB_GPW groups of
+// 32 consecutive points.
 __global__ void __launch_bounds__(256) lb_correct_kernel(LbCorrectArgs A)
 {
-    const int64_t wv = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wv >= A.wave0[LB_FACES]) return;  // uniform per wave
+    __shared__ float b1s[LB_W * LB_H];
     int f = 0;
-    while (f + 1 < LB_FACES && wv >= A.wave0[f + 1]) ++f;
+    while (f + 1 < LB_FACES && (int64_t)blockIdx.x >= A.blk0[f + 1]) ++f;
+    const uint16_t *w = A.w + f * LB_NP;
+    for (int t = threadIdx.x; t < LB_W * LB_H; t += blockDim.x) b1s[t] = bf2f(w[LB_OFF_B1 + t]);
+    __syncthreads();
+    LbWaveWeights W;
+    lb_load_wave_weights(w, W);
     const LbFaceGeom g = lb_face(A.G.n, f);
     const int64_t cnt = A.off[f + 1] - A.off[f];
-    const int64_t idx = (wv - A.wave0[f]) * 32 + (threadIdx.x & 31);
-    const int64_t ci = idx < cnt ? idx : cnt - 1;
-    const int v = (int)(ci % g.nv);
-    const int u = (int)((ci / g.nv) % g.nu);
-    const int d = (int)(ci / ((int64_t)g.nv * g.nu));
-    float x[LB_NIN], xk[12];
-    int64_t center;
-    const float s = lb_inputs(A.G, g, d, u, v, A.h, A.c1, x, &center);
-    lb_lane_inputs(x, xk);
-    const float y = lb_mlp_wave(A.w + f * LB_NP, xk, d);
-    if ((threadIdx.x & 32) == 0 && idx < cnt) A.corr[A.off[f] + idx] = A.G.vp[center] * A.c1 * s * y;
+    const int64_t g0 = ((int64_t)(blockIdx.x - A.blk0[f]) * 4 + (threadIdx.x >> 6)) * LB_GPW;
+    for (int gi = 0; gi < LB_GPW; ++gi) {
+        const int64_t base = (g0 + gi) * 32;
+        if (base >= cnt) break;  // uniform per wavefront
+        const int64_t idx = base + (threadIdx.x & 31);
+        const int ci = (int)(idx < cnt ? idx : cnt - 1);  // layers hold < 2^31 points
+        // (d, u, v) of the group's first point, then this lane's offset with
+        // carries (nv >= 16, so at most two)
+        const int b0 = (int)base;
+        int v = b0 % g.nv, u = (b0 / g.nv) % g.nu, d = b0 / (g.nv * g.nu);
+        v += ci - b0;
+        while (v >= g.nv) {
+            v -= g.nv;
+            if (++u == g.nu) { u = 0; ++d; }
+        }
+        float x[LB_NIN], xk[12];
+        int64_t center;
+        const float s = lb_inputs(A.G, g, d, u, v, A.h, A.c1, x, &center);
+        lb_lane_inputs(x, xk);
+        const float y = lb_mlp_wave(W, b1s, xk, d);
+        if ((threadIdx.x & 32) == 0 && idx < cnt) A.corr[A.off[f] + idx] = A.G.vp[center] * A.c1 * s * y;
+    }
 }
 
 // MFMA inference on given inputs (testing): y[n] for samples of one face.
 __global__ void __launch_bounds__(256) lb_infer_kernel(const uint16_t *w, const float *samples, int64_t n, float *y)
 {
+    __shared__ float b1s[LB_W * LB_H];
+    for (int t = threadIdx.x; t < LB_W * LB_H; t += blockDim.x) b1s[t] = bf2f(w[LB_OFF_B1 + t]);
+    __syncthreads();
     const int64_t wv = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (wv * 32 >= n) return;
+    LbWaveWeights W;
+    lb_load_wave_weights(w, W);
     const int64_t idx = wv * 32 + (threadIdx.x & 31);
     const int64_t ci = idx < n ? idx : n - 1;
     float x[LB_NIN], xk[12];
     for (int m = 0; m < LB_NIN; ++m) x[m] = samples[ci * LB_NSTRIDE + m];
     const int d = min(max((int)samples[ci * LB_NSTRIDE + LB_NIN + 1], 0), LB_W - 1);
     lb_lane_inputs(x, xk);
-    const float r = lb_mlp_wave(w, xk, d);
+    const float r = lb_mlp_wave(W, b1s, xk, d);
     if ((threadIdx.x & 32) == 0 && idx < n) y[idx] = r;
 }
 
 // Apply pass: one thread per point of the union of the layers; adds the
-// corrections of every face whose layer holds the point.
+// corrections of every face whose layer holds the point.  The union is walked
+// as rows (blockIdx.y, decoded once per block): 2*LB_W*ny rows along k for
+// the i-layers, 2*LB_W*(nx-2*LB_W) rows along k for the j-layers (i
+// interior), (nx-2*LB_W)*2*LB_W rows along j for the k-layers (i, j interior).
 struct LbApplyArgs {
     float *p;
     FdLayout L;
     const float *corr;
     int64_t off[LB_FACES + 1];
-    int64_t nA, nB, nC;  // points in the i-layers, j-layers (minus i), k-layers (minus i, j)
+    int rowsA, rowsB, rowsC;
+    float ldamp[LB_W];  // damping factor of the new value per layer depth (1: none)
 };
 
-__global__ void lb_apply_kernel(LbApplyArgs A)
+__global__ void __launch_bounds__(256) lb_apply_kernel(LbApplyArgs A)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int nx = A.L.nx, ny = A.L.ny, nz = A.L.nz;
+    const int row = blockIdx.y;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int i, j, k;
-    if (t < A.nA) {  // i in layers, all j, k
-        k = (int)(t % nz);
-        j = (int)((t / nz) % ny);
-        const int q = (int)(t / ((int64_t)nz * ny));
+    if (row < A.rowsA) {  // i in layers, all j, k along the row
+        const int q = row / ny;
+        j = row - q * ny;
         i = q < LB_W ? q : nx - 2 * LB_W + q;
-    } else if (t < A.nA + A.nB) {  // j in layers, i interior, all k
-        const int64_t r = t - A.nA;
-        k = (int)(r % nz);
-        const int q = (int)((r / nz) % (2 * LB_W));
-        i = LB_W + (int)(r / ((int64_t)nz * 2 * LB_W));
+        k = t;
+        if (k >= nz) return;
+    } else if (row < A.rowsA + A.rowsB) {  // j in layers, i interior, k along the row
+        const int r = row - A.rowsA;
+        const int q = r % (2 * LB_W);
+        i = LB_W + r / (2 * LB_W);
         j = q < LB_W ? q : ny - 2 * LB_W + q;
-    } else if (t < A.nA + A.nB + A.nC) {  // k in layers, i and j interior
-        const int64_t r = t - A.nA - A.nB;
-        const int q = (int)(r % (2 * LB_W));
-        j = LB_W + (int)((r / (2 * LB_W)) % (ny - 2 * LB_W));
-        i = LB_W + (int)(r / ((int64_t)2 * LB_W * (ny - 2 * LB_W)));
+        k = t;
+        if (k >= nz) return;
+    } else {  // k in layers, i and j interior, j along the row
+        const int r = row - A.rowsA - A.rowsB;
+        const int q = r % (2 * LB_W);
+        i = LB_W + r / (2 * LB_W);
         k = q < LB_W ? q : nz - 2 * LB_W + q;
-    } else {
-        return;
+        j = LB_W + t;
+        if (j >= ny - LB_W) return;
     }
     const int n[3] = {nx, ny, nz};
     const int c[3] = {i, j, k};
-    float add = 0.f;
+    float add = 0.f, damp = 1.f;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        int side = -1, d = 0;
-        if (c[a] < LB_W) { side = 0; d = c[a]; }
-        else if (c[a] >= n[a] - LB_W) { side = 1; d = n[a] - 1 - c[a]; }
-        if (side < 0) continue;
-        const LbFaceGeom g = lb_face(n, 2 * a + side);
+        const bool lo = c[a] < LB_W, hi = c[a] >= n[a] - LB_W;
+        const int side = hi ? 1 : 0;
+        const int d = lo ? c[a] : (hi ? n[a] - 1 - c[a] : 0);  // 0 when not in this axis's layers
+        const int nu = (a == 0) ? ny : nx;
+        const int nv = (a == 2) ? ny : nz;
         const int u = (a == 0) ? j : i;
         const int v = (a == 2) ? j : k;
-        add += A.corr[A.off[2 * a + side] + ((int64_t)d * g.nu + u) * g.nv + v];
+        const float cv = A.corr[A.off[2 * a + side] + ((int64_t)d * nu + u) * nv + v];
+        add += (lo || hi) ? cv : 0.f;
+        damp *= (lo || hi) ? A.ldamp[d] : 1.f;
     }
     const int64_t o = fd_off(A.L, i, j, k);
-    A.p[o] = add_nc(A.p[o], add);
+    A.p[o] = mul_nc(add_nc(A.p[o], add), damp);
 }
 
 hipError_t lb_enqueue_correct(const LbInfer &lb, const FdLayout &L, const float *pn, const float *pnm1,
@@ -510,11 +626,11 @@ hipError_t lb_enqueue_correct(const LbInfer &lb, const FdLayout &L, const float 
     A.corr = lb.corr;
     A.h = lb.halo;
     A.c1 = lb.c1;
-    A.wave0[0] = 0;
+    A.blk0[0] = 0;
     for (int f = 0; f <= LB_FACES; ++f) A.off[f] = lb.corr_off[f];
-    for (int f = 0; f < LB_FACES; ++f) A.wave0[f + 1] = A.wave0[f] + (A.off[f + 1] - A.off[f] + 31) / 32;
-    const int64_t blocks = (A.wave0[LB_FACES] + 3) / 4;
-    hipLaunchKernelGGL(lb_correct_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, A);
+    const int64_t per_block = 4 * LB_GPW * 32;  // points per block
+    for (int f = 0; f < LB_FACES; ++f) A.blk0[f + 1] = A.blk0[f] + (A.off[f + 1] - A.off[f] + per_block - 1) / per_block;
+    hipLaunchKernelGGL(lb_correct_kernel, dim3((unsigned)A.blk0[LB_FACES]), dim3(256), 0, stream, A);
     return hipGetLastError();
 }
 
@@ -525,11 +641,13 @@ hipError_t lb_enqueue_apply(const LbInfer &lb, const FdLayout &L, float *pnew, h
     A.L = L;
     A.corr = lb.corr;
     for (int f = 0; f <= LB_FACES; ++f) A.off[f] = lb.corr_off[f];
-    A.nA = (int64_t)2 * LB_W * L.ny * L.nz;
-    A.nB = (int64_t)2 * LB_W * (L.nx - 2 * LB_W) * L.nz;
-    A.nC = (int64_t)2 * LB_W * (L.nx - 2 * LB_W) * (L.ny - 2 * LB_W);
-    const int64_t n = A.nA + A.nB + A.nC;
-    hipLaunchKernelGGL(lb_apply_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, A);
+    for (int d = 0; d < LB_W; ++d) A.ldamp[d] = lb.ldamp[d];
+    A.rowsA = 2 * LB_W * L.ny;
+    A.rowsB = 2 * LB_W * (L.nx - 2 * LB_W);
+    A.rowsC = (L.nx - 2 * LB_W) * 2 * LB_W;
+    const int len = std::max(L.nz, L.ny - 2 * LB_W);
+    hipLaunchKernelGGL(lb_apply_kernel, dim3((len + 255) / 256, A.rowsA + A.rowsB + A.rowsC), dim3(256), 0, stream,
+                       A);
     return hipGetLastError();
 }
 
@@ -757,6 +875,60 @@ phc_status phc_lb_collect(phc_lb *h, phc_fd *wide, int cur, const int *origin, c
     PHC_HIP(hipStreamSynchronize(fv.stream));
     const int64_t n = A.face_off[LB_FACES];
     hipLaunchKernelGGL(lb_collect_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, A);
+    PHC_HIP(hipGetLastError());
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    return PHC_OK;
+}
+
+phc_status phc_lb_collect_onpolicy(phc_lb *h, phc_fd *thin, int thin_cur, phc_fd *wide, int wide_cur,
+                                   const int *origin, float halo, float keep, float s_min, uint64_t seed)
+{
+    if (!h || !thin || !wide || !origin) return fail(PHC_ERR_INVALID, "NULL argument");
+    if (thin_cur < 0 || thin_cur > 1 || wide_cur < 0 || wide_cur > 1)
+        return fail(PHC_ERR_INVALID, "levels must be 0 or 1");
+    if (h->cap < 1) return fail(PHC_ERR_STATE, "no sample store: call phc_lb_set_capacity first");
+    FdFieldView tv, wv;
+    PHC_TRY(fd_field_view(thin, &tv));
+    PHC_TRY(fd_field_view(wide, &wv));
+    if (tv.c1 != wv.c1) return fail(PHC_ERR_INVALID, "thin and wide runs use different stencils");
+    const int n[3] = {tv.L.nx, tv.L.ny, tv.L.nz}, N[3] = {wv.L.nx, wv.L.ny, wv.L.nz};
+    for (int a = 0; a < 3; ++a) {
+        if (n[a] < 4 * LB_W) return fail(PHC_ERR_INVALID, "thin grid side %d below %d", n[a], 4 * LB_W);
+        if (origin[a] - LB_R < wv.sw || origin[a] + n[a] + LB_R > N[a] - wv.sw)
+            return fail(PHC_ERR_INVALID, "the thin region plus %d points must lie outside the wide run's sponge",
+                        LB_R);
+    }
+    LbOnPolicyArgs A;
+    A.T.pn = tv.lev[thin_cur];
+    A.T.pnm1 = tv.lev[1 - thin_cur];
+    A.T.vp = tv.vp;
+    A.T.L = tv.L;
+    for (int a = 0; a < 3; ++a) {
+        A.T.o[a] = 0;
+        A.T.n[a] = n[a];
+        A.wo[a] = origin[a];
+    }
+    A.wcur = wv.lev[wide_cur];
+    A.LW = wv.L;
+    A.h = halo;
+    A.c1 = tv.c1;
+    for (int l = 0; l <= LB_R; ++l) A.chat[l] = tv.chat[l];
+    A.keep = keep;
+    A.s_min = s_min;
+    A.t_max = 16.f;
+    A.seed = seed;
+    for (int f = 0; f < LB_FACES; ++f) A.out[f] = h->data[f].as<float>();
+    A.count = h->count.as<unsigned long long>();
+    A.cap = h->cap;
+    A.face_off[0] = 0;
+    for (int f = 0; f < LB_FACES; ++f) {
+        const LbFaceGeom g = lb_face(n, f);
+        A.face_off[f + 1] = A.face_off[f] + (int64_t)LB_W * g.nu * g.nv;
+    }
+    PHC_HIP(hipStreamSynchronize(tv.stream));
+    PHC_HIP(hipStreamSynchronize(wv.stream));
+    const int64_t cnt = A.face_off[LB_FACES];
+    hipLaunchKernelGGL(lb_onpolicy_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, h->stream, A);
     PHC_HIP(hipGetLastError());
     PHC_HIP(hipStreamSynchronize(h->stream));
     return PHC_OK;

@@ -86,6 +86,44 @@ def training_data(lb, n_shots=200, region=64, margin=8, steps=600, collect_every
     return lb.counts()
 
 
+def onpolicy_data(lb, n_shots=60, region=64, margin=8, steps=600, collect_every=6, keep=0.006, s_min=1e-6,
+                  f0_range=(6.0, 14.0), seed=4048, log=None):
+    """Append on-policy samples: each shot runs on the region itself with the
+    model's current weights attached and, in lockstep, on the wide grid; the
+    thin run's own layer states are labelled with the wide run's field beyond
+    the region (phc_lb_collect_onpolicy)."""
+    rng = np.random.default_rng(seed)
+    dt = fd_dt()
+    N = region + 2 * (margin + SPONGE)
+    o = margin + SPONGE
+    t0 = time.time()
+    for shot in range(n_shots):
+        inter, vel = random_layers(rng, region)
+        src = [int(v) for v in rng.integers(8, region - 8, 3)]
+        f0 = float(rng.uniform(*f0_range))
+        ws = phc.FdSetup(N, N, N, L=4, dx=DX, xs=src[0] + o, ys=src[1] + o, zs=src[2] + o, f0=f0, cfl=CFL,
+                         sponge_width=SPONGE, model=layered_vp2dt2((N, N, N), o, inter, vel, dt),
+                         nsamples=steps + 1)
+        ts = phc.FdSetup(region, region, region, L=4, dx=DX, xs=src[0], ys=src[1], zs=src[2], f0=f0, cfl=CFL,
+                         sponge_width=0, model=layered_vp2dt2((region,) * 3, 0, inter, vel, dt),
+                         nsamples=steps + 1)
+        wide = phc.FdSolver(ws, arith=phc.ARITH_FP32, init_value=0.0)
+        thin = phc.FdSolver(ts, arith=phc.ARITH_FP32, init_value=0.0)
+        thin.set_learned_boundary(lb, halo=0.0)
+        for it in range(steps):
+            wide.step(it)
+            thin.step(it)
+            if (it + 1) % collect_every == 0:
+                lb.collect_onpolicy(thin, wide, (o, o, o), halo=0.0, keep=keep, s_min=s_min,
+                                    seed=seed * 100003 + shot * 1009 + it)
+        wide.close()
+        thin.close()
+        if log and (shot + 1) % 20 == 0:
+            log("  on-policy shot %d/%d  samples per face %s  %.1fs" % (shot + 1, n_shots, list(lb.counts()),
+                                                                          time.time() - t0))
+    return lb.counts()
+
+
 def train(lb, steps=6000, batch_per_face=4096, lr=2e-3, lr_final=1e-4, seed=7, log=None):
     """Adam with a geometric learning-rate decay in 6 stages; returns the loss history."""
     hist = []
@@ -124,7 +162,8 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def compare(lb, n=256, steps=2000, f0=10.0, seed=20240512, src=None, with_none=True, log=None):
+def compare(lb, n=256, steps=2000, f0=10.0, seed=20240512, src=None, with_none=True, log=None,
+            layer_damping=0.0, truth=None):
     """One shot on an n^3 region: learned layer vs 40-point sponge (vs no
     absorbing boundary) against a 2n-wide FP64 domain; returns a dict."""
     rng = np.random.default_rng(seed)
@@ -145,7 +184,7 @@ def compare(lb, n=256, steps=2000, f0=10.0, seed=20240512, src=None, with_none=T
         s = setup(pad, sw)
         sol = phc.FdSolver(s, arith=phc.ARITH_FP32, init_value=0.0)
         if model is not None:
-            sol.set_learned_boundary(model, halo=0.0)
+            sol.set_learned_boundary(model, halo=0.0, layer_damping=layer_damping)
         sol.autotune(steps_per_trial=5)
         sol.set_receivers(rec + pad)
         sol.run(0, steps)               # warm-up: captures the run's graph
@@ -165,26 +204,41 @@ def compare(lb, n=256, steps=2000, f0=10.0, seed=20240512, src=None, with_none=T
 
     # truth: 2n region in FP64 with its own sponge, region centred
     pad_t = n // 2 + SPONGE
-    t0 = time.perf_counter()
-    s64 = setup(pad_t, SPONGE)
-    f64 = phc.Fd64Solver(s64, init_value=0.0)
-    f64.set_receivers(rec + pad_t)
-    _, tr_true = f64.run(steps)
-    f64.close()
-    out["truth_wall_s"] = round(time.perf_counter() - t0, 3)
-    if log:
-        log("  truth (FP64, %d^3): %.1fs" % (n + 2 * pad_t, out["truth_wall_s"]))
+    if truth is None:
+        t0 = time.perf_counter()
+        s64 = setup(pad_t, SPONGE)
+        f64 = phc.Fd64Solver(s64, init_value=0.0)
+        f64.set_receivers(rec + pad_t)
+        _, tr_true = f64.run(steps)
+        f64.close()
+        out["truth_wall_s"] = round(time.perf_counter() - t0, 3)
+        if log:
+            log("  truth (FP64, %d^3): %.1fs" % (n + 2 * pad_t, out["truth_wall_s"]))
+    else:
+        tr_true = truth
+    out["truth_traces"] = tr_true
+    out["layer_damping"] = layer_damping
     runs = [("learned", LAYER, 0, lb), ("sponge40", SPONGE, SPONGE, None)]
     if with_none:
         runs.append(("none", LAYER, 0, None))
+    # receiver groups: the centre of each face's 3x3 grid, and the interior line
+    mid = np.linspace(8, n - 9, 3).astype(int)[1]
+    on_face = [(r == 8) | (r == n - 9) for r in rec.T]
+    n_face = np.sum(on_face, axis=0)
+    centre = (n_face == 1) & np.all((rec == mid) | np.array(on_face).T, axis=1)
+    interior = n_face == 0
+    out["receiver_groups"] = {"face_centres": int(centre.sum()), "interior": int(interior.sum())}
     for name, pad, sw, model in runs:
         tr, wall, fin, pts = run32(pad, sw, model)
         out[name] = {"grid": n + 2 * pad, "points": pts, "wall_s": round(wall, 4),
                      "ms_per_step": round(wall / steps * 1e3, 4), "finite": bool(fin),
-                     "trace_rel_l2": rel_l2(tr, tr_true) if fin else None}
+                     "trace_rel_l2": rel_l2(tr, tr_true) if fin else None,
+                     "trace_rel_l2_face_centres": rel_l2(tr[centre], tr_true[centre]) if fin else None,
+                     "trace_rel_l2_interior": rel_l2(tr[interior], tr_true[interior]) if fin else None}
         if log:
-            log("  %-9s grid %d^3: %.3f s, trace rel L2 %s" % (name, n + 2 * pad, wall,
-                                                              out[name]["trace_rel_l2"]))
+            log("  %-9s grid %d^3: %.3f s, trace rel L2 %s (face centres %s, interior %s)" % (
+                name, n + 2 * pad, wall, out[name]["trace_rel_l2"], out[name]["trace_rel_l2_face_centres"],
+                out[name]["trace_rel_l2_interior"]))
     out["speedup_learned_vs_sponge40"] = round(out["sponge40"]["wall_s"] / out["learned"]["wall_s"], 3)
     return out
 
@@ -212,6 +266,8 @@ def main():
     ap.add_argument("--eval-n", type=int, default=256)
     ap.add_argument("--eval-steps", type=int, default=2000)
     ap.add_argument("--skip-train", action="store_true")
+    ap.add_argument("--rounds", type=int, default=2, help="on-policy rounds after the first training")
+    ap.add_argument("--onpolicy-shots", type=int, default=60)
     args = ap.parse_args()
     log = lambda m: print(m, flush=True)
     lb = phc.LearnedBoundary(seed=11)
@@ -222,10 +278,15 @@ def main():
         training_data(lb, n_shots=args.shots, region=args.region, log=log)
         log("training")
         hist = train(lb, steps=args.train_steps, log=log)
+        for r in range(args.rounds):
+            log("on-policy round %d: %d shots" % (r + 1, args.onpolicy_shots))
+            onpolicy_data(lb, n_shots=args.onpolicy_shots, region=args.region, seed=4048 + 17 * r, log=log)
+            hist = train(lb, steps=args.train_steps // 2, lr=5e-4, lr_final=5e-5, seed=100 + r, log=log)
         save_params(lb, args.out)
         log("saved %s (final minibatch loss %.3e)" % (args.out, hist[-1]))
     log("comparison on %d^3, %d steps" % (args.eval_n, args.eval_steps))
     res = compare(lb, n=args.eval_n, steps=args.eval_steps, log=log)
+    res.pop("truth_traces")
     print(json.dumps(res), flush=True)
 
 

@@ -296,10 +296,10 @@ class FdSolver:
     def synchronize(self):
         check(lib().phc_fd_synchronize(self.h))
 
-    def set_learned_boundary(self, lb, halo=None):
+    def set_learned_boundary(self, lb, halo=None, layer_damping=0.0):
         """Attach a LearnedBoundary (None detaches); halo defaults to init_value."""
         h = self.init_value if halo is None else halo
-        check(lib().phc_fd_set_learned_boundary(self.h, lb.h if lb is not None else None, h))
+        check(lib().phc_fd_set_learned_boundary(self.h, lb.h if lb is not None else None, h, layer_damping))
 
     def lb_corrections(self, cur=None):
         """Per-face corrections the next step would add (list of 6 arrays)."""
@@ -538,10 +538,12 @@ SIGNATURES.update({
     "phc_lb_add_samples": (c_int, [c_void_p, c_int, f32p, ctypes.c_int64]),
     "phc_lb_collect": (c_int, [c_void_p, c_void_p, c_int, i32p, i32p, c_float, c_float, c_float,
                                ctypes.c_uint64]),
+    "phc_lb_collect_onpolicy": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, i32p, c_float, c_float,
+                                        c_float, ctypes.c_uint64]),
     "phc_lb_eval": (c_int, [c_void_p, ctypes.POINTER(c_float), c_void_p]),
     "phc_lb_train": (c_int, [c_void_p, c_int, c_int, c_float, ctypes.c_uint64, c_void_p]),
     "phc_lb_infer": (c_int, [c_void_p, c_int, f32p, ctypes.c_int64, f32p]),
-    "phc_fd_set_learned_boundary": (c_int, [c_void_p, c_void_p, c_float]),
+    "phc_fd_set_learned_boundary": (c_int, [c_void_p, c_void_p, c_float, c_float]),
     "phc_fd_lb_corrections": (c_int, [c_void_p, c_int, f32p]),
 })
 
@@ -600,6 +602,11 @@ class LearnedBoundary:
         """Samples from a wide-sponge FdSolver's levels (cur = current level)."""
         check(lib().phc_lb_collect(self.h, wide.h, cur, np.asarray(origin, np.int32),
                                    np.asarray(dims, np.int32), halo, keep, s_min, seed))
+
+    def collect_onpolicy(self, thin, wide, origin, halo=0.0, keep=1.0, s_min=0.0, seed=1):
+        """Samples from a thin run at step n and a wide run at step n+1 (FdSolvers)."""
+        check(lib().phc_lb_collect_onpolicy(self.h, thin.h, thin.i2, wide.h, wide.i2,
+                                            np.asarray(origin, np.int32), halo, keep, s_min, seed))
 
     def eval(self, with_grad=True):
         loss = c_float()

@@ -177,3 +177,45 @@ def test_training_reduces_the_loss(gpu):
     l1, _ = lb.eval(with_grad=False)
     assert np.all(np.isfinite(hist))
     assert l1 < 0.5 * l0, (l0, l1)
+
+
+def test_onpolicy_samples_match_oracle(gpu):
+    banner("on-policy collection vs oracle")
+    import learned
+    rng = np.random.default_rng(21)
+    region, margin = 32, 6
+    o = margin + 10
+    N = region + 2 * o
+    dt = learned.fd_dt()
+    inter, vel = learned.random_layers(rng, region)
+    src = (o + 15, o + 13, o + 17)
+    ws = gpu.FdSetup(N, N, N, L=4, xs=src[0], ys=src[1], zs=src[2], f0=20.0, cfl=0.4, sponge_width=10,
+                     model=learned.layered_vp2dt2((N, N, N), o, inter, vel, dt), nsamples=120)
+    ts = gpu.FdSetup(region, region, region, L=4, xs=15, ys=13, zs=17, f0=20.0, cfl=0.4, sponge_width=0,
+                     model=learned.layered_vp2dt2((region,) * 3, 0, inter, vel, dt), nsamples=120)
+    wide = gpu.FdSolver(ws, arith=gpu.ARITH_FP32, init_value=0.0)
+    thin = gpu.FdSolver(ts, arith=gpu.ARITH_FP32, init_value=0.0)
+    lb = gpu.LearnedBoundary(seed=6)
+    thin.set_learned_boundary(lb, halo=0.0)
+    for it in range(70):
+        wide.step(it)
+        thin.step(it)
+    lb.set_capacity(100000)
+    lb.collect_onpolicy(thin, wide, (o, o, o), halo=0.0)
+    tpn, tpm = thin.current().astype(np.float64), thin.previous().astype(np.float64)
+    wc = wide.current().astype(np.float64)
+    vp3 = ts.vp.reshape(region, region, region)
+    c1 = float(ts.cx[1])
+    chat = ts.cx.astype(np.float64) / c1
+    total = 0
+    for f in range(lo.FACES):
+        S = lb.samples(f)
+        X, sc, d, t, ok = lo.onpolicy_targets(tpn, tpm, vp3, wc, (o, o, o), (region,) * 3, f, 0.0, chat, c1)
+        assert len(S) == int(ok.sum()), f
+        idx = S[:, 21].astype(int)
+        assert np.all(ok[idx])
+        assert np.allclose(S[:, :lo.NIN], X[idx], rtol=1e-5, atol=1e-6)
+        assert np.allclose(S[:, lo.NIN], t[idx], rtol=1e-4, atol=1e-5)
+        assert np.array_equal(S[:, lo.NIN + 1].astype(int), d[idx])
+        total += len(S)
+    assert total > 1000
