@@ -214,6 +214,24 @@ phc_status phc_sem_synchronize(phc_sem *h);
  * A no-op returning zeros for the element kernel. */
 phc_status phc_sem_autotune(phc_sem *h, int steps_per_trial, const char *cache_path, int *tile_j,
                             int *block_waves, int *chunk_i);
+/* ==================================================================== SEM64
+ * SEMsolver::computeOneStep + the SEMproxy loop (SEMsolver.cpp:21-125,
+ * SEMproxy.cpp:41-119) in FP64 on the structured box: fields, lumped mass,
+ * dt^2 W/M and the assembled 1D operators all double, source samples double.
+ * The accuracy yardstick of the FP32 SEM paths (the kernel and tiling fields
+ * of phc_sem_params are ignored; the source must be on an interior node).
+ * Synchronous.  Levels start as i1 = 0, i2 = 1 (SEMproxy.cpp:41-45).
+ */
+typedef struct phc_sem64 phc_sem64;
+phc_status phc_sem64_create(const phc_sem_params *p, phc_sem64 **out);
+phc_status phc_sem64_destroy(phc_sem64 *h);
+phc_status phc_sem64_set_receivers(phc_sem64 *h, int nrec, const int *nodes);
+phc_status phc_sem64_run(phc_sem64 *h, int it0, int nsteps);
+/* Last run: src[nsteps] = source node after each step, rec[nrec*nsteps]. */
+phc_status phc_sem64_get_traces(phc_sem64 *h, double *src, double *rec);
+/* current != 0: the field after the last step (level i2), else level i1; node order. */
+phc_status phc_sem64_get_field(phc_sem64 *h, int current, double *host);
+
 /* SEMmesh::getElementNumberFromPoints (SEMmesh.cpp:266-285). */
 int phc_sem_element_from_point(int ex, int ey, int ez, float lx, float ly, float lz, float x,
                                float y, float z);

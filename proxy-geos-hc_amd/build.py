@@ -26,7 +26,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + INCLUDE, "-I" + CSRC]
 
-HIP_SOURCES = ["phc_common.hip", "setup.hip", "fd.hip", "sem.hip", "batch.hip", "mlp.hip", "fd64.hip"]
+HIP_SOURCES = ["phc_common.hip", "setup.hip", "fd.hip", "sem.hip", "batch.hip", "mlp.hip", "fd64.hip", "sem64.hip"]
 
 
 def _run(cmd):

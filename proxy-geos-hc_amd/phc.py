@@ -680,6 +680,81 @@ def sem_model_gaussian(ex, ey, ez, sigma=5.0, vmin=2000.0, vmax=4000.0, seed=1):
     return out
 
 
+SIGNATURES.update({
+    "phc_sem64_create": (c_int, [ctypes.POINTER(SemParams), ctypes.POINTER(c_void_p)]),
+    "phc_sem64_destroy": (c_int, [c_void_p]),
+    "phc_sem64_set_receivers": (c_int, [c_void_p, c_int, c_void_p]),
+    "phc_sem64_run": (c_int, [c_void_p, c_int, c_int]),
+    "phc_sem64_get_traces": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "phc_sem64_get_field": (c_int, [c_void_p, c_int, f64p]),
+})
+
+
+class Sem64Solver:
+    """The SEM loop in FP64 (accuracy yardstick), same arguments as SemSolver."""
+
+    def __init__(self, ex=10, lx=2000.0, order=2, dt=0.001, f0=10.0, model=None, nsamples=100,
+                 src_point=(1001.0, 1001.0, 1001.0), ey=None, ez=None, ly=None, lz=None,
+                 weights_ref=False, source_order=1, rhs=None):
+        ey = ex if ey is None else ey
+        ez = ex if ez is None else ez
+        ly = lx if ly is None else ly
+        lz = lx if lz is None else lz
+        nE = ex * ey * ez
+        self.model = (np.full(nE, 1500.0, np.float32) if model is None
+                      else np.ascontiguousarray(model, np.float32))
+        self.dt = np.float32(dt)
+        self.rhs = (source_term(nsamples, self.dt, f0, source_order, EXP_FLOAT) if rhs is None
+                    else np.ascontiguousarray(rhs, np.float32))
+        self.src_element = lib().phc_sem_element_from_point(ex, ey, ez, lx, ly, lz, *src_point)
+        p = SemParams()
+        p.ex, p.ey, p.ez = ex, ey, ez
+        p.lx, p.ly, p.lz = lx, ly, lz
+        p.order = order
+        p.model = _ptr(self.model)
+        p.dt = self.dt
+        p.rhs = _ptr(self.rhs)
+        p.nsamples = len(self.rhs)
+        p.src_element = self.src_element
+        p.weights_ref = 1 if weights_ref else 0
+        h = c_void_p()
+        check(lib().phc_sem64_create(ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        self.nx, self.ny, self.nz = ex * order + 1, ey * order + 1, ez * order + 1
+        self.nn = self.nx * self.ny * self.nz
+        self.nrec = 0
+        self.it = 0
+
+    def close(self):
+        if self.h:
+            check(lib().phc_sem64_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_receivers(self, nodes):
+        a = np.ascontiguousarray(nodes, np.int32)
+        self.nrec = len(a)
+        check(lib().phc_sem64_set_receivers(self.h, self.nrec, _ptr(a)))
+
+    def run(self, nsteps):
+        check(lib().phc_sem64_run(self.h, self.it, nsteps))
+        self.it += nsteps
+        src = np.zeros(nsteps)
+        rec = np.zeros(max(self.nrec, 1) * nsteps)
+        check(lib().phc_sem64_get_traces(self.h, _ptr(src), _ptr(rec) if self.nrec else None))
+        return src, (rec.reshape(self.nrec, nsteps) if self.nrec else None)
+
+    def field(self, current=True):
+        out = np.zeros(self.nn)
+        check(lib().phc_sem64_get_field(self.h, 1 if current else 0, out))
+        return out
+
+
 class SemSolver:
     """SEMsolver (src/sem/solver/SEMsolver.hpp:18-91) on the device: the
     structured box of SEMmesh, per-element wave speed, Ricker source in the
