@@ -102,6 +102,20 @@ phc_status phc_fd_compute_one_step_sb(phc_fd *h, int ca, int cb);
  * read them with phc_fd_get_traces. */
 phc_status phc_fd_run(phc_fd *h, int it0, int nsteps, int *cur);
 
+/* PML boundary (FDTDInit -usePML, FDTDinit.hpp:155-198, 263-275;
+ * FDTDKernel::computeOneStepPML / pml3D, FDTDkernels.hpp:36-104): slabs of
+ * ndamp[a] points at both ends of axis a run pml3D with
+ * eta(i,j,k) = etax[i+1] + etay[j+1] + etaz[k+1] (profiles over -1..n, n+2
+ * values, e.g. from phc_fd_pml_profile) and hd2[a] = 1/(4 d_a^2); the rest
+ * runs inner3D; the auxiliary field phi starts at 0.  Needs sponge_width 0.
+ * ndamp all 0 removes the PML. */
+phc_status phc_fd_set_pml(phc_fd *h, const int *ndamp, const float *etax, const float *etay, const float *etaz,
+                          const float *hd2);
+/* FDTDKernel::computeOneStepPML (FDTDkernels.hpp:87-104): level ca <- new
+ * level from cb (and ca), pml3D on the slabs, inner3D inside.  phc_fd_run
+ * uses it when a PML is set. */
+phc_status phc_fd_compute_one_step_pml(phc_fd *h, int ca, int cb);
+
 /* Receivers: interior (i,j,k) triples; replaces the previous set. */
 phc_status phc_fd_set_receivers(phc_fd *h, int nrec, const int *ijk);
 /* Traces of the last phc_fd_run: src[nsteps], rec[nrec*nsteps] (receiver-major).

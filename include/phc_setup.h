@@ -52,6 +52,15 @@ phc_status phc_fd_model_layered(int nx, int ny, int nz, int nlayers, float vtop,
 phc_status phc_sem_model_gaussian(int ex, int ey, int ez, double sigma, float vmin, float vmax,
                                   uint64_t seed, float *model);
 
+/* PML setup of FDTDInit -usePML (FDTDinit.hpp:155-173) and FDTDUtils::init_eta
+ * (FDTDutils.hpp:84-107, 129-160), float as the reference computes it:
+ * width = (int)(4 * vmin/(2.5 f0) / d); scale = dt*3*vmax*logf(1000)/(2*ndamp*d);
+ * profile[0..n+1] (grid -1..n) = 0 except quadratic ramps of ndamp points
+ * at both ends, scale at profile[0] and profile[n+1]. */
+int phc_fd_pml_width(float vmin, float f0, float d);
+float phc_fd_pml_scale(float dt, float vmax, int ndamp, float d);
+phc_status phc_fd_pml_profile(int n, int ndamp, float scale, float *profile);
+
 #ifdef __cplusplus
 }
 #endif

@@ -327,3 +327,101 @@ void orc_fd_run_f64(const orc_fd_grid *g, const double *cx, const double *cy,
     }
     *cur_is_p1 = (cur == p1);
 }
+
+/* ------------------------------------------------------------------ PML */
+
+void orc_fd_pml_profile(int n, int nd, float scale, float *profile)
+{
+    /* FDTDutils.hpp:84-107 pml_profile_init(profile, 0, n+1, nd, nd, scale):
+     * indices 0..n+1 (grid -1..n); quadratic ramps of nd points at each end,
+     * scale at the outermost index. */
+    const int i_min = 0, i_max = n + 1;
+    const int nn = i_max - i_min + 1, shift = i_min - 1;
+    const int first_beg = 1 + shift, first_end = nd + shift;
+    const int last_beg = nn - nd + 1 + shift, last_end = nn + shift;
+    for (int i = i_min; i <= i_max; i++) profile[i] = 0.f;
+    const int w = first_end - first_beg + 1;
+    const float tmp = scale / (float)(w * w);
+    for (int i = 1; i <= first_end - first_beg + 1; i++) profile[first_end - i + 1] = (float)(i * i) * tmp;
+    for (int i = 1; i <= last_end - last_beg + 1; i++) profile[last_beg + i - 1] = (float)(i * i) * tmp;
+}
+
+float orc_fd_pml_scale(float dt, float vmax, int nd, float d)
+{
+    /* FDTDutils.hpp:149-160: dt_sch*3.f*vmax*logf(1000.f)/(2.f*ndamp*dx) in float */
+    return dt * 3.f * vmax * logf(1000.f) / (2.f * nd * d);
+}
+
+/* eta at grid (i,j,k), i in -1..nx: the stored etax[i+1] + etay[j+1] + etaz[k+1]
+ * (FDTDutils.hpp:110-127, 162-170: the extension writes the sum of the three
+ * profiles on every region, and the profiles vanish on the inner box) */
+static inline float pml_eta(const float *ex, const float *ey, const float *ez, int i, int j, int k)
+{
+    return ex[i + 1] + ey[j + 1] + ez[k + 1];
+}
+
+void orc_fd_pml_step(const orc_fd_grid *g, const int *nd, const float *cx, const float *cy, const float *cz,
+                     double coef0, const float *vp, const float *ex, const float *ey, const float *ez,
+                     const float *hd2, float *phi, float *p_ca, const float *p_cb)
+{
+    /* FDTDkernels.hpp:87-104 computeOneStepPML: pml3D on the six slabs of
+     * width nd (FDTDinit.hpp:167-198 region bounds), inner3D on the rest.
+     * Every point reads level cb and its own point of ca and phi, so the
+     * order of the seven region loops does not matter. */
+    const int64_t sj = g->nz + 2 * g->lz;
+    const int64_t si = sj * (g->ny + 2 * g->ly);
+    const int x3 = nd[0], x4 = g->nx - nd[0], y3 = nd[1], y4 = g->ny - nd[1], z3 = nd[2], z4 = g->nz - nd[2];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < g->nx; i++)
+        for (int j = 0; j < g->ny; j++)
+            for (int k = 0; k < g->nz; k++) {
+                const int64_t c = orc_idx3_l(g, i, j, k);
+                const int64_t q = orc_idx3(g, i, j, k);
+                float lapx = 0;
+                for (int l = 1; l <= g->lx; l++) lapx += cx[l] * (p_cb[c + l * si] + p_cb[c - l * si]);
+                float lapy = 0;
+                for (int l = 1; l <= g->ly; l++) lapy += cy[l] * (p_cb[c + l * sj] + p_cb[c - l * sj]);
+                float lapz = 0;
+                for (int l = 1; l <= g->lz; l++) lapz += cz[l] * (p_cb[c + l] + p_cb[c - l]);
+                const double pc = (double)p_cb[c];
+                if (i >= x3 && i < x4 && j >= y3 && j < y4 && k >= z3 && k < z4) {
+                    /* inner3D (FDTDkernels.hpp:8-34) */
+                    p_ca[c] = (float)((2. * pc - (double)p_ca[c]) +
+                                      (double)vp[q] * (((coef0 * pc + (double)lapx) + (double)lapy) + (double)lapz));
+                    continue;
+                }
+                /* pml3D (FDTDkernels.hpp:36-73) */
+                const float lap = (float)(((coef0 * pc + (double)lapx) + (double)lapy) + (double)lapz);
+                const float e = pml_eta(ex, ey, ez, i, j, k);
+                const double num = (((2. - (double)(e * e)) + 2. * (double)e) * pc - (double)p_ca[c]) +
+                                   (double)(vp[q] * (phi[q] + lap));
+                p_ca[c] = (float)(num / (1. + 2. * (double)e));
+                const float t = (pml_eta(ex, ey, ez, i + 1, j, k) - pml_eta(ex, ey, ez, i - 1, j, k)) *
+                                    (p_cb[c + si] - p_cb[c - si]) * hd2[0] +
+                                (pml_eta(ex, ey, ez, i, j + 1, k) - pml_eta(ex, ey, ez, i, j - 1, k)) *
+                                    (p_cb[c + sj] - p_cb[c - sj]) * hd2[1] +
+                                (pml_eta(ex, ey, ez, i, j, k + 1) - pml_eta(ex, ey, ez, i, j, k - 1)) *
+                                    (p_cb[c + 1] - p_cb[c - 1]) * hd2[2];
+                phi[q] = (float)((double)(phi[q] - t) / (1. + (double)e));
+            }
+}
+
+void orc_fd_run_pml(const orc_fd_grid *g, const int *nd, const float *cx, const float *cy, const float *cz,
+                    double coef0, const float *vp, const float *ex, const float *ey, const float *ez,
+                    const float *hd2, float *phi, const float *rhs, int it0, int nsteps, float *p0, float *p1,
+                    int *cur_is_p1, float *trace_src, int nrec, const int *rec_ijk, float *rec_val)
+{
+    /* FDTDmain.cpp:55-81 with usePML: addRHS(i2) -> computeOneStepPML(i1,i2) -> swap */
+    float *prev = *cur_is_p1 ? p0 : p1;
+    float *cur = *cur_is_p1 ? p1 : p0;
+    for (int s = 0; s < nsteps; s++) {
+        orc_fd_add_rhs(g, cur, vp, rhs[it0 + s]);
+        orc_fd_pml_step(g, nd, cx, cy, cz, coef0, vp, ex, ey, ez, hd2, phi, prev, cur);
+        float *t = prev; prev = cur; cur = t;
+        if (trace_src) trace_src[s] = cur[orc_idx3_l(g, g->xs, g->ys, g->zs)];
+        for (int r = 0; r < nrec; r++)
+            rec_val[(int64_t)r * nsteps + s] =
+                cur[orc_idx3_l(g, rec_ijk[3 * r], rec_ijk[3 * r + 1], rec_ijk[3 * r + 2])];
+    }
+    *cur_is_p1 = (cur == p1);
+}

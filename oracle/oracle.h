@@ -109,6 +109,21 @@ void orc_fd_run_f64(const orc_fd_grid *g, const double *cx, const double *cy,
                     double *p0, double *p1, int *cur_is_p1, double *trace_src,
                     int nrec, const int *rec_ijk, double *rec_val);
 
+/* PML (FDTDkernels.hpp:36-104, FDTDutils.hpp:84-171).  pml_profile_init of
+ * one axis (profile[0..n+1], grid -1..n); init_eta's scale; one
+ * computeOneStepPML (pml3D on the slabs of width nd[3], inner3D inside, eta =
+ * ex[i+1] + ey[j+1] + ez[k+1], hd2 = 1/(4 dx^2) per axis, phi interior-sized);
+ * and the time loop with it. */
+void orc_fd_pml_profile(int n, int nd, float scale, float *profile);
+float orc_fd_pml_scale(float dt, float vmax, int nd, float d);
+void orc_fd_pml_step(const orc_fd_grid *g, const int *nd, const float *cx, const float *cy, const float *cz,
+                     double coef0, const float *vp, const float *ex, const float *ey, const float *ez,
+                     const float *hd2, float *phi, float *p_ca, const float *p_cb);
+void orc_fd_run_pml(const orc_fd_grid *g, const int *nd, const float *cx, const float *cy, const float *cz,
+                    double coef0, const float *vp, const float *ex, const float *ey, const float *ez,
+                    const float *hd2, float *phi, const float *rhs, int it0, int nsteps, float *p0, float *p1,
+                    int *cur_is_p1, float *trace_src, int nrec, const int *rec_ijk, float *rec_val);
+
 /* ------------------------------------------------------------------- SEM */
 /* GLL points of order p (SEMQkGLBasisFunctions.hpp:36-78), from the roots of
  * P'_p by Newton iteration in double. */

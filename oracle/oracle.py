@@ -66,6 +66,14 @@ def lib():
         L.orc_fd_run.argtypes = [P(FdGrid), f32p, f32p, f32p, c_double, f32p, f32p, f32p,
                                  c_int, c_int, f32p, f32p, P(c_int), ctypes.c_void_p,
                                  c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_fd_pml_profile.restype = None
+        L.orc_fd_pml_profile.argtypes = [c_int, c_int, c_float, f32p]
+        L.orc_fd_pml_scale.restype = c_float
+        L.orc_fd_pml_scale.argtypes = [c_float, c_float, c_int, c_float]
+        L.orc_fd_run_pml.restype = None
+        L.orc_fd_run_pml.argtypes = [P(FdGrid), i32p, f32p, f32p, f32p, c_double, f32p, f32p, f32p, f32p,
+                                     f32p, f32p, f32p, c_int, c_int, f32p, f32p, P(c_int), ctypes.c_void_p,
+                                     c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_fd_run_f64.restype = None
         L.orc_fd_run_f64.argtypes = [P(FdGrid), f64p, f64p, f64p, c_double, f64p, f64p, f64p,
                                      c_int, c_int, f64p, f64p, P(c_int), ctypes.c_void_p,
@@ -141,7 +149,7 @@ class FdProblem:
     def __init__(self, nx=150, ny=None, nz=None, L=4, dx=10.0, xs=None, ys=None, zs=None,
                  f0=15.0, time_max=1.0, vmin=1500.0, vmax=4500.0, cfl=0.8,
                  source_order=1, sponge_width=20, sponge_alpha=-0.00015,
-                 model="layered2", dt=None, nsamples=None):
+                 model="layered2", dt=None, nsamples=None, pml=False, ndamp=None):
         ny = nx if ny is None else ny
         nz = nx if nz is None else nz
         self.nx, self.ny, self.nz, self.L = nx, ny, nz, L
@@ -182,6 +190,22 @@ class FdProblem:
         lib().orc_fd_sponge(nx, ny, nz, sponge_width, sponge_alpha, self.sponge)
         self.sponge_width = sponge_width
         self.sponge_alpha = sponge_alpha
+        # PML (FDTDinit.hpp:155-198, FDTDutils.hpp:129-171): width ntaper *
+        # lambdamax / dx with ntaper = 4, lambdamax = vmin / (2.5 f0)
+        self.pml = pml
+        if pml:
+            fmax = np.float32(np.float32(2.5) * np.float32(f0))
+            lam = np.float32(np.float32(vmin) / fmax)
+            nd = int(np.float32(np.float32(4 * lam) / np.float32(dx))) if ndamp is None else ndamp
+            self.ndamp = np.array([nd, nd, nd], np.int32)
+            self.eta = []
+            for n in (nx, ny, nz):
+                prof = np.zeros(n + 2, np.float32)
+                if nd > 0:
+                    lib().orc_fd_pml_profile(n, nd, lib().orc_fd_pml_scale(self.dt, vmax, nd, dx), prof)
+                self.eta.append(prof)
+            self.hd2 = np.full(3, np.float32(1.0 / (4.0 * dx * dx)), np.float32)
+            self.phi = np.zeros(nx * ny * nz, np.float32)
         ext = (nx + 2 * L) * (ny + 2 * L) * (nz + 2 * L)
         self.p0 = np.full(ext, np.float32(0.000001), np.float32)
         self.p1 = np.full(ext, np.float32(0.000001), np.float32)
@@ -198,9 +222,15 @@ class FdProblem:
         nrec = 0 if receivers is None else len(receivers)
         rec = None if not nrec else np.ascontiguousarray(receivers, np.int32).reshape(-1)
         rv = None if not nrec else np.zeros(nrec * nsteps, np.float32)
-        lib().orc_fd_run(ctypes.byref(self.g), self.cx, self.cy, self.cz, self.coef0, self.vp,
-                         self.sponge, self.rhs, self.it, nsteps, self.p0, self.p1,
-                         ctypes.byref(self.cur_is_p1), _vp(tr), nrec, _vp(rec), _vp(rv))
+        if self.pml:
+            lib().orc_fd_run_pml(ctypes.byref(self.g), self.ndamp, self.cx, self.cy, self.cz, self.coef0,
+                                 self.vp, self.eta[0], self.eta[1], self.eta[2], self.hd2, self.phi, self.rhs,
+                                 self.it, nsteps, self.p0, self.p1, ctypes.byref(self.cur_is_p1), _vp(tr), nrec,
+                                 _vp(rec), _vp(rv))
+        else:
+            lib().orc_fd_run(ctypes.byref(self.g), self.cx, self.cy, self.cz, self.coef0, self.vp,
+                             self.sponge, self.rhs, self.it, nsteps, self.p0, self.p1,
+                             ctypes.byref(self.cur_is_p1), _vp(tr), nrec, _vp(rec), _vp(rv))
         self.it += nsteps
         return tr, (None if rv is None else rv.reshape(nrec, nsteps))
 

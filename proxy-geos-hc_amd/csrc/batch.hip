@@ -48,6 +48,7 @@ struct FdGrid {
     int inj;
     float rhs_next;
     float *trace;
+    int ilo, ihi, jlo, jhi, klo, khi;  // every point of the shot
 };
 
 // Work queues of one step: queue q holds items[start[q] .. start[q]+len[q]),
@@ -99,6 +100,9 @@ fd_batch_kernel(FdStepArgs c, const ShotDesc *__restrict__ shots, const WorkItem
             g.inj = inj;
             g.rhs_next = inj ? sd.rhs[it + 1] : 0.f;
             g.trace = traces + (int64_t)wi.shot * trace_stride + step;
+        g.ilo = 0; g.ihi = sd.L.nx;
+        g.jlo = 0; g.jhi = sd.L.ny;
+        g.klo = 0; g.khi = sd.L.nz;
             fd_tile<R, TY, NJ, ARITH, IT>(c, g, wi.kt, wi.jt, wi.ic, tile);
         }
     }

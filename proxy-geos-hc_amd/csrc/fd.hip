@@ -83,6 +83,127 @@ __global__ void fd_sample_kernel(const float *p, const int64_t *offs, int nrec, 
     out[(int64_t)r * stride + step] = (o == src_off) ? src_trace[step] : p[o];
 }
 
+// PML step on the six slabs of the grid (FDTDkernels.hpp:36-73 pml3D, the
+// regions of computeOneStepPML FDTDkernels.hpp:87-104): one thread per point,
+// stencil values from cache.  The slab points are walked as rows (blockIdx.y,
+// decoded once per block): i-slabs (all j, k along the row), j-slabs inside
+// the i range (k along the row), k-slabs inside both (j along the row).
+// eta(i,j,k) = ex[i+1] + ey[j+1] + ez[k+1] (profiles over -1..n).  With
+// ARITH_REF each value is rounded where the reference's expression rounds:
+// float stencil sums, lap formed in double and stored as float, the update
+// in double; phi in float with a double division.
+struct FdPmlArgs {
+    const float *pn;
+    float *pnm1;
+    const float *vp;
+    float *phi;
+    const float *ex, *ey, *ez;
+    FdLayout L;
+    float cx[FD_MAX_R + 1], cy[FD_MAX_R + 1], cz[FD_MAX_R + 1];
+    double coef0;
+    float c0f;
+    float hd2[3];
+    int x3, x4, y3, y4, z3, z4;  // inner box
+    int rowsA, rowsB, rowsC;
+    int64_t src_off;
+    int inj;
+    float rhs_next;
+    float *trace;
+};
+
+template <int R, int ARITH>
+__global__ void __launch_bounds__(128) fd_pml_kernel(FdPmlArgs a)
+{
+#pragma clang fp contract(off)
+    const FdLayout &L = a.L;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int row = blockIdx.y;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (row < a.rowsA) {
+        const int q = row / ny;
+        j = row - q * ny;
+        i = q < a.x3 ? q : a.x4 + (q - a.x3);
+        k = t;
+        if (k >= nz) return;
+    } else if (row < a.rowsA + a.rowsB) {
+        const int r = row - a.rowsA;
+        const int nyb = a.y3 + ny - a.y4;
+        const int q = r % nyb;
+        i = a.x3 + r / nyb;
+        j = q < a.y3 ? q : a.y4 + (q - a.y3);
+        k = t;
+        if (k >= nz) return;
+    } else {
+        const int r = row - a.rowsA - a.rowsB;
+        const int nzb = a.z3 + nz - a.z4;
+        const int q = r % nzb;
+        i = a.x3 + r / nzb;
+        k = q < a.z3 ? q : a.z4 + (q - a.z3);
+        j = a.y3 + t;
+        if (j >= a.y4) return;
+    }
+    const int64_t c = fd_off(L, i, j, k);
+    const int64_t si = L.pitch_j, sj = L.pitch_k;
+    const float *pn = a.pn;
+    float lapx = 0.f, lapy = 0.f, lapz = 0.f;
+#pragma unroll
+    for (int l = 1; l <= R; ++l) lapx += a.cx[l] * (pn[c + l * si] + pn[c - l * si]);
+#pragma unroll
+    for (int l = 1; l <= R; ++l) lapy += a.cy[l] * (pn[c + l * sj] + pn[c - l * sj]);
+#pragma unroll
+    for (int l = 1; l <= R; ++l) lapz += a.cz[l] * (pn[c + l] + pn[c - l]);
+    const float pc = pn[c], pm = a.pnm1[c], v = a.vp[c], ph = a.phi[c];
+    const float fy0 = a.ey[j + 1], fz0 = a.ez[k + 1];
+    const float e = (a.ex[i + 1] + fy0) + fz0;
+    const float t1 = (((a.ex[i + 2] + fy0) + fz0) - ((a.ex[i] + fy0) + fz0)) * (pn[c + si] - pn[c - si]) * a.hd2[0];
+    const float t2 = (((a.ex[i + 1] + a.ey[j + 2]) + fz0) - ((a.ex[i + 1] + a.ey[j]) + fz0)) *
+                     (pn[c + sj] - pn[c - sj]) * a.hd2[1];
+    const float t3 = (((a.ex[i + 1] + fy0) + a.ez[k + 2]) - ((a.ex[i + 1] + fy0) + a.ez[k])) *
+                     (pn[c + 1] - pn[c - 1]) * a.hd2[2];
+    const float tg = (t1 + t2) + t3;
+    float val, phin;
+    if constexpr (ARITH == PHC_FD_ARITH_REF) {
+        const double dpc = (double)pc;
+        const float lap = (float)(((a.coef0 * dpc + (double)lapx) + (double)lapy) + (double)lapz);
+        const double de = (double)e;
+        const double num = (((2.0 - (double)(e * e)) + 2.0 * de) * dpc - (double)pm) + (double)(v * (ph + lap));
+        val = (float)(num / (1.0 + 2.0 * de));
+        phin = (float)((double)(ph - tg) / (1.0 + de));
+    } else {
+        const float lap = ((a.c0f * pc + lapx) + lapy) + lapz;
+        val = ((2.f - e * e + 2.f * e) * pc - pm + v * (ph + lap)) / (1.f + 2.f * e);
+        phin = (ph - tg) / (1.f + e);
+    }
+    a.phi[c] = phin;
+    if (c == a.src_off) {
+        if (a.trace) *a.trace = val;
+        if (a.inj) val = add_nc(val, mul_nc(v, a.rhs_next));
+    }
+    a.pnm1[c] = val;
+}
+
+using FdPmlFn = void (*)(FdPmlArgs);
+
+template <int R>
+static FdPmlFn pml_arith(int arith)
+{
+    return arith == PHC_FD_ARITH_REF ? fd_pml_kernel<R, PHC_FD_ARITH_REF> : fd_pml_kernel<R, PHC_FD_ARITH_FP32>;
+}
+
+static FdPmlFn fd_pick_pml(int R, int arith)
+{
+    switch (R) {
+    case 1: return pml_arith<1>(arith);
+    case 2: return pml_arith<2>(arith);
+    case 3: return pml_arith<3>(arith);
+    case 4: return pml_arith<4>(arith);
+    case 5: return pml_arith<5>(arith);
+    case 6: return pml_arith<6>(arith);
+    default: return nullptr;
+    }
+}
+
 // ------------------------------------------------------------ dispatching
 
 using FdKernelFn = void (*)(FdStepArgs);
@@ -168,6 +289,11 @@ struct phc_fd {
     int last_n = 0, last_nrec = 0;
     int timing = 0;
     double last_ms = 0.0;
+    // PML (phc_fd_set_pml): slab widths, eta profiles, phi, 1/(4 d^2)
+    int pml_on = 0;
+    int pml_nd[3] = {0, 0, 0};
+    DevBuf pml_ex, pml_ey, pml_ez, pml_phi;
+    float pml_hd2[3] = {0.f, 0.f, 0.f};
     // learned thin boundary (phc_fd_set_learned_boundary)
     int lb_on = 0;
     LbInfer lbi;
@@ -268,11 +394,15 @@ static void fd_fill_args(const phc_fd *h, FdStepArgs &a, int ca, int cb)
     a.tiles_k = (h->nz + FD_TK - 1) / FD_TK;
     a.tiles_j = (h->ny + TJ - 1) / TJ;
     a.chunk_i = h->chunk_i;
+    // with a PML the stencil kernel updates the inner box only (FDTDinit.hpp:176-198)
+    a.ilo = h->pml_nd[0]; a.ihi = h->nx - h->pml_nd[0];
+    a.jlo = h->pml_nd[1]; a.jhi = h->ny - h->pml_nd[1];
+    a.klo = h->pml_nd[2]; a.khi = h->nz - h->pml_nd[2];
 }
 
-static int fd_num_blocks(const FdStepArgs &a, int nx)
+static int fd_num_blocks(const FdStepArgs &a)
 {
-    const int chunks = (nx + a.chunk_i - 1) / a.chunk_i;
+    const int chunks = std::max(1, (a.ihi - a.ilo + a.chunk_i - 1) / a.chunk_i);
     return a.tiles_k * a.tiles_j * chunks;
 }
 
@@ -295,7 +425,7 @@ static phc_status fd_launch_step(phc_fd *h, FdStepArgs a, bool damp)
                                    !h->index64 && h->L.total * 4 < ((int64_t)1 << 32));
     if (!fn) return fail(PHC_ERR_INVALID, "no kernel for R=%d block_waves=%d rows/thread=%d", h->R, h->ty, h->nj);
     a.damp_prev = damp ? 1 : 0;
-    const int nb = fd_num_blocks(a, h->nx);
+    const int nb = fd_num_blocks(a);
     hipLaunchKernelGGL(fn, dim3(nb), dim3(FD_TK, h->ty), 0, h->stream, a);
     PHC_HIP(hipGetLastError());
     return PHC_OK;
@@ -311,11 +441,51 @@ static void fd_drop_graph(phc_fd *h)
 // One full step from levels cb (current) and ca (previous, overwritten by the
 // new level): with the learned boundary, its correction pass before and its
 // apply pass after the stencil launch.
+static phc_status fd_launch_pml(phc_fd *h, const FdStepArgs &a, int ca, int cb)
+{
+    FdPmlArgs P;
+    P.pn = h->lev[cb].as<float>();
+    P.pnm1 = h->lev[ca].as<float>();
+    P.vp = h->vp.as<float>();
+    P.phi = h->pml_phi.as<float>();
+    P.ex = h->pml_ex.as<float>();
+    P.ey = h->pml_ey.as<float>();
+    P.ez = h->pml_ez.as<float>();
+    P.L = h->L;
+    for (int l = 0; l <= FD_MAX_R; ++l) {
+        P.cx[l] = h->cx[l];
+        P.cy[l] = h->cy[l];
+        P.cz[l] = h->cz[l];
+    }
+    P.coef0 = h->coef0;
+    P.c0f = (float)h->coef0;
+    for (int d = 0; d < 3; ++d) P.hd2[d] = h->pml_hd2[d];
+    P.x3 = h->pml_nd[0]; P.x4 = h->nx - h->pml_nd[0];
+    P.y3 = h->pml_nd[1]; P.y4 = h->ny - h->pml_nd[1];
+    P.z3 = h->pml_nd[2]; P.z4 = h->nz - h->pml_nd[2];
+    P.rowsA = (P.x3 + h->nx - P.x4) * h->ny;
+    P.rowsB = (P.x4 - P.x3) * (P.y3 + h->ny - P.y4);
+    P.rowsC = (P.x4 - P.x3) * (P.z3 + h->nz - P.z4);
+    P.src_off = a.src_off;
+    P.inj = a.inj;
+    P.rhs_next = a.rhs_next;
+    P.trace = a.trace;
+    const int rows = P.rowsA + P.rowsB + P.rowsC;
+    if (rows == 0) return PHC_OK;
+    FdPmlFn fn = fd_pick_pml(h->R, h->arith);
+    if (!fn) return fail(PHC_ERR_INVALID, "no PML kernel for half width %d", h->R);
+    const int len = std::max(h->nz, P.y4 - P.y3);
+    hipLaunchKernelGGL(fn, dim3((len + 127) / 128, rows), dim3(128), 0, h->stream, P);
+    PHC_HIP(hipGetLastError());
+    return PHC_OK;
+}
+
 static phc_status fd_launch_full_step(phc_fd *h, const FdStepArgs &a, bool damp, int ca, int cb)
 {
     if (h->lb_on)
         PHC_HIP(lb_enqueue_correct(h->lbi, h->L, h->lev[cb].as<float>(), h->lev[ca].as<float>(),
                                    h->vp.as<float>(), h->stream));
+    if (h->pml_on) PHC_TRY(fd_launch_pml(h, a, ca, cb));
     PHC_TRY(fd_launch_step(h, a, damp));
     if (h->lb_on) PHC_HIP(lb_enqueue_apply(h->lbi, h->L, h->lev[ca].as<float>(), h->stream));
     return PHC_OK;
@@ -414,7 +584,8 @@ phc_status phc_fd_destroy(phc_fd *h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     fd_drop_graph(h);
     for (DevBuf *b : {&h->lev[0], &h->lev[1], &h->vp, &h->fx, &h->fy, &h->fz, &h->rhs, &h->trace,
-                      &h->rec, &h->recoff, &h->lb_w, &h->lb_corr})
+                      &h->rec, &h->recoff, &h->lb_w, &h->lb_corr, &h->pml_ex, &h->pml_ey, &h->pml_ez,
+                      &h->pml_phi})
         b->release();
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -434,9 +605,22 @@ phc_status phc_fd_add_rhs(phc_fd *h, int it, int cb)
     return PHC_OK;
 }
 
+phc_status phc_fd_compute_one_step_pml(phc_fd *h, int ca, int cb)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    if (!h->pml_on) return fail(PHC_ERR_STATE, "no PML set (phc_fd_set_pml)");
+    if (ca < 0 || ca > 1 || cb < 0 || cb > 1 || ca == cb)
+        return fail(PHC_ERR_INVALID, "levels ca=%d cb=%d must be 0 and 1", ca, cb);
+    FdStepArgs a;
+    fd_fill_args(h, a, ca, cb);
+    PHC_TRY(fd_launch_full_step(h, a, false, ca, cb));
+    return PHC_OK;
+}
+
 phc_status phc_fd_compute_one_step_sb(phc_fd *h, int ca, int cb)
 {
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    if (h->pml_on) return fail(PHC_ERR_STATE, "a PML is set: use phc_fd_compute_one_step_pml");
     if (ca < 0 || ca > 1 || cb < 0 || cb > 1 || ca == cb)
         return fail(PHC_ERR_INVALID, "levels ca=%d cb=%d must be 0 and 1", ca, cb);
     PHC_TRY(fd_materialize(h, cb));
@@ -689,6 +873,50 @@ phc_status phc_fd_set_timing(phc_fd *h, int enable)
     return PHC_OK;
 }
 
+phc_status phc_fd_set_pml(phc_fd *h, const int *ndamp, const float *etax, const float *etay, const float *etaz,
+                          const float *hd2)
+{
+    if (!h || !ndamp) return fail(PHC_ERR_INVALID, "handle or ndamp is NULL");
+    const int n[3] = {h->nx, h->ny, h->nz};
+    bool any = false;
+    for (int a = 0; a < 3; ++a) {
+        if (ndamp[a] < 0 || 2 * ndamp[a] > n[a])
+            return fail(PHC_ERR_INVALID, "PML width %d on axis %d does not fit %d points", ndamp[a], a, n[a]);
+        any = any || ndamp[a] > 0;
+    }
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    fd_drop_graph(h);
+    if (!any) {
+        h->pml_on = 0;
+        h->pml_nd[0] = h->pml_nd[1] = h->pml_nd[2] = 0;
+        for (DevBuf *b : {&h->pml_ex, &h->pml_ey, &h->pml_ez, &h->pml_phi}) b->release();
+        return PHC_OK;
+    }
+    if (!etax || !etay || !etaz || !hd2) return fail(PHC_ERR_INVALID, "eta profile or hd2 is NULL");
+    {
+        const int64_t rows = (int64_t)2 * ndamp[0] * n[1] + (int64_t)(n[0] - 2 * ndamp[0]) * 2 * ndamp[1] +
+                             (int64_t)(n[0] - 2 * ndamp[0]) * 2 * ndamp[2];
+        if (rows > 65535) return fail(PHC_ERR_INVALID, "PML slabs of %lld rows exceed 65535", (long long)rows);
+    }
+    if (h->sw != 0) return fail(PHC_ERR_INVALID, "the PML replaces the sponge: sponge_width must be 0");
+    if (h->lb_on) return fail(PHC_ERR_STATE, "a learned boundary is attached");
+    const float *prof[3] = {etax, etay, etaz};
+    for (int a = 0; a < 3; ++a)
+        for (int i = 0; i < n[a] + 2; ++i)
+            if (!std::isfinite(prof[a][i])) return fail(PHC_ERR_INVALID, "eta profile %d is not finite", a);
+    DevBuf *dst[3] = {&h->pml_ex, &h->pml_ey, &h->pml_ez};
+    for (int a = 0; a < 3; ++a) {
+        PHC_TRY(dst[a]->alloc(sizeof(float) * (n[a] + 2)));
+        PHC_HIP(hipMemcpy(dst[a]->ptr, prof[a], sizeof(float) * (n[a] + 2), hipMemcpyHostToDevice));
+        h->pml_nd[a] = ndamp[a];
+        h->pml_hd2[a] = hd2[a];
+    }
+    PHC_TRY(h->pml_phi.alloc(sizeof(float) * (size_t)h->L.total));
+    PHC_HIP(hipMemset(h->pml_phi.ptr, 0, sizeof(float) * (size_t)h->L.total));
+    h->pml_on = 1;
+    return PHC_OK;
+}
+
 phc_status phc_fd_set_learned_boundary(phc_fd *h, const phc_lb *lb, float halo, float layer_damping)
 {
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
@@ -703,6 +931,7 @@ phc_status phc_fd_set_learned_boundary(phc_fd *h, const phc_lb *lb, float halo, 
     FdFieldView fv;
     PHC_TRY(fd_field_view(h, &fv));
     if (h->sw != 0) return fail(PHC_ERR_INVALID, "the learned boundary replaces the sponge: sponge_width must be 0");
+    if (h->pml_on) return fail(PHC_ERR_STATE, "a PML is set");
     const int n[3] = {h->nx, h->ny, h->nz}, src[3] = {h->xs, h->ys, h->zs};
     for (int a = 0; a < 3; ++a) {
         if (n[a] < 4 * LB_W || n[a] > 4000)

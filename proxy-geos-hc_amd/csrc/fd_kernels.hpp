@@ -60,6 +60,10 @@ struct FdStepArgs {
     float rhs_next;
     float *trace;       // if set: *trace = new value at src before injection
     int tiles_k, tiles_j, chunk_i;
+    // box of points this launch updates: i in [ilo, ihi) (marched in chunks
+    // from ilo), j in [jlo, jhi), k in [klo, khi); the whole grid unless a
+    // PML takes the slabs around it
+    int ilo, ihi, jlo, jhi, klo, khi;
 };
 
 } // namespace phc

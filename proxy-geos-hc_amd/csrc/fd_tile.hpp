@@ -68,8 +68,8 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
     const FdLayout &L = g.L;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int k0 = kt * FD_TK, j0 = jt * TJ;
-    const int i0 = ic * a.chunk_i;
-    const int i1 = min(i0 + a.chunk_i, L.nx);
+    const int i0 = g.ilo + ic * a.chunk_i;
+    const int i1 = min(i0 + a.chunk_i, g.ihi);
     const int k = k0 + tx;
     const IT pitch = (IT)L.pitch_j * S;
 
@@ -77,7 +77,7 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
     // the interior origin instead (their offsets point there), and what they
     // stage in LDS is never read by a point that is computed.
     const bool kload = k < L.nz + R;
-    const bool kcomp = k < L.nz;
+    const bool kcomp = k >= g.klo && k < g.khi;
     int jr[NJ];
     bool jcomp[NJ], inJ[NJ];
     float fyv[NJ];
@@ -86,7 +86,7 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
     for (int jj = 0; jj < NJ; ++jj) {
         jr[jj] = j0 + ty * NJ + jj;
         const bool jl = kload && (jr[jj] < L.ny + R);
-        jcomp[jj] = kcomp && (jr[jj] < L.ny);
+        jcomp[jj] = kcomp && (jr[jj] >= g.jlo) && (jr[jj] < g.jhi);
         inJ[jj] = (jr[jj] >= a.sw) && (jr[jj] < L.ny - a.sw);
         fyv[jj] = (a.sw > 0 && jr[jj] < L.ny) ? g.fy[jr[jj]] : 1.f;
         colq[jj] = (IT)fd_off(L, 0, jl ? jr[jj] : 0, jl ? k : 0) * S;

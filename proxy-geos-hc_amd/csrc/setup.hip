@@ -209,4 +209,33 @@ phc_status phc_sem_model_gaussian(int ex, int ey, int ez, double sigma, float vm
     return PHC_OK;
 }
 
+// PML setup (FDTDinit.hpp:155-173, FDTDutils.hpp:84-107, 129-160), float as
+// the reference computes it.
+int phc_fd_pml_width(float vmin, float f0, float d)
+{
+    const float fmax = 2.5f * f0;
+    const float lambdamax = vmin / fmax;
+    return (int)(4 * lambdamax / d);  // ntaper = 4
+}
+
+float phc_fd_pml_scale(float dt, float vmax, int ndamp, float d)
+{
+    return dt * 3.f * vmax * std::log(1000.f) / (2.f * ndamp * d);
+}
+
+phc_status phc_fd_pml_profile(int n, int ndamp, float scale, float *profile)
+{
+    if (!profile) return phc::fail(PHC_ERR_INVALID, "profile is NULL");
+    if (n < 1 || ndamp < 0 || 2 * ndamp > n + 2)
+        return phc::fail(PHC_ERR_INVALID, "bad PML width %d for %d points", ndamp, n);
+    // pml_profile_init(profile, 0, n+1, ndamp, ndamp, scale): quadratic ramps,
+    // scale at the outermost index of each end
+    for (int i = 0; i <= n + 1; ++i) profile[i] = 0.f;
+    if (ndamp == 0) return PHC_OK;
+    const float tmp = scale / (float)(ndamp * ndamp);
+    for (int i = 1; i <= ndamp; ++i) profile[ndamp - i] = (float)(i * i) * tmp;
+    for (int i = 1; i <= ndamp; ++i) profile[n + 2 - ndamp + i - 1] = (float)(i * i) * tmp;
+    return PHC_OK;
+}
+
 } // extern "C"

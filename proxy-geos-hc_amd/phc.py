@@ -85,6 +85,11 @@ SIGNATURES = {
     "phc_fd_synchronize": (c_int, [c_void_p]),
     "phc_fd_autotune": (c_int, [c_void_p, c_int, ctypes.c_char_p, ctypes.POINTER(c_int),
                                 ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
+    "phc_fd_set_pml": (c_int, [c_void_p, i32p, f32p, f32p, f32p, f32p]),
+    "phc_fd_compute_one_step_pml": (c_int, [c_void_p, c_int, c_int]),
+    "phc_fd_pml_width": (c_int, [c_float, c_float, c_float]),
+    "phc_fd_pml_scale": (c_float, [c_float, c_float, c_int, c_float]),
+    "phc_fd_pml_profile": (c_int, [c_int, c_int, c_float, f32p]),
 }
 
 
@@ -235,10 +240,34 @@ class FdSolver:
     def compute_one_step_sb(self, ca, cb):
         check(lib().phc_fd_compute_one_step_sb(self.h, ca, cb))
 
+    def compute_one_step_pml(self, ca, cb):
+        check(lib().phc_fd_compute_one_step_pml(self.h, ca, cb))
+
+    def set_pml(self, ndamp=None, f0=None, vmin=1500.0, vmax=4500.0, dx=10.0):
+        """FDTDInit -usePML: width (int)(4 vmin/(2.5 f0)/dx) unless given,
+        init_eta profiles from the setup's dt; ndamp=0 removes the PML."""
+        s = self.setup
+        f0 = s.f0 if f0 is None else f0
+        nd = lib().phc_fd_pml_width(vmin, f0, dx) if ndamp is None else ndamp
+        prof = []
+        for n in (self.nx, self.ny, self.nz):
+            p = np.zeros(n + 2, np.float32)
+            if nd > 0:
+                check(lib().phc_fd_pml_profile(n, nd, lib().phc_fd_pml_scale(s.dt, vmax, nd, dx), p))
+            prof.append(p)
+        hd2 = np.full(3, np.float32(1.0 / (4.0 * dx * dx)), np.float32)
+        check(lib().phc_fd_set_pml(self.h, np.array([nd] * 3, np.int32), prof[0], prof[1], prof[2], hd2))
+        self.pml = nd > 0
+        self.ndamp = nd
+        return nd
+
     def step(self, it):
         """One iteration of FDTDmain.cpp:55-81 through the per-call surface."""
         self.add_rhs(it, self.i2)
-        self.compute_one_step_sb(self.i1, self.i2)
+        if getattr(self, "pml", False):
+            self.compute_one_step_pml(self.i1, self.i2)
+        else:
+            self.compute_one_step_sb(self.i1, self.i2)
         self.i1, self.i2 = self.i2, self.i1
 
     def run(self, it0, nsteps):

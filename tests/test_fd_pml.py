@@ -1,0 +1,120 @@
+"""FD PML boundary (FDTDkernels.hpp:36-104, FDTDutils.hpp:84-171): the CPU
+restatement pinned by closed forms and invariants, and the GPU path (through
+the C ABI) bit-exact against it in reference rounding."""
+import numpy as np
+import pytest
+
+
+def test_profile_closed_form_and_library_helper(phc, oracle):
+    n, nd, scale = 40, 7, 0.3
+    o = np.zeros(n + 2, np.float32)
+    oracle.lib().orc_fd_pml_profile(n, nd, np.float32(scale), o)
+    g = np.zeros(n + 2, np.float32)
+    phc.check(phc.lib().phc_fd_pml_profile(n, nd, scale, g))
+    assert np.array_equal(o, g)
+    tmp = np.float32(np.float32(scale) / np.float32(nd * nd))
+    for m in range(n + 2):
+        if m < nd:
+            exp = np.float32((nd - m) ** 2) * tmp
+        elif m > n + 1 - nd:
+            exp = np.float32((m - (n + 1 - nd)) ** 2) * tmp
+        else:
+            exp = 0.0
+        assert o[m] == exp, m
+    assert o[0] == o[n + 1] == np.float32(scale) or abs(o[0] - scale) < 1e-7
+    # the reference's defaults: vmin 1500, f0 15, dx 10 -> 16 points
+    assert phc.lib().phc_fd_pml_width(1500.0, 15.0, 10.0) == 16
+    assert phc.lib().phc_fd_pml_scale(np.float32(8e-4), 4500.0, 16, 10.0) == \
+        oracle.lib().orc_fd_pml_scale(np.float32(8e-4), 4500.0, 16, 10.0)
+
+
+def test_oracle_pml_mirror_symmetry(oracle):
+    """Homogeneous model, source at the centre of an odd grid: the field is
+    mirror-symmetric bit for bit (a one-off eta index or ghost breaks it)."""
+    P = oracle.FdProblem(nx=37, ny=41, nz=39, L=4, f0=30.0, model="homogeneous", nsamples=90, pml=True,
+                         sponge_width=0)
+    assert list(P.ndamp) == [8, 8, 8]
+    P.run(80)
+    for c in (P.current(), P.previous()):
+        assert np.array_equal(c, c[::-1]) and np.array_equal(c, c[:, ::-1]) and np.array_equal(c, c[:, :, ::-1])
+    assert np.max(np.abs(P.phi)) > 0
+
+
+def test_oracle_pml_width_zero_is_the_plain_loop(oracle):
+    kw = dict(nx=24, ny=21, nz=27, L=4, f0=20.0, nsamples=40, sponge_width=0)
+    A = oracle.FdProblem(pml=True, ndamp=0, **kw)
+    B = oracle.FdProblem(**kw)
+    ta, _ = A.run(35)
+    tb, _ = B.run(35)
+    assert np.array_equal(ta, tb) and np.array_equal(A.current(), B.current())
+
+
+def test_oracle_pml_absorbs(oracle):
+    """A wave leaving through the PML leaves much less energy behind than
+    the bare grid (whose halo reflects)."""
+    kw = dict(nx=60, L=4, f0=30.0, model="homogeneous", nsamples=700, sponge_width=0)
+    A = oracle.FdProblem(pml=True, **kw)
+    B = oracle.FdProblem(**kw)
+    A.run(650)
+    B.run(650)
+    nd = int(A.ndamp[0])
+    sl = slice(4 + nd, 4 + 60 - nd)
+    ea = float(np.sum(A.current()[sl, sl, sl].astype(np.float64) ** 2))
+    eb = float(np.sum(B.current()[sl, sl, sl].astype(np.float64) ** 2))
+    assert ea < 0.05 * eb, (ea, eb)
+
+
+def pair(phc, oracle, shape, f0=30.0, xs=None, ys=None, zs=None, nsamples=60, ndamp=None, model="two_layer"):
+    nx, ny, nz = shape
+    s = phc.FdSetup(nx=nx, ny=ny, nz=nz, L=4, f0=f0, xs=xs, ys=ys, zs=zs, sponge_width=0, model=model,
+                    nsamples=nsamples)
+    P = oracle.FdProblem(nx=nx, ny=ny, nz=nz, L=4, xs=s.xs, ys=s.ys, zs=s.zs, f0=f0, model=s.vp.copy(),
+                         nsamples=nsamples, sponge_width=0, pml=True, ndamp=ndamp)
+    assert np.array_equal(P.rhs, s.rhs)
+    return s, P
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape,ndamp,src", [
+    ((40, 37, 70), None, None),          # default width (8 at f0 = 30)
+    ((33, 29, 45), 5, (2, 14, 22)),      # source inside an i-slab
+    ((26, 31, 24), 3, (13, 15, 22)),     # source inside a k-slab
+    ((48, 20, 130), 9, None),
+])
+def test_gpu_pml_bit_exact(gpu, oracle, shape, ndamp, src):
+    print("\n[pml] %s ndamp %s src %s" % (shape, ndamp, src), flush=True)
+    xs, ys, zs = src or (None, None, None)
+    s, P = pair(gpu, oracle, shape, xs=xs, ys=ys, zs=zs, ndamp=ndamp)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF)
+    nd = sol.set_pml(ndamp=int(P.ndamp[0]), f0=s.f0)
+    assert nd == int(P.ndamp[0])
+    for it in range(25):
+        sol.step(it)
+    tr, _ = P.run(25)
+    assert np.array_equal(sol.current(), P.current())
+    assert np.array_equal(sol.previous(), P.previous())
+    assert sol.read_point(sol.i2, s.xs, s.ys, s.zs) == tr[-1]
+    # fused, graph-captured run with receivers, continuing
+    rec = np.array([[s.xs, s.ys, s.zs], [0, 0, 0], [shape[0] - 1, 3, shape[2] // 2]], np.int32)
+    sol.set_receivers(rec)
+    src_t, rv = sol.run(25, 20)
+    otr, orv = P.run(20, receivers=rec)
+    assert np.array_equal(src_t, otr) and np.array_equal(rv, orv)
+    assert np.array_equal(sol.current(), P.current())
+
+
+@pytest.mark.gpu
+def test_gpu_pml_fp32_close_and_state_errors(gpu, oracle):
+    s, P = pair(gpu, oracle, (44, 41, 66))
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_FP32)
+    with pytest.raises(gpu.PhcError):
+        sol.compute_one_step_pml(0, 1)          # no PML set yet
+    sol.set_pml(f0=s.f0)
+    with pytest.raises(gpu.PhcError):
+        sol.compute_one_step_sb(0, 1)           # a PML is set
+    src, _ = sol.run(0, 40)
+    tr, _ = P.run(40)
+    rel = np.linalg.norm(src.astype(np.float64) - tr) / np.linalg.norm(tr.astype(np.float64))
+    assert rel < 1e-5
+    sol.set_pml(ndamp=0)                         # removing it restores the plain step
+    sol.compute_one_step_sb(0, 1)
