@@ -6,7 +6,8 @@
 //   FDTDKernel               src/fd/utilsFDTD/FDTDkernels.hpp:4-131
 //   FDTDUtils::output/write_io src/fd/utilsFDTD/FDTDutils.hpp:193-232
 //
-// Same class names, method names, argument meaning and argv flags.  Additional
+// Same class names, method names, argument meaning and argv flags (including
+// -usePML: the PML of FDTDKernel::computeOneStepPML).  Additional
 // flags (not in the reference): -f0 F, -cfl C, -timemax T, -sponge W,
 // -layers N SEED (seeded layered model), -homogeneous (vmin everywhere, the
 // model of the published test_20240512 runs), -fp32 (all-FP32 arithmetic),
@@ -92,10 +93,11 @@ struct FDTDKernel {
         phc_check(phc_fd_compute_one_step_sb(m.dev, ca, cb), "computeOneStepSB");
         return 0;
     }
-    // FDTDkernels.hpp:86-104 (PML variant): not part of this library
-    int computeOneStepPML(FDTDGRIDS &, FDTDMODELS &, int &, int &)
+    // FDTDkernels.hpp:86-104 (pml3D on the slabs, inner3D inside)
+    int computeOneStepPML(FDTDGRIDS &, FDTDMODELS &m, int &ca, int &cb)
     {
-        throw std::runtime_error("computeOneStepPML: the PML boundary is not implemented (use the sponge)");
+        phc_check(phc_fd_compute_one_step_pml(m.dev, ca, cb), "computeOneStepPML");
+        return 0;
     }
 };
 
@@ -174,11 +176,16 @@ struct FDTDInit {
         g.hdy_2 = 1. / (4. * g.dy * g.dy);
         g.hdz_2 = 1. / (4. * g.dz * g.dz);
         printf("usePML=%d\n", usePML);
-        if (usePML) throw std::runtime_error("-usePML: the PML boundary is not implemented (use the sponge)");
-        g.ndampx = g.ndampy = g.ndampz = 0;
-        g.x1 = 0; g.x2 = 0; g.x3 = 0; g.x4 = g.nx; g.x5 = g.nx; g.x6 = g.nx;
-        g.y1 = 0; g.y2 = 0; g.y3 = 0; g.y4 = g.ny; g.y5 = g.ny; g.y6 = g.ny;
-        g.z1 = 0; g.z2 = 0; g.z3 = 0; g.z4 = g.nz; g.z5 = g.nz; g.z6 = g.nz;
+        if (usePML) {  // FDTDinit.hpp:167-172: ntaper * lambdamax / d, lambdamax = vmin / fmax
+            g.ndampx = phc_fd_pml_width(vmin, f0, g.dx);
+            g.ndampy = phc_fd_pml_width(vmin, f0, g.dy);
+            g.ndampz = phc_fd_pml_width(vmin, f0, g.dz);
+        } else {
+            g.ndampx = g.ndampy = g.ndampz = 0;
+        }
+        g.x1 = 0; g.x2 = g.ndampx; g.x3 = g.ndampx; g.x4 = g.nx - g.ndampx; g.x5 = g.nx - g.ndampx; g.x6 = g.nx;
+        g.y1 = 0; g.y2 = g.ndampy; g.y3 = g.ndampy; g.y4 = g.ny - g.ndampy; g.y5 = g.ny - g.ndampy; g.y6 = g.ny;
+        g.z1 = 0; g.z2 = g.ndampz; g.z3 = g.ndampz; g.z4 = g.nz - g.ndampz; g.z5 = g.nz - g.ndampz; g.z6 = g.nz;
     }
 
     // FDTDinit.hpp:202-233
@@ -233,12 +240,27 @@ struct FDTDInit {
         p.vp = m.vp.data();
         p.rhs = m.RHSTerm.data();
         p.nsamples = nSamples;
-        p.sponge_width = spongeWidth;
+        p.sponge_width = usePML ? 0 : spongeWidth;  // computeOneStepPML applies no sponge
         p.sponge_alpha = spongeAlpha;
         p.init_value = 0.000001f;
         p.arith = arith;
         if (m.dev) phc_fd_destroy(m.dev);
         m.dev = nullptr;
         phc_check(phc_fd_create(&p, &m.dev), "phc_fd_create");
+        if (usePML) {
+            // FDTDutils.hpp:129-171 init_eta: per-axis quadratic profiles
+            const int nd[3] = {g.ndampx, g.ndampy, g.ndampz};
+            const int n[3] = {g.nx, g.ny, g.nz};
+            const float d[3] = {g.dx, g.dy, g.dz};
+            std::vector<float> prof[3];
+            for (int a = 0; a < 3; ++a) {
+                prof[a].assign(n[a] + 2, 0.f);
+                if (nd[a] > 0)
+                    phc_check(phc_fd_pml_profile(n[a], nd[a], phc_fd_pml_scale(timeStep, vmax, nd[a], d[a]),
+                                                 prof[a].data()), "pml profile");
+            }
+            const float hd2[3] = {g.hdx_2, g.hdy_2, g.hdz_2};
+            phc_check(phc_fd_set_pml(m.dev, nd, prof[0].data(), prof[1].data(), prof[2].data(), hd2), "set_pml");
+        }
     }
 };

@@ -13,7 +13,8 @@
 // runtime options with the reference's values as defaults:
 //   -order P, -dt DT, -f0 F, -timemax T, -nsteps N, -element (element kernel),
 //   -gaussian SEED (smooth random model 2000-4000 m/s), -receivers (8 on the
-//   line y = ly/2, z = 100 m), -fused (graph-captured runs).
+//   line y = ly/2, z = 100 m), -fused (graph-captured runs), -snapshots (the
+//   reference's SEM_SAVE_SNAPSHOTS build option: ASCII snapshot every 100 steps).
 #pragma once
 
 #include <chrono>
@@ -21,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -98,6 +100,26 @@ class SEMmesh {
     }
     // node id of the grid point (ix, iy, iz) (SEMmesh.cpp:214-223 numbering)
     int node(int ix, int iy, int iz) const { return ix + iz * nx + iy * nx * nz; }
+    // SEMmesh.cpp:69-79: the element sizes, returned as int
+    int getDx() const { return (int)hx; }
+    int getDy() const { return (int)hy; }
+    int getDz() const { return (int)hz; }
+    // SEMmesh.cpp:624-654: the node plane iy = ny/2 - 1 of level i1 as ASCII
+    // "x z value" lines (x = ix * element size, z = iz * element size, as the
+    // reference writes them), file "snapshot<step>"
+    template <class Field> void saveSnapShot(const int indexTimeStep, const int i1, const Field &u) const
+    {
+        const float dx = getDx(), dz = getDz();
+        const int numberOfNodes = nx * nz;
+        const int offset = (ny == 1) ? 0 : nx * nz * (ny / 2 - 1);
+        std::ofstream snapFile("snapshot" + std::to_string(indexTimeStep), std::ios::out | std::ios::trunc);
+        if (!snapFile) throw std::runtime_error("cannot open snapshot file");
+        for (int i = 0; i < nx; i++)
+            for (int j = 0; j < nz; j++) {
+                (void)numberOfNodes;
+                snapFile << i * dx << " " << j * dz << " " << u(offset + i + j * nx, i1) << std::endl;
+            }
+    }
 };
 
 class SEMsolver {
@@ -149,8 +171,10 @@ class SEMsolver {
                              (e / (mesh.ex * mesh.ez)) * mesh.order * mesh.nx * mesh.nz;
             printf("TimeStep=%d;  pnGlobal @ elementSource location %d after computeOneStep = %g\n",
                    indexTimeStep, myElementSource, pnGlobal(node, i1));
+            if (saveSnapShots) mesh.saveSnapShot(indexTimeStep, i1, pnGlobal);
         }
     }
+    bool saveSnapShots = false;  // the reference's SEM_SAVE_SNAPSHOTS build option
 };
 
 class SEMproxy {
@@ -188,6 +212,7 @@ class SEMproxy {
             else if (a == "-gaussian") { gaussian = true; seed = std::stoull(val()); }
             else if (a == "-receivers") receivers = true;
             else if (a == "-fused") fused = true;
+            else if (a == "-snapshots") mySolver.saveSnapShots = true;
             else throw std::runtime_error("unknown argument " + a);
         }
         myMesh = SEMmesh(ex, ex, ex, lx, lx, lx, myInfo.myOrderNumber);

@@ -35,7 +35,10 @@ int main(int argc, char *argv[])
                 auto t0 = system_clock::now();
                 kernel.addRHS(g, it, m, init.i2);
                 auto t1 = system_clock::now();
-                kernel.computeOneStepSB(g, m, init.i1, init.i2);
+                if (init.usePML)
+                    kernel.computeOneStepPML(g, m, init.i1, init.i2);
+                else
+                    kernel.computeOneStepSB(g, m, init.i1, init.i2);
                 auto t2 = system_clock::now();
                 std::swap(init.i1, init.i2);
                 utils.output(g, m, it, init.i2, init.saveSnapShots);

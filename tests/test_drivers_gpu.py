@@ -56,6 +56,25 @@ def test_fd_driver_snapshots(gpu, oracle, tmp_path):
     assert np.array_equal(snap, ref)
 
 
+@pytest.mark.parametrize("fused", [False, True])
+def test_fd_driver_pml(gpu, oracle, tmp_path, fused):
+    """fd_hip -usePML (FDTDmain.cpp:64-71 computeOneStepPML): the prints and
+    the last snapshot equal the PML oracle's values bit for bit."""
+    print("\n[drivers] fd_hip -usePML fused=%d" % fused, flush=True)
+    args = ["-nx", "48", "-xs", "21", "-f0", "30", "-usePML", "-nsteps", "151", "-savesnapshots"]
+    out = run("fd_hip", args + (["-fused"] if fused else []), str(tmp_path))
+    assert "usePML=1" in out
+    got = dict((int(m.group(1)), float(m.group(2))) for m in
+               re.finditer(r"TimeStep=(\d+)\t; Pressure value at source \[21 21 21\] = (\S+)", out))
+    P = oracle.FdProblem(nx=48, xs=21, ys=21, zs=21, f0=30.0, sponge_width=0, pml=True)
+    tr, _ = P.run(151)
+    for step in (0, 50, 100, 150):
+        assert got[step] == float("%f" % tr[step]), step
+    snap = np.fromfile(str(tmp_path / "snapshot_it_150.H@"), np.float32)
+    ref = P.current()[4:-4, 4:-4, 4:-4][:, 24, :].T.reshape(-1)
+    assert np.array_equal(snap, ref)
+
+
 def test_sem_driver_prints(gpu, oracle, tmp_path):
     """sem_hip 8 320 -order 2 ... prints the source-node value every 100 steps;
     compared with the FP32 reference-semantics oracle (%g, 6 digits)."""
@@ -71,3 +90,28 @@ def test_sem_driver_prints(gpu, oracle, tmp_path):
         assert abs(got[step] - tr[step]) <= 1e-4 * np.abs(tr).max() + 1e-5 * abs(tr[step]), (step, got[step], tr[step])
     lines = open(str(tmp_path / "receivers.txt")).read().split("\n")
     assert len([l for l in lines if l.strip()]) == 301
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_sem_driver_snapshots(gpu, oracle, tmp_path, fused):
+    """sem_hip -snapshots writes the reference's ASCII snapshot (SEMmesh.cpp:
+    624-654): plane iy = ny/2 - 1, lines "ix*hx iz*hz value" with hx the int
+    element size, x-major, stream formatting."""
+    print("\n[drivers] sem_hip -snapshots fused=%d" % fused, flush=True)
+    args = ["6", "1000", "-order", "3", "-dt", "0.0008", "-f0", "25", "-nsteps", "201", "-snapshots"]
+    run("sem_hip", args + (["-fused"] if fused else []), str(tmp_path))
+    O = oracle.SemProblem(ex=6, lx=1000.0, order=3, dt=0.0008, f0=25.0, nsamples=201,
+                          src_point=(1001.0, 1001.0, 1001.0))
+    O.run(201)
+    nx = ny = nz = 6 * 3 + 1
+    field = O.current()           # the newest level (level i1 when the reference prints)
+    lines = [l.split() for l in open(str(tmp_path / "snapshot200")).read().split("\n") if l.strip()]
+    assert len(lines) == nx * nz
+    hx = int(1000.0 / 6)          # 166: SEMmesh::getDx returns the element size as an int
+    off = nx * nz * (ny // 2 - 1)
+    scale = np.abs(field).max()
+    for n, (x, z, v) in enumerate(lines):
+        i, j = n // nz, n % nz
+        assert x == "%g" % (i * hx) and z == "%g" % (j * hx)
+        assert abs(float(v) - field[off + i + j * nx]) <= 1e-4 * scale + 1e-5 * abs(field[off + i + j * nx])
+    assert os.path.exists(str(tmp_path / "snapshot0")) and os.path.exists(str(tmp_path / "snapshot100"))
