@@ -10,9 +10,12 @@ on the solver's stream; value = interior points x K / wall time.  The
 roofline block uses the mean stencil-kernel duration from those events and
 16 algorithmic bytes per point update (read pn, pnm1, vp^2dt^2; write pnp1).
 
-Beside the headline line, "batch" reports the batched multi-shot workload
+Beside the headline line: "batch" reports the batched multi-shot workload
 (64 uneven shots, 800 steps, one persistent launch per step) and the same shots
-run one at a time.
+run one at a time; "sem_p4" the SEM order-4 workload (401^3 nodes, FP32
+separable kernel, and its FP32-vs-FP64 precision over 2000 steps);
+"learned_boundary" the learned thin layer trained from scratch on the GPU and
+compared with the 40-point sponge against a 2x-wide FP64 domain.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
 """
@@ -36,6 +39,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fd_step_r4_512_traffic.txt")
 
 
 BATCH_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fd_batch_64shots_traffic.txt")
+SEM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sem_p4_401_traffic.txt")
 
 
 def read_traffic(path=TRAFFIC_FILE):
@@ -45,6 +49,10 @@ def read_traffic(path=TRAFFIC_FILE):
         if line.startswith("bytes_per_launch:"):
             return int(float(line.split(":")[1]))
     return None
+
+
+def progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 def cpu_baseline(setup, nsteps=3):
@@ -127,6 +135,41 @@ def bench_batch(phc, nsteps=800, nshots=64, baselines=True, shape=None):
     return out
 
 
+def bench_sem_p4(phc, steps=2000, fp64=True):
+    """SEM order 4, 100^3 elements of 20 m (401^3 nodes), Gaussian random vp
+    2000-4000 m/s, Ricker 8 Hz, dt 2e-4 s, 2000 steps: the FP32 separable
+    kernel (autotuned; per-launch HIP events) and the same run in FP64."""
+    import precision
+    r = precision.sem_p4_study(steps=steps, fp64=fp64)
+    n = r["nodes"]
+    ms = r["fp32_kernel_ms_per_step"]
+    ach = BYTES_PER_UPDATE * n / (ms * 1e-3) / 1e9
+    r["value"] = round(n / (ms * 1e-3) / 1e6, 1)
+    r["unit"] = "Mnode-updates/s"
+    r["roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": read_traffic(SEM_TRAFFIC_FILE)}
+    return r
+
+
+def bench_learned(phc, n=256, steps=2000):
+    """Learned thin boundary: trained from scratch on the GPU (200 wide-sponge
+    shots, Adam), then one shot on an n^3 region: the 4-point learned layer,
+    the 40-point sponge and the bare grid against a 2x-wide FP64 domain."""
+    import learned
+    t0 = time.perf_counter()
+    lb = phc.LearnedBoundary(seed=11)
+    learned.training_data(lb, n_shots=200, region=64)
+    hist = learned.train(lb, steps=6000)
+    train_s = time.perf_counter() - t0
+    r = learned.compare(lb, n=n, steps=steps)
+    r.pop("truth_traces")
+    r["train_wall_s"] = round(train_s, 2)
+    r["train_final_loss"] = float(hist[-1])
+    r["samples_per_face"] = [int(c) for c in lb.counts()]
+    lb.close()
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,13 +180,14 @@ def main():
     ap.add_argument("--arith", default="fp32", choices=["fp32", "ref"])
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched multi-shot measurement reported beside the headline")
-    ap.add_argument("--workload", default="fd", choices=["fd", "batch"],
+    ap.add_argument("--workload", default="fd", choices=["fd", "batch", "sem"],
                     help="fd: the 512^3 headline; batch: only the batched multi-shot workload "
-                         "(--steps steps, default 800)")
+                         "(--steps steps, default 800); sem: only the SEM order-4 workload")
     ap.add_argument("--batch-shape", default=None,
                     help="tile_j,block_waves,chunk_i,blocks_per_cu for the batch (default: autotuned)")
     ap.add_argument("--no-baselines", action="store_true",
                     help="batch workload: skip the one-at-a-time runs")
+    ap.add_argument("--no-fp64", action="store_true", help="sem workload: skip the FP64 run")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -155,6 +199,16 @@ def main():
     if phc.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "sem":
+        r = bench_sem_p4(phc, steps=args.steps if "--steps" in sys.argv else 2000, fp64=not args.no_fp64)
+        out = {"metric": METRIC, "value": r["value"], "unit": r["unit"], "n_gpus": 1, "steps": r["steps"],
+               "warmup": 0, "ms_per_step": r["fp32_kernel_ms_per_step"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+               "data": "synthetic: seeded Gaussian random vp per element, Ricker source",
+               "config": {"workload": "SEM order 4, 100^3 elements of 20 m (401^3 nodes), 2000 steps"},
+               "roofline": r["roofline"], "sem_p4": r}
+        print(json.dumps(out), flush=True)
+        return
     if args.workload == "batch":
         shape = [int(v) for v in args.batch_shape.split(",")] if args.batch_shape else None
         nst = args.steps if "--steps" in sys.argv else 800
@@ -247,8 +301,14 @@ def main():
         "finite": fin == 0,
     }
     sol.close()
+    progress("FD 512^3 done: %.1f Mpt/s, kernel %.4f ms" % (value, kernel_ms))
     if not args.no_extras:
         out["batch"] = bench_batch(phc)
+        progress("batched shots done")
+        out["sem_p4"] = bench_sem_p4(phc)
+        progress("SEM order 4 done")
+        out["learned_boundary"] = bench_learned(phc)
+        progress("learned boundary done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(setup)
     print(json.dumps(out), flush=True)

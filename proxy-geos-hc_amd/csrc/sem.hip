@@ -170,9 +170,13 @@ sem_sep_kernel(SemSepArgs a)
     stage(0, R, 0);
     lds_barrier();
 
-    for (int ib = i0; ib < i1; ib += Q) {
+    // Unroll U planes per iteration.  U is a multiple of the ring length Q; at
+    // order 4 it is lcm(Q, P) = 20 and chunks start at multiples of P, so the
+    // coefficient row i % P of the march axis is a compile-time constant.
+    constexpr int U = (P == 4) ? 20 : Q;
+    for (int ib = i0; ib < i1; ib += U) {
 #pragma unroll
-        for (int u = 0; u < Q; ++u) {
+        for (int u = 0; u < U; ++u) {
             const int i = ib + u;
             if (i >= i1) break;
             {
@@ -185,7 +189,7 @@ sem_sep_kernel(SemSepArgs a)
             }
             const float(*tl)[LW] = tile[u & 1];
             const bool iin = i >= 1 && i < L.nx - 1;
-            const int my = i % P;
+            const int my = (P == 4) ? (u % P) : (i % P);
             const IT pi = (IT)i * pitch;
 #pragma unroll
             for (int jj = 0; jj < NJ; ++jj) {
@@ -472,6 +476,8 @@ static phc_status sem_launch(phc_sem *h, int i1, int i2, int inj, float rhs_next
         a.tiles_k = (h->L.nz + FD_TK - 1) / FD_TK;
         a.tiles_j = (h->L.ny + TJ - 1) / TJ;
         a.chunk_i = h->chunk_i;
+        // order 4: chunks start at multiples of 4 (the kernel's coefficient rows assume it)
+        if (h->p == 4 && a.chunk_i < h->L.nx && a.chunk_i % 4) a.chunk_i += 4 - a.chunk_i % 4;
         const int nb = a.tiles_k * a.tiles_j * ((h->L.nx + a.chunk_i - 1) / a.chunk_i);
         SemSepFn fn = sem_pick_sep(h->p, h->ty, h->nj, h->L.total * 4 < ((int64_t)1 << 32));
         if (!fn) return fail(PHC_ERR_INVALID, "no separable kernel for p=%d", h->p);

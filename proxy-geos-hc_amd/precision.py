@@ -25,7 +25,7 @@ def rel_l2(a, b):
 
 
 def sem_p4_study(ex=100, order=4, h=20.0, steps=2000, dt=2e-4, f0=8.0, seed=20240512, kernel=phc.SEM_SEPARABLE,
-                 log=None):
+                 log=None, fp64=True):
     lx = h * ex
     model = phc.sem_model_gaussian(ex, ex, ex, sigma=5.0, vmin=2000.0, vmax=4000.0, seed=seed)
     centre = (lx / 2 + 0.5 * h / order, lx / 2 + 0.5 * h / order, lx / 2 + 0.5 * h / order)
@@ -49,6 +49,8 @@ def sem_p4_study(ex=100, order=4, h=20.0, steps=2000, dt=2e-4, f0=8.0, seed=2024
     out["fp32_wall_s"] = round(time.perf_counter() - t0, 2)
     if log:
         log("  FP32 done (%.1fs, %.4f ms/step)" % (out["fp32_wall_s"], out["fp32_kernel_ms_per_step"]))
+    if not fp64:
+        return out
     t0 = time.perf_counter()
     s64 = phc.Sem64Solver(weights_ref=False, **kw)
     s64.set_receivers(rec)
