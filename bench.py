@@ -20,6 +20,7 @@ compared with the 40-point sponge against a 2x-wide FP64 domain.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -40,6 +41,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fd_step_r4_512_traffic.txt")
 
 BATCH_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fd_batch_64shots_traffic.txt")
 SEM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sem_p4_401_traffic.txt")
+SEM2D_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sem2d_p4_8001_traffic.txt")
 
 
 def read_traffic(path=TRAFFIC_FILE):
@@ -151,6 +153,58 @@ def bench_sem_p4(phc, steps=2000, fp64=True):
     return r
 
 
+# the reference's published SEM runs (2D, 65 x 65 elements of order 3, about
+# 999 steps): best published rates in Mnode-updates/s
+PUBLISHED_SEM2D = {"H100 (Kokkos)": 237.0, "MI100 (Kokkos)": 39.1, "Grace 72 threads (OpenMP)": 228.0}
+
+
+def bench_sem2d(phc, steps=1000, big_ex=2000, big_order=4, big_steps=200):
+    """SEM2D (the reference's SEM2D build): its published configuration, 65 x 65
+    elements of order 3 (38,416 nodes), c = 1500 m/s, dt = 1e-3 s, Ricker 10 Hz,
+    1000 steps, graph-captured and autotuned; and a large 2D mesh
+    (big_ex^2 elements of order big_order) for the HBM roofline."""
+    out = {}
+    G = phc.Sem2dSolver(ex=65, order=3, lx=2000.0, dt=1e-3, f0=10.0, nsamples=steps + 1)
+    chunk = G.autotune(steps_per_trial=20)
+    G.run(steps)  # capture + warm
+    G.synchronize()
+    G.it = 0
+    t0 = time.perf_counter()
+    G.run(steps)  # replay of the captured run (samples 0..steps-1 again)
+    G.synchronize()
+    wall = time.perf_counter() - t0
+    G.set_timing(True)
+    G.it = 0
+    G.run(steps)
+    kms = G.last_kernel_ms()
+    G.close()
+    nodes = 196 * 196
+    v = nodes * steps / wall / 1e6
+    out["published_config"] = {
+        "workload": "SEM2D 65 x 65 elements of order 3 (38,416 nodes), %d steps" % steps,
+        "wall_s": round(wall, 5), "value": round(v, 1), "unit": "Mnode-updates/s",
+        "us_per_step": round(wall / steps * 1e6, 2), "kernel_us": round(kms * 1e3, 2), "chunk": chunk,
+        "published_best": PUBLISHED_SEM2D,
+        "vs_published_best": round(v / max(PUBLISHED_SEM2D.values()), 1)}
+    n = big_ex * big_order + 1
+    B = phc.Sem2dSolver(ex=big_ex, order=big_order, lx=20.0 * big_ex, dt=2e-4, f0=8.0, nsamples=big_steps + 1)
+    chunk = B.autotune(steps_per_trial=5)
+    B.set_timing(True)
+    B.run(big_steps)
+    kms = B.last_kernel_ms()
+    fin = B.check_finite(B.i2) == 0
+    B.close()
+    ach = BYTES_PER_UPDATE * n * n / (kms * 1e-3) / 1e9
+    out["large"] = {
+        "workload": "SEM2D %d x %d elements of order %d (%d^2 nodes), %d steps" % (big_ex, big_ex, big_order, n,
+                                                                                 big_steps),
+        "kernel_ms_per_step": round(kms, 5), "value": round(n * n / (kms * 1e-3) / 1e6, 1),
+        "unit": "Mnode-updates/s", "chunk": chunk, "finite": fin,
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": read_traffic(SEM2D_TRAFFIC_FILE)}}
+    return out
+
+
 def bench_learned(phc, n=256, steps=2000):
     """Learned thin boundary: trained from scratch on the GPU (200 wide-sponge
     shots, Adam), then one shot on an n^3 region: the 4-point learned layer,
@@ -180,7 +234,7 @@ def main():
     ap.add_argument("--arith", default="fp32", choices=["fp32", "ref"])
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched multi-shot measurement reported beside the headline")
-    ap.add_argument("--workload", default="fd", choices=["fd", "batch", "sem"],
+    ap.add_argument("--workload", default="fd", choices=["fd", "batch", "sem", "sem2d"],
                     help="fd: the 512^3 headline; batch: only the batched multi-shot workload "
                          "(--steps steps, default 800); sem: only the SEM order-4 workload")
     ap.add_argument("--batch-shape", default=None,
@@ -199,8 +253,21 @@ def main():
     if phc.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "sem2d":
+        r = bench_sem2d(phc, big_steps=args.steps if "--steps" in sys.argv else 200)
+        big = r["large"]
+        out = {"metric": METRIC, "value": big["value"], "unit": big["unit"], "n_gpus": 1,
+               "steps": args.steps if "--steps" in sys.argv else 200, "warmup": 0,
+               "ms_per_step": big["kernel_ms_per_step"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32", "data": "synthetic: constant vp, Ricker source",
+               "config": {"workload": big["workload"]}, "roofline": big["roofline"], "sem2d": r}
+        print(json.dumps(out), flush=True)
+        return
     if args.workload == "sem":
         r = bench_sem_p4(phc, steps=args.steps if "--steps" in sys.argv else 2000, fp64=not args.no_fp64)
+        cms = ctypes.c_double()  # plain 1 GiB copy: calibrates the rocprof byte counters
+        phc.check(phc.lib().phc_copy_bandwidth(1 << 30, 5, ctypes.byref(cms)))
+        r["copy_gbs"] = round(2 * (1 << 30) / (cms.value * 1e-3) / 1e9, 1)
         out = {"metric": METRIC, "value": r["value"], "unit": r["unit"], "n_gpus": 1, "steps": r["steps"],
                "warmup": 0, "ms_per_step": r["fp32_kernel_ms_per_step"], "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -268,7 +335,6 @@ def main():
     traffic = read_traffic()
     # a plain 1 GiB copy: the counter calibration kernel and the practical
     # streaming bandwidth of this device
-    import ctypes
     cms = ctypes.c_double()
     phc.check(phc.lib().phc_copy_bandwidth(1 << 30, 5, ctypes.byref(cms)))
     copy_gbs = 2 * (1 << 30) / (cms.value * 1e-3) / 1e9
@@ -307,6 +373,8 @@ def main():
         progress("batched shots done")
         out["sem_p4"] = bench_sem_p4(phc)
         progress("SEM order 4 done")
+        out["sem2d"] = bench_sem2d(phc)
+        progress("SEM2D done")
         out["learned_boundary"] = bench_learned(phc)
         progress("learned boundary done")
     if not args.no_cpu:

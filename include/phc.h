@@ -180,6 +180,12 @@ typedef struct phc_sem phc_sem;
 /* Step kernel (SEMsolver::computeOneStep, SEMsolver.cpp:21-125). */
 #define PHC_SEM_SEPARABLE 0 /* assembled operator as a radius-p stencil (axis-aligned box) */
 #define PHC_SEM_ELEMENT 1   /* element-by-element sum factorisation + atomic scatter */
+/* element kernel with the scatter in 8 colour passes (elements of equal
+ * (x, z, y) index parities share no node): plain adds in a fixed order, so
+ * results repeat bit for bit.  The 3D form of SEM_MESHCOLOR
+ * (SEMmesh.cpp:380-429, commonMacros.hpp:58-63), whose 4 colours cover
+ * only the first element layer in 3D. */
+#define PHC_SEM_ELEMENT_COLOR 2
 
 typedef struct {
     int ex, ey, ez;        /* elements per axis (SEMproxy.cpp:12-14) */
@@ -245,6 +251,43 @@ phc_status phc_sem64_run(phc_sem64 *h, int it0, int nsteps);
 phc_status phc_sem64_get_traces(phc_sem64 *h, double *src, double *rec);
 /* current != 0: the field after the last step (level i2), else level i1; node order. */
 phc_status phc_sem64_get_field(phc_sem64 *h, int current, double *host);
+
+/* ================================================================= SEM2D
+ * SEMsolver::computeOneStep of the reference built with SEM2D
+ * (src/sem/solver/SEMsolver.cpp:21-125 including the SEM2D block 91-123):
+ * ex x ez quadrilaterals of order p on the x-z plane (SEMmesh with ey = 0,
+ * SEMmesh.cpp:3-26), node = ix + iz*nx, element e = i + k*ex; leapfrog on the
+ * interior nodes and the first-order paraxial absorbing condition on the four
+ * edges.  Parameters are phc_sem_params with ey = 0 (ly ignored, kernel
+ * PHC_SEM_SEPARABLE, chunk_i = rows per wavefront, 0 = default).
+ */
+typedef struct phc_sem2d phc_sem2d;
+phc_status phc_sem2d_create(const phc_sem_params *p, phc_sem2d **out);
+phc_status phc_sem2d_destroy(phc_sem2d *h);
+phc_status phc_sem2d_sizes(phc_sem2d *h, int *num_nodes, int *nx, int *nz);
+/* inject sample time_sample into level i2 at the first node of the source
+ * element (SEMsolver.cpp:39-43), then level i1 <- the new level */
+phc_status phc_sem2d_compute_one_step(phc_sem2d *h, int time_sample, int i1, int i2);
+/* SEMproxy::run (SEMproxy.cpp:41-64) for steps it0..it0+nsteps-1, captured
+ * in one hipGraph; *i1 / *i2 swapped as the reference swaps them */
+phc_status phc_sem2d_run(phc_sem2d *h, int it0, int nsteps, int *i1, int *i2);
+phc_status phc_sem2d_set_receivers(phc_sem2d *h, int nrec, const int *nodes);
+/* last run: src[nsteps] = source node after each step (what
+ * SEMsolver::outputPnValues prints), rec[nrec*nsteps] */
+phc_status phc_sem2d_get_traces(phc_sem2d *h, float *src, float *rec);
+phc_status phc_sem2d_get_field(phc_sem2d *h, int level, float *host);
+phc_status phc_sem2d_set_field(phc_sem2d *h, int level, const float *host);
+/* lumped mass M (with 1/c^2) and boundary term S (sum over boundary faces of
+ * w (h/2) / c, the ShGlobal of SEMsolver.cpp:95-110), per node in node order
+ * (S is 0 inside); FP64 sums rounded once.  Either pointer may be NULL. */
+phc_status phc_sem2d_get_mass(phc_sem2d *h, float *mass, float *boundary);
+phc_status phc_sem2d_check_finite(phc_sem2d *h, int level);
+phc_status phc_sem2d_set_timing(phc_sem2d *h, int enable);
+phc_status phc_sem2d_last_run_kernel_ms(phc_sem2d *h, double *ms_per_step);
+phc_status phc_sem2d_synchronize(phc_sem2d *h);
+/* Autotuner (no reference counterpart): rows per wavefront, timed on scratch
+ * copies of the levels. */
+phc_status phc_sem2d_autotune(phc_sem2d *h, int steps_per_trial, int *chunk);
 
 /* SEMmesh::getElementNumberFromPoints (SEMmesh.cpp:266-285). */
 int phc_sem_element_from_point(int ex, int ey, int ez, float lx, float ly, float lz, float x,

@@ -52,6 +52,19 @@ phc_status phc_fd_model_layered(int nx, int ny, int nz, int nlayers, float vtop,
 phc_status phc_sem_model_gaussian(int ex, int ey, int ez, double sigma, float vmin, float vmax,
                                   uint64_t seed, float *model);
 
+/* Element colouring of SEMmesh::getNumberOfElementsByColor / sortElementsByColor
+ * (SEMmesh.cpp:380-429).  ey == 0 (the SEM2D mesh): the reference's 4 colours,
+ * red (even x, even z), green (odd, even), blue (even, odd), yellow (odd, odd),
+ * each list z-major as the reference fills it.  ey > 0: 8 colours
+ * c = px + 2 pz + 4 py by index parities (the reference's 4-colour scheme
+ * ignores ey and leaves every element layer but the first uncoloured), each
+ * list ordered y, z, x with x fastest.  No two elements of a colour share a
+ * node.  Returns the number of colours; counts[c] and
+ * lists[c * max_per_color + k] are filled when non-NULL; max_per_color is
+ * getNumberOfElementsByColor's value. */
+int phc_sem_color_count(int ex, int ey, int ez, int *max_per_color);
+phc_status phc_sem_color_elements(int ex, int ey, int ez, int *counts, int *lists);
+
 /* PML setup of FDTDInit -usePML (FDTDinit.hpp:155-173) and FDTDUtils::init_eta
  * (FDTDutils.hpp:84-107, 129-160), float as the reference computes it:
  * width = (int)(4 * vmin/(2.5 f0) / d); scale = dt*3*vmax*logf(1000)/(2*ndamp*d);

@@ -192,6 +192,41 @@ void orc_sem64_apply_k(orc_sem64 *s, const double *p, double *y, double *M);
 void orc_sem_dense_element_f64(int p, const double *X, const double *Y,
                                const double *Z, double *K, double *Mloc);
 
+/* ---------------------------------------------------------------- SEM2D */
+/* The reference's SEM2D build (sem2d_oracle.c): ex x ez quadrilaterals of
+ * order p on the x-z plane, node = ix + iz nx, element e = i + k ex, first-
+ * order paraxial boundary on all four edges (SEMsolver.cpp:91-123). */
+typedef struct orc_sem2d orc_sem2d;
+orc_sem2d *orc_sem2d_create(int ex, int ez, float lx, float lz, int order, const float *model, float dt);
+void orc_sem2d_destroy(orc_sem2d *s);
+/* sizes[7] = nx, nz, nodes, elements, interior nodes, boundary nodes, faces */
+void orc_sem2d_sizes(const orc_sem2d *s, int *sizes);
+/* element node lists [nE][q*q], interior nodes, boundary nodes (in the
+ * reference's order), faceInfos [nF][2+q], face node -> boundary index [nF][q];
+ * any pointer may be NULL */
+void orc_sem2d_lists(const orc_sem2d *s, int *nodes, int *interior, int *bnodes, int *face, int *fmap);
+void orc_sem2d_coords(const orc_sem2d *s, float *cx, float *cz);
+/* SEMsolver::computeOneStep with SEM2D: source, element loop, interior and
+ * boundary updates of p_i1 from p_i2 */
+void orc_sem2d_step(orc_sem2d *s, float *p_i1, float *p_i2, int src_element, float rhs);
+void orc_sem2d_apply_k(orc_sem2d *s, const float *p, float *y, float *M);
+/* the last step's M, y and boundary term Sh (per boundary node); NULL skips */
+void orc_sem2d_get_my(const orc_sem2d *s, float *M, float *y, float *Sh);
+void orc_sem2d_boundary_term(orc_sem2d *s, float *Sh);
+/* 0 leaves the boundary term out: the edges reflect (a check of what the
+ * paraxial term absorbs) */
+void orc_sem2d_set_paraxial(orc_sem2d *s, int on);
+/* SEMmesh.cpp:380-429: the reference's 4-colour element lists (2D); returns
+ * the per-colour capacity, lists[c * capacity + k] */
+int orc_sem2d_colors(int ex, int ez, int *counts, int *lists);
+/* The 2D step in FP64 on exact box geometry with the same weights: the
+ * yardstick of the FP32 paths' rounding. */
+typedef struct orc_sem2d64 orc_sem2d64;
+orc_sem2d64 *orc_sem2d64_create(int ex, int ez, float lx, float lz, int order, const double *model, double dt,
+                                int src_element);
+void orc_sem2d64_destroy(orc_sem2d64 *s);
+void orc_sem2d64_step(orc_sem2d64 *s, double *p_i1, double *p_i2, double rhs);
+
 #ifdef __cplusplus
 }
 #endif

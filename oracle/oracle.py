@@ -126,6 +126,35 @@ def _bind_sem(L):
     L.orc_sem64_apply_k.argtypes = [ctypes.c_void_p, f64p, f64p, f64p]
     L.orc_sem_dense_element_f64.restype = None
     L.orc_sem_dense_element_f64.argtypes = [c_int, f64p, f64p, f64p, f64p, f64p]
+    V = ctypes.c_void_p
+    L.orc_sem2d_create.restype = V
+    L.orc_sem2d_create.argtypes = [c_int, c_int, c_float, c_float, c_int, f32p, c_float]
+    L.orc_sem2d_destroy.restype = None
+    L.orc_sem2d_destroy.argtypes = [V]
+    L.orc_sem2d_sizes.restype = None
+    L.orc_sem2d_sizes.argtypes = [V, i32p]
+    L.orc_sem2d_lists.restype = None
+    L.orc_sem2d_lists.argtypes = [V, V, V, V, V, V]
+    L.orc_sem2d_coords.restype = None
+    L.orc_sem2d_coords.argtypes = [V, f32p, f32p]
+    L.orc_sem2d_step.restype = None
+    L.orc_sem2d_step.argtypes = [V, f32p, f32p, c_int, c_float]
+    L.orc_sem2d_apply_k.restype = None
+    L.orc_sem2d_apply_k.argtypes = [V, f32p, f32p, f32p]
+    L.orc_sem2d_get_my.restype = None
+    L.orc_sem2d_get_my.argtypes = [V, V, V, V]
+    L.orc_sem2d_boundary_term.restype = None
+    L.orc_sem2d_boundary_term.argtypes = [V, f32p]
+    L.orc_sem2d_set_paraxial.restype = None
+    L.orc_sem2d_set_paraxial.argtypes = [V, c_int]
+    L.orc_sem2d_colors.restype = c_int
+    L.orc_sem2d_colors.argtypes = [c_int, c_int, i32p, i32p]
+    L.orc_sem2d64_create.restype = V
+    L.orc_sem2d64_create.argtypes = [c_int, c_int, c_float, c_float, c_int, f64p, c_double, c_int]
+    L.orc_sem2d64_destroy.restype = None
+    L.orc_sem2d64_destroy.argtypes = [V]
+    L.orc_sem2d64_step.restype = None
+    L.orc_sem2d64_step.argtypes = [V, f64p, f64p, c_double]
 
 
 def _vp(a):
@@ -373,3 +402,165 @@ class SemProblem:
         iz = (n // m.nx) % m.nz
         iy = n // (m.nx * m.nz)
         return cx[ix], cy[iy], cz[iz]
+
+
+class Sem2dProblem:
+    """The reference SEM proxy built with SEM2D (SEMproxy.cpp:10-119 with
+    ey = 0, SEMsolver.cpp:21-125): ex x ez quadrilaterals of order p on the x-z
+    plane, per-element velocity, Ricker source at the first node of
+    `src_element` (or of the element holding `src_point`), paraxial boundary."""
+
+    def __init__(self, ex=65, ez=None, lx=2000.0, lz=None, order=3, dt=0.001, f0=10.0, model=None,
+                 nsamples=1000, src_element=None, src_point=(1001.0, 1001.0), source_order=1,
+                 paraxial=True):
+        ez = ex if ez is None else ez
+        lz = lx if lz is None else lz
+        self.ex, self.ez, self.p = ex, ez, order
+        self.nE = ex * ez
+        if model is None:
+            model = np.full(self.nE, 1500.0, np.float32)
+        self.model = np.ascontiguousarray(model, np.float32)
+        self.dt = np.float32(dt)
+        self.h = lib().orc_sem2d_create(ex, ez, lx, lz, order, self.model, self.dt)
+        if not self.h:
+            raise ValueError("bad 2D mesh")
+        if not paraxial:
+            lib().orc_sem2d_set_paraxial(self.h, 0)
+        sz = np.zeros(7, np.int32)
+        lib().orc_sem2d_sizes(self.h, sz)
+        self.nx, self.nz, self.nN, _, self.nI, self.nB, self.nF = (int(v) for v in sz)
+        q = order + 1
+        self.nodes = np.zeros(self.nE * q * q, np.int32)
+        self.interior = np.zeros(self.nI, np.int32)
+        self.bnodes = np.zeros(self.nB, np.int32)
+        self.faces = np.zeros(self.nF * (2 + q), np.int32)
+        self.fmap = np.zeros(self.nF * q, np.int32)
+        lib().orc_sem2d_lists(self.h, _vp(self.nodes), _vp(self.interior), _vp(self.bnodes),
+                              _vp(self.faces), _vp(self.fmap))
+        self.nodes = self.nodes.reshape(self.nE, q * q)
+        self.faces = self.faces.reshape(self.nF, 2 + q)
+        self.fmap = self.fmap.reshape(self.nF, q)
+        if src_element is None:
+            hx, hz = np.float32(lx / ex), np.float32(lz / ez)
+            ix = min(int(np.float32(src_point[0]) // hx), ex - 1)
+            iz = min(int(np.float32(src_point[1]) // hz), ez - 1)
+            src_element = ix + iz * ex
+        self.src_element = int(src_element)
+        self.src_node = int(self.nodes[self.src_element, 0])
+        self.rhs = np.zeros(nsamples, np.float32)
+        lib().orc_source_term(nsamples, self.dt, f0, source_order, 0, self.rhs)
+        self.p0 = np.zeros(self.nN, np.float32)
+        self.p1 = np.zeros(self.nN, np.float32)
+        self.i1, self.i2 = 0, 1
+        self.it = 0
+
+    def __del__(self):
+        try:
+            lib().orc_sem2d_destroy(self.h)
+        except Exception:
+            pass
+
+    def level(self, i):
+        return self.p0 if i == 0 else self.p1
+
+    def run(self, nsteps, receivers=None):
+        """computeOneStep(i1, i2) -> output(i1) -> swap, as SEMproxy::run does;
+        returns the source-node trace and the receiver traces (node ids)."""
+        tr = np.zeros(nsteps, np.float32)
+        rec = None if receivers is None else np.zeros((len(receivers), nsteps), np.float32)
+        for s in range(nsteps):
+            a, b = self.level(self.i1), self.level(self.i2)
+            lib().orc_sem2d_step(self.h, a, b, self.src_element, self.rhs[self.it + s])
+            tr[s] = a[self.src_node]
+            if rec is not None:
+                rec[:, s] = a[np.asarray(receivers)]
+            self.i1, self.i2 = self.i2, self.i1
+        self.it += nsteps
+        return tr, rec
+
+    def current(self):
+        return self.level(self.i2)
+
+    def previous(self):
+        return self.level(self.i1)
+
+    def coords(self):
+        cx = np.zeros(self.nx, np.float32)
+        cz = np.zeros(self.nz, np.float32)
+        lib().orc_sem2d_coords(self.h, cx, cz)
+        return cx, cz
+
+    def mass_y_sh(self):
+        M = np.zeros(self.nN, np.float32)
+        y = np.zeros(self.nN, np.float32)
+        sh = np.zeros(self.nB, np.float32)
+        lib().orc_sem2d_get_my(self.h, _vp(M), _vp(y), _vp(sh))
+        return M, y, sh
+
+    def boundary_term(self):
+        sh = np.zeros(self.nB, np.float32)
+        lib().orc_sem2d_boundary_term(self.h, sh)
+        return sh
+
+    def apply_k(self, p):
+        y = np.zeros(self.nN, np.float32)
+        M = np.zeros(self.nN, np.float32)
+        lib().orc_sem2d_apply_k(self.h, np.ascontiguousarray(p, np.float32), y, M)
+        return y, M
+
+
+def sem2d_colors(ex, ez):
+    """The reference's 4-colour element lists (SEMmesh.cpp:380-429)."""
+    mpc = (ex // 2 + ex % 2) * (ez // 2 + ez % 2)
+    counts = np.zeros(4, np.int32)
+    lists = np.zeros(4 * mpc, np.int32)
+    lib().orc_sem2d_colors(ex, ez, counts, lists)
+    return [lists[c * mpc:c * mpc + counts[c]] for c in range(4)]
+
+
+class Sem2d64Problem:
+    """The SEM2D step in FP64 on exact box geometry (same mesh, weights,
+    source samples and driver loop as Sem2dProblem)."""
+
+    def __init__(self, ex=65, ez=None, lx=2000.0, lz=None, order=3, dt=0.001, f0=10.0, model=None,
+                 nsamples=1000, src_element=0, source_order=1):
+        ez = ex if ez is None else ez
+        lz = lx if lz is None else lz
+        nE = ex * ez
+        model = np.full(nE, 1500.0) if model is None else np.asarray(model, np.float64)
+        dt32 = np.float32(dt)
+        self.h = lib().orc_sem2d64_create(ex, ez, lx, lz, order, np.ascontiguousarray(model, np.float64),
+                                          float(dt32), int(src_element))
+        if not self.h:
+            raise ValueError("bad 2D mesh")
+        self.nx, self.nz = ex * order + 1, ez * order + 1
+        self.nN = self.nx * self.nz
+        self.src_node = (src_element % ex) * order + (src_element // ex) * order * self.nx
+        self.rhs = np.zeros(nsamples, np.float32)
+        lib().orc_source_term(nsamples, dt32, f0, source_order, 0, self.rhs)
+        self.p0 = np.zeros(self.nN)
+        self.p1 = np.zeros(self.nN)
+        self.i1, self.i2 = 0, 1
+        self.it = 0
+
+    def __del__(self):
+        try:
+            lib().orc_sem2d64_destroy(self.h)
+        except Exception:
+            pass
+
+    def level(self, i):
+        return self.p0 if i == 0 else self.p1
+
+    def run(self, nsteps):
+        tr = np.zeros(nsteps)
+        for s in range(nsteps):
+            a, b = self.level(self.i1), self.level(self.i2)
+            lib().orc_sem2d64_step(self.h, a, b, float(self.rhs[self.it + s]))
+            tr[s] = a[self.src_node]
+            self.i1, self.i2 = self.i2, self.i1
+        self.it += nsteps
+        return tr
+
+    def current(self):
+        return self.level(self.i2)

@@ -26,7 +26,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + INCLUDE, "-I" + CSRC]
 
-HIP_SOURCES = ["phc_common.hip", "setup.hip", "fd.hip", "sem.hip", "batch.hip", "mlp.hip", "fd64.hip", "sem64.hip"]
+HIP_SOURCES = ["phc_common.hip", "setup.hip", "fd.hip", "sem.hip", "batch.hip", "mlp.hip", "fd64.hip", "sem64.hip", "sem2d.hip"]
 
 
 def _run(cmd):
@@ -55,7 +55,7 @@ def build_libphc(jobs=8):
 
 
 def build_oracle():
-    srcs = [os.path.join(ORACLE, s) for s in ("fd_oracle.c", "sem_oracle.c")
+    srcs = [os.path.join(ORACLE, s) for s in ("fd_oracle.c", "sem_oracle.c", "sem2d_oracle.c")
             if os.path.exists(os.path.join(ORACLE, s))]
     tmp = LIBORACLE + ".tmp"
     # -ffp-contract=off: no FMA contraction, so float/double operations round

@@ -17,6 +17,16 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Orders one wavefront's LDS accesses across its lanes (a row one lane wrote
+// and another reads): compiler fences at wavefront scope around a
+// wave-level scheduling barrier; no workgroup barrier.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
 __device__ __forceinline__ float mul_nc(float a, float b)
 {
 #pragma clang fp contract(off)
@@ -52,10 +62,13 @@ __device__ __forceinline__ void sto_nt(T *base, IT off, T v)
 
 // XCD-aware block order: hardware deals consecutive block ids round-robin
 // over the 8 XCDs; give each XCD a contiguous range of tiles so halo re-reads
-// hit its own L2.
+// hit its own L2.  XCD x receives the hardware ids x, x+8, ...: nb/8 + 1 of
+// them for x < nb%8, nb/8 otherwise; the map is a bijection of [0, nb).
 __device__ __forceinline__ int xcd_block(int b, int nb)
 {
-    return ((nb & 7) == 0) ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, l = b >> 3;
+    return (x < r) ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
 }
 
 } // namespace phc

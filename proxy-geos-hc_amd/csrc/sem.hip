@@ -19,6 +19,8 @@
 //    W = Wx Wy Wz turns the update into a radius-p star stencil whose
 //    coefficients depend only on each index modulo p:
 //        pnew = 2 p - pold - Q (Lx p + Ly p + Lz p),   Q = dt^2 W / M.
+//    Each stencil row sums to zero, so it is evaluated on differences from
+//    the centre node, which FP32 resolves far better than the plain sum.
 //    It runs on the marching-tile scheme of the FD kernel: 16 B of HBM traffic
 //    per node update (p, pold, Q read; pnew written).
 //  * element: one thread per GLL point, the element's (p+1)^3 block and the
@@ -196,14 +198,17 @@ sem_sep_kernel(SemSepArgs a)
                 if (!(jcomp[jj] && iin)) continue;
                 const int r = ty * NJ + jj + R;
                 const int c = tx + R;
+                // rows sum to zero: sum over differences from the centre value
+                // (the common part cancels exactly; FP32 keeps the differences)
+                const float pc = q[jj][(u + R) % Q];
                 float lap = 0.f;
 #pragma unroll
                 for (int d = 0; d < W; ++d) {
-                    lap = fmaf(a.cy[my][d], q[jj][(u + d) % Q], lap);
-                    lap = fmaf(czr[jj][d], tl[r - R + d][c], lap);
-                    lap = fmaf(cxr[d], tl[r][c - R + d], lap);
+                    if (d == R) continue;
+                    lap = fmaf(a.cy[my][d], q[jj][(u + d) % Q] - pc, lap);
+                    lap = fmaf(czr[jj][d], tl[r - R + d][c] - pc, lap);
+                    lap = fmaf(cxr[d], tl[r][c - R + d] - pc, lap);
                 }
-                const float pc = q[jj][(u + R) % Q];
                 float val = fmaf(-qv[u & 1][jj], lap, 2.f * pc - pm[u & 1][jj]);
                 const IT o = colq[jj] + pi;
                 if ((int64_t)(o / S) == a.src_off) {
@@ -228,7 +233,12 @@ struct SemElemArgs {
     int64_t geo_stride;  // floats between elements in geo
     float D[(SEM_MAX_P + 1) * (SEM_MAX_P + 1)];  // D[a][b] = l_a'(xi_b)
     int ex, ey, ez, nx, ny, nz;
-    int nelem;
+    int nelem;           // elements this launch covers
+    // colour < 0: every element, atomic scatter.  colour c in 0..7: only the
+    // elements with (ex_i, ez_i, ey_i) parities (c&1, c>>1&1, c>>2&1), which
+    // share no node, scattered with plain read-add-write (deterministic)
+    int colour;
+    int ncx, ncz;        // elements of the colour along x and z
 };
 
 // Thread per GLL point; EPB elements per block.  The element's nodal block
@@ -246,24 +256,35 @@ sem_elem_kernel(SemElemArgs a)
     const int t = threadIdx.x;
     const int le = t / n3;          // element within the block
     const int pt = t - le * n3;     // GLL point (a fastest: x, then z (b), then y (c))
-    const int e = blockIdx.x * EPB + le;
-    const bool act = le < EPB && e < a.nelem;
+    const int k = blockIdx.x * EPB + le;  // element index within this launch
+    const bool act = le < EPB && k < a.nelem;
     if (t < q * q) Ds[t] = a.D[t];
     const int pa = pt % q, pb = (pt / q) % q, pc = pt / (q * q);
-    int node = 0;
+    int node = 0, e = 0;
     if (act) {
-        const int exi = e % a.ex, ezi = (e / a.ex) % a.ez, eyi = e / (a.ex * a.ez);
+        int exi, ezi, eyi;
+        if (a.colour < 0) {
+            exi = k % a.ex; ezi = (k / a.ex) % a.ez; eyi = k / (a.ex * a.ez);
+        } else {
+            exi = 2 * (k % a.ncx) + (a.colour & 1);
+            ezi = 2 * ((k / a.ncx) % a.ncz) + ((a.colour >> 1) & 1);
+            eyi = 2 * (k / (a.ncx * a.ncz)) + ((a.colour >> 2) & 1);
+        }
+        e = exi + ezi * a.ex + eyi * a.ex * a.ez;
         node = (exi * P + pa) + (ezi * P + pb) * a.nx + (eyi * P + pc) * a.nx * a.nz;
         u[le][pt] = a.pn[node];
     }
     __syncthreads();
     if (act) {
+        // the derivative of a constant is zero (sum_m l_m' = 0): differences
+        // from the point's own value keep FP32 precision on smooth fields
+        const float uc = u[le][pt];
         float s1 = 0.f, s2 = 0.f, s3 = 0.f;
 #pragma unroll
         for (int m = 0; m < q; ++m) {
-            s1 = fmaf(Ds[m * q + pa], u[le][m + pb * q + pc * q * q], s1);
-            s2 = fmaf(Ds[m * q + pb], u[le][pa + m * q + pc * q * q], s2);
-            s3 = fmaf(Ds[m * q + pc], u[le][pa + pb * q + m * q * q], s3);
+            s1 = fmaf(Ds[m * q + pa], u[le][m + pb * q + pc * q * q] - uc, s1);
+            s2 = fmaf(Ds[m * q + pb], u[le][pa + m * q + pc * q * q] - uc, s2);
+            s3 = fmaf(Ds[m * q + pc], u[le][pa + pb * q + m * q * q] - uc, s3);
         }
         const float *G = a.geo + (int64_t)e * a.geo_stride + 6 * pt;
         // flux = (w w w B) grad u; B Voigt order 11, 22, 33, 12, 13, 23
@@ -280,7 +301,8 @@ sem_elem_kernel(SemElemArgs a)
             acc = fmaf(Ds[pb * q + m], g[le][1][pa + m * q + pc * q * q], acc);
             acc = fmaf(Ds[pc * q + m], g[le][2][pa + pb * q + m * q * q], acc);
         }
-        atomicAdd(&a.y[node], acc);
+        if (a.colour < 0) atomicAdd(&a.y[node], acc);
+        else a.y[node] += acc;
     }
 }
 
@@ -495,13 +517,29 @@ static phc_status sem_launch(phc_sem *h, int i1, int i2, int inj, float rhs_next
         std::memcpy(a.D, h->Dm, sizeof(a.D));
         a.ex = h->ex; a.ey = h->ey; a.ez = h->ez;
         a.nx = h->nx; a.ny = h->ny; a.nz = h->nz;
-        a.nelem = h->ex * h->ey * h->ez;
         int epb = 0;
         SemElemFn fn = sem_pick_elem(h->p, &epb);
         if (!fn) return fail(PHC_ERR_INVALID, "no element kernel for p=%d", h->p);
         if (e0) PHC_HIP(hipEventRecord(e0, h->stream));
-        hipLaunchKernelGGL(fn, dim3((a.nelem + epb - 1) / epb), dim3(256), 0, h->stream, a);
-        PHC_HIP(hipGetLastError());
+        if (h->kernel == PHC_SEM_ELEMENT) {
+            a.colour = -1;
+            a.nelem = h->ex * h->ey * h->ez;
+            hipLaunchKernelGGL(fn, dim3((a.nelem + epb - 1) / epb), dim3(256), 0, h->stream, a);
+            PHC_HIP(hipGetLastError());
+        } else {
+            // 8 colours in a fixed order: each node receives its (at most 8)
+            // element contributions in the same order on every run
+            for (int c = 0; c < 8; ++c) {
+                a.colour = c;
+                a.ncx = (h->ex - (c & 1) + 1) / 2;
+                a.ncz = (h->ez - ((c >> 1) & 1) + 1) / 2;
+                const int ncy = (h->ey - ((c >> 2) & 1) + 1) / 2;
+                a.nelem = a.ncx * a.ncz * ncy;
+                if (a.nelem == 0) continue;
+                hipLaunchKernelGGL(fn, dim3((a.nelem + epb - 1) / epb), dim3(256), 0, h->stream, a);
+                PHC_HIP(hipGetLastError());
+            }
+        }
         hipLaunchKernelGGL(sem_update_kernel, dim3((unsigned)((h->nn + 255) / 256)), dim3(256), 0,
                            h->stream, h->lev[i1].as<float>(), h->lev[i2].as<float>(),
                            h->y.as<float>(), h->dt2m.as<float>(), h->nx, h->ny, h->nz,
@@ -550,7 +588,7 @@ phc_status phc_sem_create(const phc_sem_params *p, phc_sem **out)
     if (!(p->lx > 0) || !(p->ly > 0) || !(p->lz > 0)) return fail(PHC_ERR_INVALID, "bad domain size");
     if (!p->model) return fail(PHC_ERR_INVALID, "model is NULL");
     if (p->nsamples < 0 || (p->nsamples > 0 && !p->rhs)) return fail(PHC_ERR_INVALID, "bad rhs");
-    if (p->kernel != PHC_SEM_SEPARABLE && p->kernel != PHC_SEM_ELEMENT)
+    if (p->kernel != PHC_SEM_SEPARABLE && p->kernel != PHC_SEM_ELEMENT && p->kernel != PHC_SEM_ELEMENT_COLOR)
         return fail(PHC_ERR_INVALID, "unknown SEM kernel %d", p->kernel);
     const int64_t nE = (int64_t)p->ex * p->ey * p->ez;
     if (p->src_element < 0 || p->src_element >= nE)

@@ -238,4 +238,34 @@ phc_status phc_fd_pml_profile(int n, int ndamp, float scale, float *profile)
     return PHC_OK;
 }
 
+int phc_sem_color_count(int ex, int ey, int ez, int *max_per_color)
+{
+    // SEMmesh.cpp:380-381
+    if (ex < 1 || ez < 1 || ey < 0) return 0;
+    const int m = (ex / 2 + ex % 2) * (ez / 2 + ez % 2) * (ey == 0 ? 1 : ey / 2 + ey % 2);
+    if (max_per_color) *max_per_color = m;
+    return ey == 0 ? 4 : 8;
+}
+
+phc_status phc_sem_color_elements(int ex, int ey, int ez, int *counts, int *lists)
+{
+    int mpc = 0;
+    const int nc = phc_sem_color_count(ex, ey, ez, &mpc);
+    if (nc == 0) return phc::fail(PHC_ERR_INVALID, "mesh %d x %d x %d has no elements", ex, ey, ez);
+    const int ny = ey == 0 ? 1 : ey;
+    for (int c = 0; c < nc; ++c) {
+        // colour bits: x parity, z parity, y parity (2D: red, green, blue, yellow)
+        const int px = c & 1, pz = (c >> 1) & 1, py = (c >> 2) & 1;
+        int k = 0;
+        for (int j = py; j < ny; j += 2)
+            for (int l = pz; l < ez; l += 2)
+                for (int i = px; i < ex; i += 2) {
+                    if (lists) lists[(int64_t)c * mpc + k] = i + l * ex + j * ex * ez;
+                    ++k;
+                }
+        if (counts) counts[c] = k;
+    }
+    return PHC_OK;
+}
+
 } // extern "C"

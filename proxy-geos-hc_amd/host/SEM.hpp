@@ -14,7 +14,9 @@
 //   -order P, -dt DT, -f0 F, -timemax T, -nsteps N, -element (element kernel),
 //   -gaussian SEED (smooth random model 2000-4000 m/s), -receivers (8 on the
 //   line y = ly/2, z = 100 m), -fused (graph-captured runs), -snapshots (the
-//   reference's SEM_SAVE_SNAPSHOTS build option: ASCII snapshot every 100 steps).
+//   reference's SEM_SAVE_SNAPSHOTS build option: ASCII snapshot every 100 steps),
+//   -2d (the reference's SEM2D build option: ey = 0, quadrilaterals on the x-z
+//   plane, paraxial absorbing edges; SEMmacros.hpp:7-11, SEMsolver.cpp:91-123).
 #pragma once
 
 #include <chrono>
@@ -73,24 +75,29 @@ class SEMmesh {
     float lx = 0, ly = 0, lz = 0, hx = 0, hy = 0, hz = 0;
     std::vector<float> model;
     SEMmesh() = default;
-    // SEMmesh.cpp:3-26 (3D)
+    // SEMmesh.cpp:3-26; dim2 is the reference's SEM2D build, which zeroes ey
+    // and ly (ZEROED2D, SEMmacros.hpp:10) and keeps one node layer (ny = 1)
     SEMmesh(const int &ex_in, const int &ey_in, const int &ez_in, const float &lx_in,
-            const float &ly_in, const float &lz_in, const int &order_in)
-        : ex(ex_in), ey(ey_in), ez(ez_in), order(order_in), lx(lx_in), ly(ly_in), lz(lz_in)
+            const float &ly_in, const float &lz_in, const int &order_in, bool dim2 = false)
+        : ex(ex_in), ey(dim2 ? 0 : ey_in), ez(ez_in), order(order_in), lx(lx_in), ly(dim2 ? 0.f : ly_in),
+          lz(lz_in)
     {
         nx = ex * order + 1;
-        ny = ey * order + 1;
+        ny = (ey == 0) ? 1 : ey * order + 1;
         nz = ez * order + 1;
         hx = (float)(lx / (1. * ex));
-        hy = (float)(ly / (1. * ey));
+        hy = (ey == 0) ? 1.f : (float)(ly / (1. * ey));
         hz = (float)(lz / (1. * ez));
-        model.assign((size_t)ex * ey * ez, 1500.f);  // SEMmesh::getModel, SEMmesh.cpp:288-312
+        model.assign((size_t)ex * (ey == 0 ? 1 : ey) * ez, 1500.f);  // SEMmesh::getModel, SEMmesh.cpp:288-312
         printf("SEMmesh initiliazed\nex, ey, ez=%d, %d, %d\nnx, ny, nz=%d, %d, %d\n", ex, ey, ez, nx, ny, nz);
     }
     int getNumberOfNodes() const { return nx * ny * nz; }
-    int getNumberOfElements() const { return ex * ey * ez; }
-    int getNumberOfPointsPerElement() const { return (order + 1) * (order + 1) * (order + 1); }
-    int getNumberOfInteriorNodes() const { return (nx - 2) * (ny - 2) * (nz - 2); }
+    int getNumberOfElements() const { return (ey == 0) ? ex * ez : ex * ey * ez; }
+    int getNumberOfPointsPerElement() const
+    {
+        return (ny == 1) ? (order + 1) * (order + 1) : (order + 1) * (order + 1) * (order + 1);
+    }
+    int getNumberOfInteriorNodes() const { return (ny == 1) ? (nx - 2) * (nz - 2) : (nx - 2) * (ny - 2) * (nz - 2); }
     int getNx() const { return nx; }
     int getNy() const { return ny; }
     int getNz() const { return nz; }
@@ -124,13 +131,32 @@ class SEMmesh {
 
 class SEMsolver {
   public:
-    phc_sem *dev = nullptr;
+    phc_sem *dev = nullptr;      // 3D mesh
+    phc_sem2d *dev2 = nullptr;   // SEM2D mesh (ey = 0)
     int kernel = PHC_SEM_SEPARABLE;
     std::vector<float> rhs;
     ~SEMsolver()
     {
         if (dev) phc_sem_destroy(dev);
+        if (dev2) phc_sem2d_destroy(dev2);
     }
+    phc_status getField(int level, float *host) const
+    {
+        return dev2 ? phc_sem2d_get_field(dev2, level, host) : phc_sem_get_field(dev, level, host);
+    }
+    phc_status run(int it0, int n, int *a, int *b)
+    {
+        return dev2 ? phc_sem2d_run(dev2, it0, n, a, b) : phc_sem_run(dev, it0, n, a, b);
+    }
+    phc_status traces(float *src, float *rec)
+    {
+        return dev2 ? phc_sem2d_get_traces(dev2, src, rec) : phc_sem_get_traces(dev, src, rec);
+    }
+    phc_status setReceivers(int n, const int *nodes)
+    {
+        return dev2 ? phc_sem2d_set_receivers(dev2, n, nodes) : phc_sem_set_receivers(dev, n, nodes);
+    }
+    phc_status synchronize() { return dev2 ? phc_sem2d_synchronize(dev2) : phc_sem_synchronize(dev); }
     // SEMsolver.cpp:12-17: mesh, basis, geometry, mass -- built once in FP64
     void computeFEInit(SEMinfo &info, const SEMmesh &mesh, const arrayReal &RHS)
     {
@@ -148,15 +174,19 @@ class SEMsolver {
         p.kernel = kernel;
         p.weights_ref = 1;
         if (dev) phc_sem_destroy(dev);
+        if (dev2) phc_sem2d_destroy(dev2);
         dev = nullptr;
-        phc_check_sem(phc_sem_create(&p, &dev), "phc_sem_create");
+        dev2 = nullptr;
+        if (mesh.ey == 0) phc_check_sem(phc_sem2d_create(&p, &dev2), "phc_sem2d_create");
+        else phc_check_sem(phc_sem_create(&p, &dev), "phc_sem_create");
     }
     // SEMsolver.cpp:21-125 (same signature; the source samples and the source
     // element were handed over at computeFEInit, pnGlobal stays on the GPU)
     void computeOneStep(const int &timeSample, const int &, const int &, const int &i1, const int &i2,
                         SEMinfo &, const arrayReal &, arrayReal const &, const vectorInt &)
     {
-        phc_check_sem(phc_sem_compute_one_step(dev, timeSample, i1, i2), "computeOneStep");
+        phc_check_sem(dev2 ? phc_sem2d_compute_one_step(dev2, timeSample, i1, i2)
+                           : phc_sem_compute_one_step(dev, timeSample, i1, i2), "computeOneStep");
     }
     // SEMsolver.cpp:127-142
     void outputPnValues(const SEMmesh &mesh, const int &indexTimeStep, int &i1, int &myElementSource,
@@ -164,11 +194,11 @@ class SEMsolver {
     {
         if (indexTimeStep % 100 == 0) {
             std::vector<float> f((size_t)mesh.getNumberOfNodes());
-            phc_check_sem(phc_sem_get_field(dev, i1, f.data()), "get_field");
+            phc_check_sem(getField(i1, f.data()), "get_field");
             for (size_t n = 0; n < f.size(); ++n) pnGlobal((int)n, i1) = f[n];
             const int e = myElementSource;
             const int node = (e % mesh.ex) * mesh.order + ((e / mesh.ex) % mesh.ez) * mesh.order * mesh.nx +
-                             (e / (mesh.ex * mesh.ez)) * mesh.order * mesh.nx * mesh.nz;
+                             (mesh.ey == 0 ? 0 : (e / (mesh.ex * mesh.ez)) * mesh.order * mesh.nx * mesh.nz);
             printf("TimeStep=%d;  pnGlobal @ elementSource location %d after computeOneStep = %g\n",
                    indexTimeStep, myElementSource, pnGlobal(node, i1));
             if (saveSnapShots) mesh.saveSnapShot(indexTimeStep, i1, pnGlobal);
@@ -196,7 +226,7 @@ class SEMproxy {
         const int ex = (argc > 1) ? std::stoi(argv[1]) : 100;
         const float lx = (argc > 2) ? std::stof(argv[2]) : 2000.f;
         unsigned long long seed = 0;
-        bool gaussian = false;
+        bool gaussian = false, dim2 = false;
         for (int i = 3; i < argc; ++i) {
             const std::string a = argv[i];
             auto val = [&]() {
@@ -213,12 +243,13 @@ class SEMproxy {
             else if (a == "-receivers") receivers = true;
             else if (a == "-fused") fused = true;
             else if (a == "-snapshots") mySolver.saveSnapShots = true;
+            else if (a == "-2d") dim2 = true;
             else throw std::runtime_error("unknown argument " + a);
         }
-        myMesh = SEMmesh(ex, ex, ex, lx, lx, lx, myInfo.myOrderNumber);
+        myMesh = SEMmesh(ex, ex, ex, lx, lx, lx, myInfo.myOrderNumber, dim2);
         if (gaussian)
-            phc_check_sem(phc_sem_model_gaussian(ex, ex, ex, 5.0, 2000.f, 4000.f, seed, myMesh.model.data()),
-                          "gaussian model");
+            phc_check_sem(phc_sem_model_gaussian(ex, dim2 ? 1 : ex, ex, 5.0, 2000.f, 4000.f, seed,
+                                                 myMesh.model.data()), "gaussian model");
     }
 
     // SEMproxy.cpp:25-38
@@ -255,12 +286,12 @@ class SEMproxy {
             for (int r = 0; r < 8; ++r) {
                 const double x = myMesh.lx / 10.0 * (r + 1), y = myMesh.ly / 2.0, z = 100.0;
                 const int ix = (int)std::lround(x / (myMesh.lx / (double)(myMesh.nx - 1)));
-                const int iy = (int)std::lround(y / (myMesh.ly / (double)(myMesh.ny - 1)));
+                const int iy = (myMesh.ny == 1) ? 0 : (int)std::lround(y / (myMesh.ly / (double)(myMesh.ny - 1)));
                 const int iz = (int)std::lround(z / (myMesh.lz / (double)(myMesh.nz - 1)));
                 (void)p;
                 recNodes.push_back(myMesh.node(ix, iy, iz));
             }
-            phc_check_sem(phc_sem_set_receivers(mySolver.dev, (int)recNodes.size(), recNodes.data()), "receivers");
+            phc_check_sem(mySolver.setReceivers((int)recNodes.size(), recNodes.data()), "receivers");
         }
     }
 
@@ -285,10 +316,10 @@ class SEMproxy {
                 if (last > n - 1) last = n - 1;
                 const int len = last - it + 1;
                 int a = i1, b = i2;
-                phc_check_sem(phc_sem_run(mySolver.dev, it, len, &a, &b), "phc_sem_run");
+                phc_check_sem(mySolver.run(it, len, &a, &b), "run");
                 if (!recNodes.empty()) {
                     part.assign(recNodes.size() * (size_t)len, 0.f);
-                    phc_check_sem(phc_sem_get_traces(mySolver.dev, nullptr, part.data()), "traces");
+                    phc_check_sem(mySolver.traces(nullptr, part.data()), "traces");
                     for (size_t r = 0; r < recNodes.size(); ++r)
                         std::memcpy(&recTraces[r * n + it], &part[r * len], sizeof(float) * len);
                 }
@@ -301,7 +332,7 @@ class SEMproxy {
                 it += len;
             }
         }
-        phc_check_sem(phc_sem_synchronize(mySolver.dev), "synchronize");
+        phc_check_sem(mySolver.synchronize(), "synchronize");
         const double s = std::chrono::duration<double>(std::chrono::system_clock::now() - t0).count();
         printf("------------------------------------------------ \n\n---- Elapsed Kernel Time : %g seconds.\n", s);
         return s;

@@ -669,6 +669,7 @@ def lb_corr_sizes(nx, ny, nz):
 # ----------------------------------------------------------------------- SEM
 SEM_SEPARABLE = 0
 SEM_ELEMENT = 1
+SEM_ELEMENT_COLOR = 2
 
 
 class SemParams(ctypes.Structure):
@@ -701,6 +702,165 @@ SIGNATURES.update({
     "phc_sem_element_from_point": (c_int, [c_int, c_int, c_int, c_float, c_float, c_float,
                                            c_float, c_float, c_float]),
 })
+
+
+SIGNATURES.update({
+    "phc_sem2d_create": (c_int, [ctypes.POINTER(SemParams), ctypes.POINTER(c_void_p)]),
+    "phc_sem2d_destroy": (c_int, [c_void_p]),
+    "phc_sem2d_sizes": (c_int, [c_void_p] + [ctypes.POINTER(c_int)] * 3),
+    "phc_sem2d_compute_one_step": (c_int, [c_void_p, c_int, c_int, c_int]),
+    "phc_sem2d_run": (c_int, [c_void_p, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
+    "phc_sem2d_set_receivers": (c_int, [c_void_p, c_int, c_void_p]),
+    "phc_sem2d_get_traces": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "phc_sem2d_get_field": (c_int, [c_void_p, c_int, f32p]),
+    "phc_sem2d_set_field": (c_int, [c_void_p, c_int, f32p]),
+    "phc_sem2d_get_mass": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "phc_sem2d_check_finite": (c_int, [c_void_p, c_int]),
+    "phc_sem2d_set_timing": (c_int, [c_void_p, c_int]),
+    "phc_sem2d_last_run_kernel_ms": (c_int, [c_void_p, ctypes.POINTER(c_double)]),
+    "phc_sem2d_synchronize": (c_int, [c_void_p]),
+    "phc_sem2d_autotune": (c_int, [c_void_p, c_int, ctypes.POINTER(c_int)]),
+})
+
+
+class Sem2dSolver:
+    """SEMsolver of the reference's SEM2D build on the device: ex x ez
+    quadrilaterals of order p on the x-z plane (SEMmesh with ey = 0), per-element
+    wave speed, Ricker source at the first node of the element holding
+    `src_point` (or `src_element`), paraxial boundary on the four edges."""
+
+    def __init__(self, ex=65, lx=2000.0, order=3, dt=0.001, f0=10.0, model=None, nsamples=1000,
+                 src_point=(1001.0, 1001.0, 1001.0), ez=None, lz=None, src_element=None,
+                 weights_ref=True, source_order=1, rhs=None, chunk=0):
+        ez = ex if ez is None else ez
+        lz = lx if lz is None else lz
+        nE = ex * ez
+        self.ex, self.ez, self.p = ex, ez, order
+        self.model = (np.full(nE, 1500.0, np.float32) if model is None
+                      else np.ascontiguousarray(model, np.float32))
+        self.dt = np.float32(dt)
+        self.rhs = (source_term(nsamples, self.dt, f0, source_order, EXP_FLOAT) if rhs is None
+                    else np.ascontiguousarray(rhs, np.float32))
+        if src_element is None:
+            src_element = lib().phc_sem_element_from_point(ex, 0, ez, lx, 0.0, lz, *src_point)
+        self.src_element = int(src_element)
+        p = SemParams()
+        p.ex, p.ey, p.ez = ex, 0, ez
+        p.lx, p.ly, p.lz = lx, 0.0, lz
+        p.order = order
+        p.model = _ptr(self.model)
+        p.dt = self.dt
+        p.rhs = _ptr(self.rhs)
+        p.nsamples = len(self.rhs)
+        p.src_element = self.src_element
+        p.kernel = SEM_SEPARABLE
+        p.weights_ref = 1 if weights_ref else 0
+        p.chunk_i = chunk
+        h = c_void_p()
+        check(lib().phc_sem2d_create(ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        nn, nx, nz = c_int(), c_int(), c_int()
+        check(lib().phc_sem2d_sizes(h, ctypes.byref(nn), ctypes.byref(nx), ctypes.byref(nz)))
+        self.nn, self.nx, self.nz = nn.value, nx.value, nz.value
+        exi, ezi = self.src_element % ex, self.src_element // ex
+        self.src_node = exi * order + ezi * order * self.nx
+        self.i1, self.i2 = 0, 1
+        self.nrec = 0
+        self.it = 0
+
+    def close(self):
+        if self.h:
+            check(lib().phc_sem2d_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def compute_one_step(self, time_sample, i1, i2):
+        check(lib().phc_sem2d_compute_one_step(self.h, time_sample, i1, i2))
+
+    def step(self):
+        """One iteration of SEMproxy::run (SEMproxy.cpp:45-55)."""
+        self.compute_one_step(self.it, self.i1, self.i2)
+        self.i1, self.i2 = self.i2, self.i1
+        self.it += 1
+
+    def run(self, nsteps):
+        i1, i2 = c_int(self.i1), c_int(self.i2)
+        check(lib().phc_sem2d_run(self.h, self.it, nsteps, ctypes.byref(i1), ctypes.byref(i2)))
+        self.i1, self.i2 = i1.value, i2.value
+        self.it += nsteps
+        src = np.zeros(nsteps, np.float32)
+        rec = np.zeros(max(self.nrec, 1) * nsteps, np.float32)
+        check(lib().phc_sem2d_get_traces(self.h, _ptr(src), _ptr(rec) if self.nrec else None))
+        return src, (rec.reshape(self.nrec, nsteps) if self.nrec else None)
+
+    def set_receivers(self, nodes):
+        a = np.ascontiguousarray(nodes, np.int32)
+        self.nrec = len(a)
+        check(lib().phc_sem2d_set_receivers(self.h, self.nrec, _ptr(a)))
+
+    def field(self, level):
+        out = np.zeros(self.nn, np.float32)
+        check(lib().phc_sem2d_get_field(self.h, level, out))
+        return out
+
+    def set_field(self, level, arr):
+        check(lib().phc_sem2d_set_field(self.h, level, np.ascontiguousarray(arr, np.float32)))
+
+    def current(self):
+        return self.field(self.i2)
+
+    def previous(self):
+        return self.field(self.i1)
+
+    def mass(self):
+        """(lumped mass M, boundary term S) per node."""
+        m = np.zeros(self.nn, np.float32)
+        b = np.zeros(self.nn, np.float32)
+        check(lib().phc_sem2d_get_mass(self.h, _ptr(m), _ptr(b)))
+        return m, b
+
+    def set_timing(self, on):
+        check(lib().phc_sem2d_set_timing(self.h, 1 if on else 0))
+
+    def last_kernel_ms(self):
+        v = c_double()
+        check(lib().phc_sem2d_last_run_kernel_ms(self.h, ctypes.byref(v)))
+        return v.value
+
+    def synchronize(self):
+        check(lib().phc_sem2d_synchronize(self.h))
+
+    def check_finite(self, level):
+        return lib().phc_sem2d_check_finite(self.h, level)
+
+    def autotune(self, steps_per_trial=10):
+        c = c_int()
+        check(lib().phc_sem2d_autotune(self.h, steps_per_trial, ctypes.byref(c)))
+        return c.value
+
+
+SIGNATURES.update({
+    "phc_sem_color_count": (c_int, [c_int, c_int, c_int, ctypes.POINTER(c_int)]),
+    "phc_sem_color_elements": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p]),
+})
+
+
+def sem_color_elements(ex, ey, ez):
+    """Element lists per colour (SEMmesh::sortElementsByColor; 8 colours in 3D,
+    the reference's 4 when ey == 0)."""
+    mpc = c_int()
+    nc = lib().phc_sem_color_count(ex, ey, ez, ctypes.byref(mpc))
+    if nc == 0:
+        raise PhcError("bad mesh")
+    counts = np.zeros(nc, np.int32)
+    lists = np.zeros(nc * mpc.value, np.int32)
+    check(lib().phc_sem_color_elements(ex, ey, ez, _ptr(counts), _ptr(lists)))
+    return [lists[c * mpc.value:c * mpc.value + counts[c]] for c in range(nc)]
 
 
 def sem_model_gaussian(ex, ey, ez, sigma=5.0, vmin=2000.0, vmax=4000.0, seed=1):

@@ -37,3 +37,16 @@ def test_errors_are_reported_without_a_device(phc):
     assert st == -1
     assert b"empty" in phc.lib().phc_last_error()
     assert phc.lib().phc_fd_run(None, 0, 1, None) == -1
+
+
+def test_sem2d_parameters_checked_without_a_device(phc):
+    import ctypes
+    p = phc.SemParams()
+    p.ex, p.ey, p.ez, p.order = 4, 4, 4, 2
+    h = ctypes.c_void_p()
+    assert phc.lib().phc_sem2d_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    assert b"ey = 0" in phc.lib().phc_last_error()
+    p.ey = 0
+    p.ez = 1
+    assert phc.lib().phc_sem2d_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    assert b"2 x 2" in phc.lib().phc_last_error()

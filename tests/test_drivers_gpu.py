@@ -115,3 +115,35 @@ def test_sem_driver_snapshots(gpu, oracle, tmp_path, fused):
         assert x == "%g" % (i * hx) and z == "%g" % (j * hx)
         assert abs(float(v) - field[off + i + j * nx]) <= 1e-4 * scale + 1e-5 * abs(field[off + i + j * nx])
     assert os.path.exists(str(tmp_path / "snapshot0")) and os.path.exists(str(tmp_path / "snapshot100"))
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_sem_driver_2d(gpu, oracle, tmp_path, fused):
+    """sem_hip 65 2000 -2d: the reference's SEM2D build on its published mesh
+    (65 x 65 elements of order 3); header lines as the reference prints them,
+    source element from SEMmesh::getElementNumberFromPoints with the float
+    element size of the checked-in revision, prints and the ASCII snapshot of
+    the single node plane against the SEM2D oracle."""
+    print("\n[drivers] sem_hip -2d fused=%d" % fused, flush=True)
+    args = ["65", "2000", "-2d", "-order", "3", "-dt", "0.001", "-f0", "10", "-nsteps", "301", "-snapshots"]
+    out = run("sem_hip", args + (["-fused"] if fused else []), str(tmp_path))
+    assert "ex, ey, ez=65, 0, 65" in out and "nx, ny, nz=196, 1, 196" in out
+    assert "Number of nodes: 38416" in out and "Number of elements: 4225" in out
+    m = re.search(r"Element number for the source location: (\d+)", out)
+    e = int(m.group(1))
+    assert e == 32 + 32 * 65  # 1001 / (2000/65) = 32.5 on both axes
+    got = {int(m.group(1)): float(m.group(2)) for m in
+           re.finditer(r"TimeStep=(\d+);  pnGlobal @ elementSource location \d+ after computeOneStep = (\S+)", out)}
+    O = oracle.Sem2dProblem(ex=65, order=3, lx=2000.0, dt=1e-3, f0=10.0, nsamples=301, src_element=e)
+    tr, _ = O.run(301)
+    for step in (0, 100, 200, 300):
+        assert abs(got[step] - tr[step]) <= 1e-4 * np.abs(tr).max() + 1e-5 * abs(tr[step]), (step, got[step], tr[step])
+    field = O.current()
+    lines = [l.split() for l in open(str(tmp_path / "snapshot300")).read().split("\n") if l.strip()]
+    assert len(lines) == 196 * 196
+    scale = np.abs(field).max()
+    for n in range(0, len(lines), 97):
+        x, z, v = lines[n]
+        i, j = n // 196, n % 196
+        assert x == "%g" % (i * 30) and z == "%g" % (j * 30)
+        assert abs(float(v) - field[i + j * 196]) <= 1e-4 * scale + 1e-5 * abs(field[i + j * 196])

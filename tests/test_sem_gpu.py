@@ -12,7 +12,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = [0, 1]  # SEM_SEPARABLE, SEM_ELEMENT
+KERNELS = [0, 1, 2]  # SEM_SEPARABLE, SEM_ELEMENT, SEM_ELEMENT_COLOR
 
 
 def banner(msg):
@@ -142,6 +142,25 @@ def test_separable_and_element_kernels_agree(gpu):
     tb, _ = b.run(100)
     assert rel_l2(a.current(), b.current()) < 1e-5
     assert rel_l2(ta, tb) < 1e-5
+
+
+@pytest.mark.parametrize("ex,ey,ez", [(7, 5, 6), (4, 4, 4)])
+def test_colour_scatter_repeats_bit_for_bit(gpu, ex, ey, ez):
+    """8-colour scatter: plain adds in a fixed order, so two runs agree bit
+    for bit; the atomic scatter agrees to rounding."""
+    banner("colour scatter %dx%dx%d" % (ex, ey, ez))
+    kw = dict(ex=ex, ey=ey, ez=ez, lx=40.0 * ex, ly=40.0 * ey, lz=40.0 * ez, order=3, dt=0.0008, f0=15.0,
+              nsamples=90, src_point=(20.0 * ex + 1, 20.0 * ey + 1, 20.0 * ez + 1))
+    a = gpu.SemSolver(kernel=2, **kw)
+    b = gpu.SemSolver(kernel=2, **kw)
+    c = gpu.SemSolver(kernel=1, **kw)
+    ta, _ = a.run(80)
+    for _ in range(80):
+        b.step()
+    tc, _ = c.run(80)
+    assert np.array_equal(a.current(), b.current()) and np.array_equal(a.previous(), b.previous())
+    assert rel_l2(a.current(), c.current()) < 2e-6
+    assert rel_l2(ta, tc) < 2e-6
 
 
 def test_fp32_gpu_vs_fp64_oracle(gpu, oracle):
