@@ -255,6 +255,9 @@ def main():
         sys.exit(1)
     if args.workload == "sem2d":
         r = bench_sem2d(phc, big_steps=args.steps if "--steps" in sys.argv else 200)
+        cms = ctypes.c_double()  # plain 1 GiB copy: calibrates the rocprof byte counters
+        phc.check(phc.lib().phc_copy_bandwidth(1 << 30, 5, ctypes.byref(cms)))
+        r["copy_gbs"] = round(2 * (1 << 30) / (cms.value * 1e-3) / 1e9, 1)
         big = r["large"]
         out = {"metric": METRIC, "value": big["value"], "unit": big["unit"], "n_gpus": 1,
                "steps": args.steps if "--steps" in sys.argv else 200, "warmup": 0,

@@ -61,8 +61,12 @@ struct SemSepArgs {
 // Separable SEM step.  Same march as fd_step_kernel (register rings unrolled
 // by 2P+2, halo FIFO, double-buffered LDS plane, LDS-only barrier); the
 // stencil has per-position coefficients and a Dirichlet mask.
+// Register budget: the one-row, 16-wave shapes of orders 1-4 fit in 64 VGPRs
+// without spilling, i.e. 8 waves per SIMD instead of 6 (more loads in flight);
+// the other shapes keep the budget their block size implies.
 template <int P, int TY, int NJ, typename IT>
 __global__ void __launch_bounds__(FD_TK *TY)
+__attribute__((amdgpu_waves_per_eu((NJ == 1 && TY == 16 && P <= 4) ? 8 : (TY >= 4 ? TY / 4 : 1))))
 sem_sep_kernel(SemSepArgs a)
 {
     constexpr int R = P;

@@ -41,6 +41,7 @@ namespace phc {
 constexpr int S2_MAX_P = 5;
 constexpr int S2_W = 2 * S2_MAX_P + 1;
 constexpr int S2_WAVES = 4;  // waves per block
+constexpr int S2_DEPTH = 2;  // rows prefetched ahead of the one computed
 
 // 2D layout: node (ix, iz), ix in [-p, tiles_x*64 + p), iz in [-p, nz + p),
 // at origin + iz*pitch + ix; interior ix = 0 starts 32 floats into a
@@ -117,54 +118,71 @@ __global__ void __launch_bounds__(64 * S2_WAVES) sem2d_step_kernel(Sem2dArgs a)
     float rz[W];  // rz[d] = p at row z - P + d
 #pragma unroll
     for (int d = 0; d < 2 * P; ++d) rz[d + 1] = ldo(a.pn, col + (IT)(z0 - P + d) * pitch);
-    float nxt = ldo(a.pn, col + (IT)min(z0 + P, L.nz - 1 + P) * pitch);
-    float hv = ldo(a.pn, hcolo + (IT)z0 * pitch);
-
-    for (int z = z0; z < z1; ++z) {
+    // rows z+1 .. z+D are in flight while row z is computed: slot s of the
+    // buffers holds row zb + s (the ring entry P rows further down, the halo
+    // columns, pold and Q); rows are clamped inside the array
+    float nb[S2_DEPTH], hb[S2_DEPTH], pb[S2_DEPTH], qb[S2_DEPTH];
 #pragma unroll
-        for (int d = 0; d < 2 * P; ++d) rz[d] = rz[d + 1];
-        rz[2 * P] = nxt;
-        const float hcur = hv;
-        const IT zo = (IT)z * pitch;
-        const float po = ldo_nt(a.pnm1, col + zo);
-        const float q = ldo_nt(a.qf, col + zo);
-        // prefetch the next row's ring entry and halo (clamped inside the halo rows)
-        nxt = ldo(a.pn, col + (IT)min(z + 1 + P, L.nz - 1 + P) * pitch);
-        hv = ldo(a.pn, hcolo + (IT)min(z + 1, L.nz - 1) * pitch);
+    for (int s = 0; s < S2_DEPTH; ++s) {
+        const IT zr = (IT)min(z0 + s, L.nz - 1) * pitch;
+        nb[s] = ldo(a.pn, col + (IT)min(z0 + s + P, L.nz - 1 + P) * pitch);
+        hb[s] = ldo(a.pn, hcolo + zr);
+        pb[s] = ldo_nt(a.pnm1, col + zr);
+        qb[s] = ldo_nt(a.qf, col + zr);
+    }
 
-        row[P + lane] = rz[P];
-        if (hl) row[hcol] = hcur;
-        wave_lds_sync();
-        const int cz = (z == 0) ? P : (z == L.nz - 1) ? P + 1 : z % P;
-        // every stencil row sums to zero (the operator annihilates constants),
-        // so the sum is taken over differences from the centre value: the
-        // large common part cancels exactly and FP32 keeps the small
-        // differences a smooth field produces
-        const float pc = rz[P];
-        float lap = 0.f;
+    for (int zb = z0; zb < z1; zb += S2_DEPTH) {
 #pragma unroll
-        for (int d = 0; d < W; ++d) {
-            if (d == P) continue;
-            lap = fmaf(a.cz[cz][d], rz[d] - pc, lap);
-            lap = fmaf(cxr[d], row[lane + d] - pc, lap);
+        for (int s = 0; s < S2_DEPTH; ++s) {
+            const int z = zb + s;
+            if (z >= z1) break;
+#pragma unroll
+            for (int d = 0; d < 2 * P; ++d) rz[d] = rz[d + 1];
+            rz[2 * P] = nb[s];
+            const float hcur = hb[s], po = pb[s], q = qb[s];
+            {
+                const IT zn = (IT)min(z + S2_DEPTH, L.nz - 1) * pitch;
+                nb[s] = ldo(a.pn, col + (IT)min(z + S2_DEPTH + P, L.nz - 1 + P) * pitch);
+                hb[s] = ldo(a.pn, hcolo + zn);
+                pb[s] = ldo_nt(a.pnm1, col + zn);
+                qb[s] = ldo_nt(a.qf, col + zn);
+            }
+            const IT zo = (IT)z * pitch;
+
+            row[P + lane] = rz[P];
+            if (hl) row[hcol] = hcur;
+            wave_lds_sync();
+            const int cz = (z == 0) ? P : (z == L.nz - 1) ? P + 1 : z % P;
+            // every stencil row sums to zero (the operator annihilates
+            // constants), so the sum is taken over differences from the centre
+            // value: the large common part cancels exactly and FP32 keeps the
+            // small differences a smooth field produces
+            const float pc = rz[P];
+            float lap = 0.f;
+#pragma unroll
+            for (int d = 0; d < W; ++d) {
+                if (d == P) continue;
+                lap = fmaf(a.cz[cz][d], rz[d] - pc, lap);
+                lap = fmaf(cxr[d], row[lane + d] - pc, lap);
+            }
+            wave_lds_sync();
+            if (!xin) continue;
+            float val;
+            const bool edge = z == 0 || z == L.nz - 1 || x == 0 || x == L.nx - 1;
+            if (!edge) {
+                val = fmaf(-q, lap, 2.f * pc - po);
+            } else {
+                const float sg = (z == 0) ? a.sig_z0[x] : (z == L.nz - 1) ? a.sig_z1[x]
+                               : (x == 0) ? a.sig_x0[z] : a.sig_x1[z];
+                val = fmaf(-q, lap, fmaf(1.f - sg, -po, 2.f * pc)) / (1.f + sg);
+            }
+            const IT o = col + zo;
+            if ((int64_t)(o / S) == a.src_off) {
+                if (a.trace) *a.trace = val;
+                if (a.inj) val = add_nc(val, mul_nc(a.inj_coef, a.rhs_next));
+            }
+            sto_nt(a.pnm1, o, val);
         }
-        wave_lds_sync();
-        if (!xin) continue;
-        float val;
-        const bool edge = z == 0 || z == L.nz - 1 || x == 0 || x == L.nx - 1;
-        if (!edge) {
-            val = fmaf(-q, lap, 2.f * pc - po);
-        } else {
-            const float sg = (z == 0) ? a.sig_z0[x] : (z == L.nz - 1) ? a.sig_z1[x]
-                           : (x == 0) ? a.sig_x0[z] : a.sig_x1[z];
-            val = fmaf(-q, lap, fmaf(1.f - sg, -po, 2.f * pc)) / (1.f + sg);
-        }
-        const IT o = col + zo;
-        if ((int64_t)(o / S) == a.src_off) {
-            if (a.trace) *a.trace = val;
-            if (a.inj) val = add_nc(val, mul_nc(a.inj_coef, a.rhs_next));
-        }
-        sto_nt(a.pnm1, o, val);
     }
 }
 
