@@ -14,8 +14,11 @@ Beside the headline line: "batch" reports the batched multi-shot workload
 (64 uneven shots, 800 steps, one persistent launch per step) and the same shots
 run one at a time; "sem_p4" the SEM order-4 workload (401^3 nodes, FP32
 separable kernel, and its FP32-vs-FP64 precision over 2000 steps);
-"learned_boundary" the learned thin layer trained from scratch on the GPU and
-compared with the 40-point sponge against a 2x-wide FP64 domain.
+"sem2d" the SEM2D step on the reference's published 2D mesh and on a large 2D
+mesh; "pml" the FD step with the reference's PML; "sem_element_kernels" the
+element kernels (atomic and 8-colour scatter) on the reference's CPU-runnable
+SEM case; "learned_boundary" the learned thin layer trained from scratch on the
+GPU and compared with the 40-point sponge against a 2x-wide FP64 domain.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
 """
@@ -73,6 +76,36 @@ def cpu_baseline(setup, nsteps=3):
     return {"value": pts * nsteps / dt / 1e6, "unit": "Mpoint-updates/s", "cores": cores,
             "kind": "port",
             "sample": "%d steps of the same 512^3 layered workload (oracle/fd_oracle.c, OpenMP)" % nsteps}
+
+
+def cpu_baseline_sem_p4(ex=20, steps=2):
+    """The SEM oracle (the reference's element loop restated: computeB V2 +
+    gradPhiGradPhi per element, sequential scatter) on one host core, on a
+    bounded sample of the order-4 workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    P = oracle.SemProblem(ex=ex, lx=20.0 * ex, order=4, dt=2e-4, f0=8.0, nsamples=steps + 1,
+                          src_point=(10.0 * ex + 2.5, 10.0 * ex + 2.5, 10.0 * ex + 2.5))
+    P.run(1)
+    t0 = time.perf_counter()
+    P.run(steps)
+    dt = time.perf_counter() - t0
+    return {"value": round(P.nN * steps / dt / 1e6, 2), "unit": "Mnode-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d steps of a %d^3-element order-4 mesh (%d^3 nodes), oracle/sem_oracle.c"
+                      % (steps, ex, 4 * ex + 1)}
+
+
+def cpu_baseline_sem2d(steps=200):
+    """The SEM2D oracle (the reference's SEM2D element loop and paraxial edges
+    restated) on one host core, on the published 2D mesh."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    O = oracle.Sem2dProblem(ex=65, order=3, lx=2000.0, dt=1e-3, f0=10.0, nsamples=steps, src_element=2178)
+    t0 = time.perf_counter()
+    O.run(steps)
+    dt = time.perf_counter() - t0
+    return {"value": round(O.nN * steps / dt / 1e6, 2), "unit": "Mnode-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d steps of the published 65 x 65 order-3 mesh, oracle/sem2d_oracle.c" % steps}
 
 
 def bench_batch(phc, nsteps=800, nshots=64, baselines=True, shape=None):
@@ -205,6 +238,51 @@ def bench_sem2d(phc, steps=1000, big_ex=2000, big_order=4, big_steps=200):
     return out
 
 
+def bench_pml(phc, n=512, steps=200):
+    """FD with the reference's PML (-usePML, FDTDkernels.hpp:36-104) on the
+    headline grid: width (int)(4 vmin/(2.5 f0)/dx) = 24 at vmin 1500 m/s,
+    f0 10 Hz, dx 10 m; one PML launch on the slabs plus the stencil on the inner
+    box per step (HIP events around both).  Algorithmic bytes: 16 per inner
+    point, 24 per slab point (pn, pnm1, vp^2dt^2, phi read; phi, pnew written)."""
+    setup = phc.FdSetup(nx=n, model=("layered", 6, 20240512), f0=10.0, cfl=0.4, nsamples=steps + 1,
+                        sponge_width=0)
+    sol = phc.FdSolver(setup, arith=phc.ARITH_FP32)
+    sol.autotune(steps_per_trial=5)
+    nd = sol.set_pml(f0=10.0)
+    sol.set_timing(True)
+    sol.run(0, steps)
+    kms = sol.last_kernel_ms()
+    fin = sol.check_finite(sol.i2) == 0
+    sol.close()
+    inner = (n - 2 * nd) ** 3
+    slab = n ** 3 - inner
+    ach = (16 * inner + 24 * slab) / (kms * 1e-3) / 1e9
+    return {"workload": "FD %d^3 radius 4 with a %d-point PML, %d steps, FP32" % (n, nd, steps),
+            "pml_width": nd, "ms_per_step": round(kms, 5), "value": round(n ** 3 / (kms * 1e-3) / 1e6, 1),
+            "unit": "Mpoint-updates/s", "finite": fin,
+            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}}
+
+
+def bench_sem_elem(phc, steps=100):
+    """The reference's CPU-runnable SEM case (50^3 elements of order 2, 101^3
+    nodes, c = 1500, f0 = 5 Hz, dt = 1e-3) through the three SEM step kernels:
+    separable, element with atomic scatter, element with the 8-colour
+    scatter."""
+    out = {"workload": "SEM 50^3 elements of order 2 (101^3 nodes), %d steps, FP32" % steps}
+    for name, kern in (("separable", 0), ("element_atomic", 1), ("element_colour", 2)):
+        s = phc.SemSolver(ex=50, lx=2000.0, order=2, dt=1e-3, f0=5.0, nsamples=steps + 1, kernel=kern)
+        if kern == 0:
+            s.autotune(steps_per_trial=5)
+        s.set_timing(True)
+        s.run(steps)
+        ms = s.last_kernel_ms()
+        s.close()
+        out[name] = {"ms_per_step": round(ms, 5), "value": round(101 ** 3 / (ms * 1e-3) / 1e6, 1),
+                     "unit": "Mnode-updates/s"}
+    return out
+
+
 def bench_learned(phc, n=256, steps=2000):
     """Learned thin boundary: trained from scratch on the GPU (200 wide-sponge
     shots, Adam), then one shot on an n^3 region: the 4-point learned layer,
@@ -234,7 +312,7 @@ def main():
     ap.add_argument("--arith", default="fp32", choices=["fp32", "ref"])
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched multi-shot measurement reported beside the headline")
-    ap.add_argument("--workload", default="fd", choices=["fd", "batch", "sem", "sem2d"],
+    ap.add_argument("--workload", default="fd", choices=["fd", "batch", "sem", "sem2d", "pml", "semelem"],
                     help="fd: the 512^3 headline; batch: only the batched multi-shot workload "
                          "(--steps steps, default 800); sem: only the SEM order-4 workload")
     ap.add_argument("--batch-shape", default=None,
@@ -253,6 +331,17 @@ def main():
     if phc.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload in ("pml", "semelem"):
+        r = bench_pml(phc) if args.workload == "pml" else bench_sem_elem(phc)
+        head = r if args.workload == "pml" else r["element_colour"]
+        out = {"metric": METRIC, "value": head["value"], "unit": head["unit"], "n_gpus": 1, "steps": 0,
+               "warmup": 0, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+               "config": {"workload": r["workload"]}, args.workload: r}
+        if args.workload == "pml":
+            out["roofline"] = r["roofline"]
+        print(json.dumps(out), flush=True)
+        return
     if args.workload == "sem2d":
         r = bench_sem2d(phc, big_steps=args.steps if "--steps" in sys.argv else 200)
         cms = ctypes.c_double()  # plain 1 GiB copy: calibrates the rocprof byte counters
@@ -375,9 +464,17 @@ def main():
         out["batch"] = bench_batch(phc)
         progress("batched shots done")
         out["sem_p4"] = bench_sem_p4(phc)
+        if not args.no_cpu:
+            out["sem_p4"]["cpu_baseline"] = cpu_baseline_sem_p4()
         progress("SEM order 4 done")
         out["sem2d"] = bench_sem2d(phc)
+        if not args.no_cpu:
+            out["sem2d"]["cpu_baseline"] = cpu_baseline_sem2d()
         progress("SEM2D done")
+        out["pml"] = bench_pml(phc)
+        progress("PML done")
+        out["sem_element_kernels"] = bench_sem_elem(phc)
+        progress("SEM element kernels done")
         out["learned_boundary"] = bench_learned(phc)
         progress("learned boundary done")
     if not args.no_cpu:

@@ -105,6 +105,8 @@ struct FdPmlArgs {
     float hd2[3];
     int x3, x4, y3, y4, z3, z4;  // inner box
     int rowsA, rowsB, rowsC;
+    int kblocks;                 // blocks of 128 along k per x- or y-slab row
+    int nbAB;                    // blocks of the x and y slabs (rowsA + rowsB) * kblocks
     int64_t src_off;
     int inj;
     float rhs_next;
@@ -117,31 +119,37 @@ __global__ void __launch_bounds__(128) fd_pml_kernel(FdPmlArgs a)
 #pragma clang fp contract(off)
     const FdLayout &L = a.L;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
-    const int row = blockIdx.y;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    // 1D grid.  x slabs (rows A) and y slabs (rows B): full k rows, 128
+    // consecutive k per block.  z slabs (C): the points of all inner (i, j)
+    // columns flattened with the 2 nd slab values of k fastest, so consecutive
+    // threads read consecutive k.
+    const int b = blockIdx.x;
     int i, j, k;
-    if (row < a.rowsA) {
-        const int q = row / ny;
-        j = row - q * ny;
-        i = q < a.x3 ? q : a.x4 + (q - a.x3);
-        k = t;
+    if (b < a.nbAB) {
+        const int row = b / a.kblocks;
+        k = (b - row * a.kblocks) * 128 + threadIdx.x;
         if (k >= nz) return;
-    } else if (row < a.rowsA + a.rowsB) {
-        const int r = row - a.rowsA;
-        const int nyb = a.y3 + ny - a.y4;
-        const int q = r % nyb;
-        i = a.x3 + r / nyb;
-        j = q < a.y3 ? q : a.y4 + (q - a.y3);
-        k = t;
-        if (k >= nz) return;
+        if (row < a.rowsA) {
+            const int q = row / ny;
+            j = row - q * ny;
+            i = q < a.x3 ? q : a.x4 + (q - a.x3);
+        } else {
+            const int r = row - a.rowsA;
+            const int nyb = a.y3 + ny - a.y4;
+            const int q = r % nyb;
+            i = a.x3 + r / nyb;
+            j = q < a.y3 ? q : a.y4 + (q - a.y3);
+        }
     } else {
-        const int r = row - a.rowsA - a.rowsB;
+        const int64_t g = (int64_t)(b - a.nbAB) * 128 + threadIdx.x;
         const int nzb = a.z3 + nz - a.z4;
-        const int q = r % nzb;
-        i = a.x3 + r / nzb;
+        const int64_t col = g / nzb;
+        const int q = (int)(g - col * nzb);
+        const int nyi = a.y4 - a.y3;
+        if (col >= (int64_t)(a.x4 - a.x3) * nyi) return;
+        i = a.x3 + (int)(col / nyi);
+        j = a.y3 + (int)(col % nyi);
         k = q < a.z3 ? q : a.z4 + (q - a.z3);
-        j = a.y3 + t;
-        if (j >= a.y4) return;
     }
     const int64_t c = fd_off(L, i, j, k);
     const int64_t si = L.pitch_j, sj = L.pitch_k;
@@ -474,8 +482,12 @@ static phc_status fd_launch_pml(phc_fd *h, const FdStepArgs &a, int ca, int cb)
     if (rows == 0) return PHC_OK;
     FdPmlFn fn = fd_pick_pml(h->R, h->arith);
     if (!fn) return fail(PHC_ERR_INVALID, "no PML kernel for half width %d", h->R);
-    const int len = std::max(h->nz, P.y4 - P.y3);
-    hipLaunchKernelGGL(fn, dim3((len + 127) / 128, rows), dim3(128), 0, h->stream, P);
+    P.kblocks = (h->nz + 127) / 128;
+    P.nbAB = (P.rowsA + P.rowsB) * P.kblocks;
+    const int64_t nc = (int64_t)(P.x4 - P.x3) * (P.y4 - P.y3) * (P.z3 + h->nz - P.z4);
+    const int64_t nb = (int64_t)P.nbAB + (nc + 127) / 128;
+    if (nb > INT32_MAX) return fail(PHC_ERR_INVALID, "PML slabs too large for one launch");
+    hipLaunchKernelGGL(fn, dim3((unsigned)nb), dim3(128), 0, h->stream, P);
     PHC_HIP(hipGetLastError());
     return PHC_OK;
 }
@@ -896,7 +908,7 @@ phc_status phc_fd_set_pml(phc_fd *h, const int *ndamp, const float *etax, const 
     {
         const int64_t rows = (int64_t)2 * ndamp[0] * n[1] + (int64_t)(n[0] - 2 * ndamp[0]) * 2 * ndamp[1] +
                              (int64_t)(n[0] - 2 * ndamp[0]) * 2 * ndamp[2];
-        if (rows > 65535) return fail(PHC_ERR_INVALID, "PML slabs of %lld rows exceed 65535", (long long)rows);
+        if (rows > INT32_MAX) return fail(PHC_ERR_INVALID, "PML slabs of %lld rows exceed 2^31 - 1", (long long)rows);
     }
     if (h->sw != 0) return fail(PHC_ERR_INVALID, "the PML replaces the sponge: sponge_width must be 0");
     if (h->lb_on) return fail(PHC_ERR_STATE, "a learned boundary is attached");
