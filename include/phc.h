@@ -408,6 +408,11 @@ phc_status phc_lb_eval(phc_lb *h, float *loss, float *grad);
 /* Adam (lr, 0.9, 0.999, 1e-8) on minibatches of batch_per_face random samples
  * of every face; loss_history[s] (may be NULL) = minibatch loss at step s. */
 phc_status phc_lb_train(phc_lb *h, int steps, int batch_per_face, float lr, uint64_t seed, float *loss_history);
+/* Gaussian noise of standard deviation std added to the field inputs of every
+ * training sample (the clean target kept), from the next phc_lb_train on:
+ * the network then learns not to amplify errors in its own inputs, which a
+ * rollout feeds back.  0 (the default) trains on the samples as stored. */
+phc_status phc_lb_set_train_noise(phc_lb *h, float std);
 /* Network output y of face `face` for n given samples (MFMA inference path). */
 phc_status phc_lb_infer(phc_lb *h, int face, const float *samples, int64_t n, float *y);
 /* Attach a copy of the model's bf16 weights to an FD handle (lb NULL detaches).

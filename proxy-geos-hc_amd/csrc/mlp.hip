@@ -136,39 +136,80 @@ struct LbCollectArgs {
     int64_t face_off[LB_FACES + 1];  // compact offsets of the region's faces
 };
 
-// One thread per layer point of the region.  Target: the term the region's
-// update misses, t = sum_{l>d} (c_l/c_1) (p(P + l n_out) - h) / s, read from
-// the wide grid around the region.
-__global__ void lb_collect_kernel(LbCollectArgs A)
+// Ordered append: every thread of the block calls this with its keep flag;
+// kept threads get consecutive slots from *base in thread order (wave ballots
+// and a prefix over the block's waves), and *base advances by the block's
+// total.  Sample stores therefore fill in the same order on every run.
+constexpr int LB_CB = 256;  // threads per collection block
+__device__ inline unsigned long long lb_ordered_slot(bool kept, unsigned long long *base, int *wtot)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= A.face_off[LB_FACES]) return;
-    int f = 0;
-    while (f + 1 < LB_FACES && t >= A.face_off[f + 1]) ++f;
+    const unsigned long long bal = __ballot(kept);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) wtot[wv] = __popcll(bal);
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int w = 0; w < LB_CB / 64; ++w) {
+        before += (w < wv) ? wtot[w] : 0;
+        total += wtot[w];
+    }
+    const unsigned long long slot = *base + (unsigned long long)before +
+                                    (unsigned long long)__popcll(bal & ((1ull << lane) - 1ull));
+    __syncthreads();
+    if (threadIdx.x == 0) *base += (unsigned long long)total;
+    __syncthreads();
+    return slot;
+}
+
+// One block per face, walking its layer points in compact order (d, u, v).
+// Target: the term the region's update misses,
+// t = sum_{l>d} (c_l/c_1) (p(P + l n_out) - h) / s, read from the wide grid
+// around the region.
+__global__ void __launch_bounds__(LB_CB) lb_collect_kernel(LbCollectArgs A)
+{
+    __shared__ int wtot[LB_CB / 64];
+    __shared__ unsigned long long base;
+    const int f = blockIdx.x;
     const LbFaceGeom g = lb_face(A.G.n, f);
-    int64_t r = t - A.face_off[f];
-    const int v = (int)(r % g.nv);
-    r /= g.nv;
-    const int u = (int)(r % g.nu);
-    const int d = (int)(r / g.nu);
-    const uint32_t hv = lb_hash(A.seed * 0x9e3779b97f4a7c15ull + (uint64_t)t);
-    if ((float)(hv >> 8) * (1.f / 16777216.f) >= A.keep) return;
-    float x[LB_NIN];
-    int64_t center;
-    const float s = lb_inputs(A.G, g, d, u, v, A.h, A.c1, x, &center);
-    if (!(s >= A.s_min) || !(s > LB_S_TINY)) return;
+    const int64_t nf = A.face_off[f + 1] - A.face_off[f];
+    if (threadIdx.x == 0) base = A.count[f];
+    __syncthreads();
     const int dir = g.side ? -1 : 1;
     const int64_t step = g.axis == 0 ? A.G.L.pitch_j : (g.axis == 1 ? A.G.L.pitch_k : 1);
-    float tgt = 0.f;
-    for (int l = d + 1; l <= LB_R; ++l) tgt += A.chat[l] * (A.G.pn[center - (int64_t)l * dir * step] - A.h);
-    tgt /= s;
-    const unsigned long long slot = atomicAdd(A.count + f, 1ull);
-    if ((int64_t)slot >= A.cap) return;
-    float *o = A.out[f] + slot * LB_NSTRIDE;
-    for (int m = 0; m < LB_NIN; ++m) o[m] = x[m];
-    o[LB_NIN] = tgt;
-    o[LB_NIN + 1] = (float)d;
-    o[LB_NIN + 2] = (float)(t - A.face_off[f]);  // compact index in the face's layer
+    for (int64_t c0 = 0; c0 < nf; c0 += LB_CB) {
+        const int64_t r0 = c0 + threadIdx.x;
+        bool kept = false;
+        float x[LB_NIN];
+        float tgt = 0.f;
+        int d = 0;
+        if (r0 < nf) {
+            const int64_t t = A.face_off[f] + r0;
+            int64_t r = r0;
+            const int v = (int)(r % g.nv);
+            r /= g.nv;
+            const int u = (int)(r % g.nu);
+            d = (int)(r / g.nu);
+            const uint32_t hv = lb_hash(A.seed * 0x9e3779b97f4a7c15ull + (uint64_t)t);
+            if ((float)(hv >> 8) * (1.f / 16777216.f) < A.keep) {
+                int64_t center;
+                const float sc = lb_inputs(A.G, g, d, u, v, A.h, A.c1, x, &center);
+                if (sc >= A.s_min && sc > LB_S_TINY) {
+                    for (int l = d + 1; l <= LB_R; ++l)
+                        tgt += A.chat[l] * (A.G.pn[center - (int64_t)l * dir * step] - A.h);
+                    tgt /= sc;
+                    kept = true;
+                }
+            }
+        }
+        const unsigned long long slot = lb_ordered_slot(kept, &base, wtot);
+        if (kept && (int64_t)slot < A.cap) {
+            float *o = A.out[f] + slot * LB_NSTRIDE;
+            for (int m = 0; m < LB_NIN; ++m) o[m] = x[m];
+            o[LB_NIN] = tgt;
+            o[LB_NIN + 1] = (float)d;
+            o[LB_NIN + 2] = (float)r0;  // compact index in the face's layer
+        }
+    }
+    if (threadIdx.x == 0) A.count[f] = base;
 }
 
 // On-policy samples: a thin run (the region itself, with the learned layer)
@@ -192,40 +233,57 @@ struct LbOnPolicyArgs {
     int64_t face_off[LB_FACES + 1];
 };
 
-__global__ void lb_onpolicy_kernel(LbOnPolicyArgs A)
+// One block per face of the thin run, layer points in compact order.
+__global__ void __launch_bounds__(LB_CB) lb_onpolicy_kernel(LbOnPolicyArgs A)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= A.face_off[LB_FACES]) return;
-    int f = 0;
-    while (f + 1 < LB_FACES && t >= A.face_off[f + 1]) ++f;
+    __shared__ int wtot[LB_CB / 64];
+    __shared__ unsigned long long base;
+    const int f = blockIdx.x;
     const LbFaceGeom g = lb_face(A.T.n, f);
-    int64_t r = t - A.face_off[f];
-    const int v = (int)(r % g.nv);
-    r /= g.nv;
-    const int u = (int)(r % g.nu);
-    const int d = (int)(r / g.nu);
-    const uint32_t hv = lb_hash(A.seed * 0x9e3779b97f4a7c15ull + (uint64_t)t);
-    if ((float)(hv >> 8) * (1.f / 16777216.f) >= A.keep) return;
-    float x[LB_NIN];
-    int64_t P;
-    const float s = lb_inputs(A.T, g, d, u, v, A.h, A.c1, x, &P);
-    if (!(s >= A.s_min) || !(s > LB_S_TINY)) return;
-    int c[3];
-    lb_point(A.T.n, g, d, u, v, c);
+    const int64_t nf = A.face_off[f + 1] - A.face_off[f];
+    if (threadIdx.x == 0) base = A.count[f];
+    __syncthreads();
     const int dir = g.side ? -1 : 1;
     const int64_t step = g.axis == 0 ? A.LW.pitch_j : (g.axis == 1 ? A.LW.pitch_k : 1);
-    const int64_t Pw = fd_off(A.LW, A.wo[0] + c[0], A.wo[1] + c[1], A.wo[2] + c[2]);
-    float tgt = 0.f;
-    for (int l = d + 1; l <= LB_R; ++l) tgt += A.chat[l] * (A.wcur[Pw - (int64_t)l * dir * step] - A.h);
-    tgt /= s;
-    if (!(fabsf(tgt) <= A.t_max)) return;  // the thin state has lost the wave: no usable label
-    const unsigned long long slot = atomicAdd(A.count + f, 1ull);
-    if ((int64_t)slot >= A.cap) return;
-    float *o = A.out[f] + slot * LB_NSTRIDE;
-    for (int m = 0; m < LB_NIN; ++m) o[m] = x[m];
-    o[LB_NIN] = tgt;
-    o[LB_NIN + 1] = (float)d;
-    o[LB_NIN + 2] = (float)(t - A.face_off[f]);
+    for (int64_t c0 = 0; c0 < nf; c0 += LB_CB) {
+        const int64_t r0 = c0 + threadIdx.x;
+        bool kept = false;
+        float x[LB_NIN];
+        float tgt = 0.f;
+        int d = 0;
+        if (r0 < nf) {
+            const int64_t t = A.face_off[f] + r0;
+            int64_t r = r0;
+            const int v = (int)(r % g.nv);
+            r /= g.nv;
+            const int u = (int)(r % g.nu);
+            d = (int)(r / g.nu);
+            const uint32_t hv = lb_hash(A.seed * 0x9e3779b97f4a7c15ull + (uint64_t)t);
+            if ((float)(hv >> 8) * (1.f / 16777216.f) < A.keep) {
+                int64_t P;
+                const float sc = lb_inputs(A.T, g, d, u, v, A.h, A.c1, x, &P);
+                if (sc >= A.s_min && sc > LB_S_TINY) {
+                    int c[3];
+                    lb_point(A.T.n, g, d, u, v, c);
+                    const int64_t Pw = fd_off(A.LW, A.wo[0] + c[0], A.wo[1] + c[1], A.wo[2] + c[2]);
+                    for (int l = d + 1; l <= LB_R; ++l)
+                        tgt += A.chat[l] * (A.wcur[Pw - (int64_t)l * dir * step] - A.h);
+                    tgt /= sc;
+                    // the thin state has lost the wave: no usable label
+                    kept = fabsf(tgt) <= A.t_max;
+                }
+            }
+        }
+        const unsigned long long slot = lb_ordered_slot(kept, &base, wtot);
+        if (kept && (int64_t)slot < A.cap) {
+            float *o = A.out[f] + slot * LB_NSTRIDE;
+            for (int m = 0; m < LB_NIN; ++m) o[m] = x[m];
+            o[LB_NIN] = tgt;
+            o[LB_NIN + 1] = (float)d;
+            o[LB_NIN + 2] = (float)r0;
+        }
+    }
+    if (threadIdx.x == 0) A.count[f] = base;
 }
 
 // ------------------------------------------------------------ training
@@ -237,7 +295,22 @@ struct LbBatch {
     int count[LB_FACES];
     int block_start[LB_FACES + 1];
     float inv_total;  // 1 / total samples in the batch
+    float noise;      // std of the Gaussian noise added to the field inputs (0: none)
+    uint64_t nseed;
+    // non-null: each block writes its gradient sum to partial[block][p] and
+    // its loss to ploss[block] (summed in a fixed order by lb_reduce_kernel,
+    // so training repeats bit for bit); null: float atomics into grad, loss
+    float *partial, *ploss;
 };
+
+// Standard normal from four hashed uniforms (Irwin-Hall, rescaled to unit variance).
+__device__ inline float lb_gauss(uint64_t key)
+{
+    float acc = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc += (float)(lb_hash(key * 4ull + (uint64_t)r) >> 8) * (1.f / 16777216.f);
+    return (acc - 2.f) * 1.7320508f;
+}
 
 // Forward, loss and backward of one batch: every block takes LB_TB samples of
 // one face; the weights are used rounded to bf16 (the stored inference
@@ -269,6 +342,14 @@ __global__ void __launch_bounds__(LB_TB) lb_train_kernel(const float *__restrict
         for (int m = 0; m < LB_NIN; ++m) sx[tid][m] = s[m];
         t = s[LB_NIN];
         d = min(max((int)s[LB_NIN + 1], 0), LB_W - 1);
+        if (B.noise > 0.f) {
+            // denoising: perturb the field inputs that exist at this depth
+            // (offsets inside the region), keep the clean target, so the
+            // network learns not to amplify errors in its own inputs
+            const uint64_t key = (B.nseed * 0x9e3779b97f4a7c15ull) ^ ((uint64_t)f << 40) ^ (uint64_t)si;
+            for (int m = 0; m < 18; ++m)
+                if (m % 9 >= 4 - d) sx[tid][m] += B.noise * lb_gauss(key * 32ull + (uint64_t)m);
+        }
     } else {
         for (int m = 0; m < LB_NIN; ++m) sx[tid][m] = 0.f;
     }
@@ -305,7 +386,8 @@ __global__ void __launch_bounds__(LB_TB) lb_train_kernel(const float *__restrict
     if (tid == 0) {
         float sum = 0.f;
         for (int k = 0; k < LB_TB; ++k) sum += sloss[k];
-        atomicAdd(loss, sum * B.inv_total);
+        if (B.ploss) B.ploss[blockIdx.x] = sum * B.inv_total;
+        else atomicAdd(loss, sum * B.inv_total);
     }
     float *gf = grad + f * LB_NP;
     for (int p = tid; p < LB_NP; p += LB_TB) {
@@ -328,7 +410,26 @@ __global__ void __launch_bounds__(LB_TB) lb_train_kernel(const float *__restrict
         } else {
             for (int k = 0; k < LB_TB; ++k) g += sdy[k];
         }
-        atomicAdd(gf + p, g);
+        if (B.partial) B.partial[(int64_t)blockIdx.x * LB_NP + p] = g;
+        else atomicAdd(gf + p, g);
+    }
+}
+
+// Gradient of each face = its blocks' partial sums added in block order;
+// loss = the blocks' partial losses added in block order.
+__global__ void lb_reduce_kernel(LbBatch B, float *grad, float *loss)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < LB_FACES * LB_NP) {
+        const int f = t / LB_NP, p = t - f * LB_NP;
+        float g = 0.f;
+        for (int b = B.block_start[f]; b < B.block_start[f + 1]; ++b) g += B.partial[(int64_t)b * LB_NP + p];
+        grad[t] = g;
+    }
+    if (t == 0) {
+        float acc = 0.f;
+        for (int b = 0; b < B.block_start[LB_FACES]; ++b) acc += B.ploss[b];
+        *loss = acc;
     }
 }
 
@@ -661,6 +762,7 @@ struct phc_lb {
     DevBuf count;  // unsigned long long per face
     int64_t cap = 0;
     int64_t adam_t = 0;
+    float noise = 0.f;  // training input noise (phc_lb_set_train_noise)
     hipStream_t stream = nullptr;
 };
 
@@ -873,8 +975,7 @@ phc_status phc_lb_collect(phc_lb *h, phc_fd *wide, int cur, const int *origin, c
         A.face_off[f + 1] = A.face_off[f] + (int64_t)LB_W * g.nu * g.nv;
     }
     PHC_HIP(hipStreamSynchronize(fv.stream));
-    const int64_t n = A.face_off[LB_FACES];
-    hipLaunchKernelGGL(lb_collect_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, A);
+    hipLaunchKernelGGL(lb_collect_kernel, dim3(LB_FACES), dim3(LB_CB), 0, h->stream, A);
     PHC_HIP(hipGetLastError());
     PHC_HIP(hipStreamSynchronize(h->stream));
     return PHC_OK;
@@ -927,8 +1028,7 @@ phc_status phc_lb_collect_onpolicy(phc_lb *h, phc_fd *thin, int thin_cur, phc_fd
     }
     PHC_HIP(hipStreamSynchronize(tv.stream));
     PHC_HIP(hipStreamSynchronize(wv.stream));
-    const int64_t cnt = A.face_off[LB_FACES];
-    hipLaunchKernelGGL(lb_onpolicy_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, h->stream, A);
+    hipLaunchKernelGGL(lb_onpolicy_kernel, dim3(LB_FACES), dim3(LB_CB), 0, h->stream, A);
     PHC_HIP(hipGetLastError());
     PHC_HIP(hipStreamSynchronize(h->stream));
     return PHC_OK;
@@ -946,6 +1046,9 @@ phc_status phc_lb_eval(phc_lb *h, float *loss, float *grad)
     for (int f = 0; f < LB_FACES; ++f) tot += c[f];
     if (tot == 0) return fail(PHC_ERR_STATE, "no samples");
     LbBatch B;
+    B.noise = 0.f;
+    B.nseed = 0;
+    B.partial = B.ploss = nullptr;
     B.block_start[0] = 0;
     for (int f = 0; f < LB_FACES; ++f) {
         B.samples[f] = h->data[f].as<float>();
@@ -968,8 +1071,18 @@ phc_status phc_lb_eval(phc_lb *h, float *loss, float *grad)
     return PHC_OK;
 }
 
+phc_status phc_lb_set_train_noise(phc_lb *h, float std)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    if (!(std >= 0.f) || std > 1.f) return fail(PHC_ERR_INVALID, "noise std %g outside [0, 1]", std);
+    h->noise = std;
+    return PHC_OK;
+}
+
 // Adam on minibatches of batch_per_face random samples of each face;
-// loss_history[s] (may be NULL) = minibatch loss before step s.
+// loss_history[s] (may be NULL) = minibatch loss before step s.  The gradient
+// of each step is summed per block and the blocks in a fixed order, so a run
+// repeats bit for bit.
 phc_status phc_lb_train(phc_lb *h, int steps, int batch_per_face, float lr, uint64_t seed, float *loss_history)
 {
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
@@ -992,18 +1105,24 @@ phc_status phc_lb_train(phc_lb *h, int steps, int batch_per_face, float lr, uint
     B.inv_total = 1.f / ((float)batch_per_face * nf);
     const int np = LB_FACES * LB_NP;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    DevBuf lh;
+    DevBuf lh, part, pl;
     PHC_TRY(lh.alloc(sizeof(float) * std::max(steps, 1)));
-    lb_fill(h, h->grad.as<float>(), np, 0.f);
+    PHC_TRY(part.alloc(sizeof(float) * (size_t)B.block_start[LB_FACES] * LB_NP));
+    PHC_TRY(pl.alloc(sizeof(float) * (size_t)B.block_start[LB_FACES]));
+    B.partial = part.as<float>();
+    B.ploss = pl.as<float>();
     for (int s = 0; s < steps; ++s) {
         for (int f = 0; f < LB_FACES; ++f)
             if (c[f] > 0)
                 hipLaunchKernelGGL(lb_random_idx_kernel, dim3((batch_per_face + 255) / 256), dim3(256), 0, h->stream,
                                    h->idx.as<int>() + (size_t)f * batch_per_face, batch_per_face, (int)c[f],
                                    seed * 1000003ull + (uint64_t)s * LB_FACES + f);
-        lb_fill(h, lh.as<float>() + s, 1, 0.f);
+        B.noise = h->noise;
+        B.nseed = seed * 7919ull + (uint64_t)s;
         hipLaunchKernelGGL(lb_train_kernel, dim3(B.block_start[LB_FACES]), dim3(LB_TB), 0, h->stream,
                            h->theta.as<float>(), h->grad.as<float>(), lh.as<float>() + s, B);
+        hipLaunchKernelGGL(lb_reduce_kernel, dim3((np + 255) / 256), dim3(256), 0, h->stream, B,
+                           h->grad.as<float>(), lh.as<float>() + s);
         ++h->adam_t;
         const float c1 = 1.f / (1.f - std::pow(b1, (float)h->adam_t));
         const float c2 = 1.f / (1.f - std::pow(b2, (float)h->adam_t));

@@ -32,10 +32,25 @@ phc_status fail(phc_status code, const char *fmt, ...);
         if (s_ != PHC_OK) return s_;                                           \
     } while (0)
 
-// Owning device allocation (hipMalloc) with explicit release.
+// Owning device allocation (hipMalloc): move-only, freed by release() or
+// when it goes out of scope.
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) {
+            release();
+            ptr = o.ptr; bytes = o.bytes;
+            o.ptr = nullptr; o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
     phc_status alloc(size_t n);
     void release();
     template <class T> T *as() const { return static_cast<T *>(ptr); }

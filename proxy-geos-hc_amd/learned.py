@@ -124,8 +124,10 @@ def onpolicy_data(lb, n_shots=60, region=64, margin=8, steps=600, collect_every=
     return lb.counts()
 
 
-def train(lb, steps=6000, batch_per_face=4096, lr=2e-3, lr_final=1e-4, seed=7, log=None):
-    """Adam with a geometric learning-rate decay in 6 stages; returns the loss history."""
+def train(lb, steps=6000, batch_per_face=4096, lr=2e-3, lr_final=1e-4, seed=7, log=None, noise=0.0):
+    """Adam with a geometric learning-rate decay in 6 stages; returns the loss
+    history.  noise: std of the Gaussian noise on the training inputs."""
+    lb.set_train_noise(noise)
     hist = []
     stages = 6
     for k in range(stages):

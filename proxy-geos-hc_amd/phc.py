@@ -571,6 +571,7 @@ SIGNATURES.update({
                                         c_float, ctypes.c_uint64]),
     "phc_lb_eval": (c_int, [c_void_p, ctypes.POINTER(c_float), c_void_p]),
     "phc_lb_train": (c_int, [c_void_p, c_int, c_int, c_float, ctypes.c_uint64, c_void_p]),
+    "phc_lb_set_train_noise": (c_int, [c_void_p, c_float]),
     "phc_lb_infer": (c_int, [c_void_p, c_int, f32p, ctypes.c_int64, f32p]),
     "phc_fd_set_learned_boundary": (c_int, [c_void_p, c_void_p, c_float, c_float]),
     "phc_fd_lb_corrections": (c_int, [c_void_p, c_int, f32p]),
@@ -642,6 +643,9 @@ class LearnedBoundary:
         g = np.zeros(LB_FACES * LB_NP, np.float32) if with_grad else None
         check(lib().phc_lb_eval(self.h, ctypes.byref(loss), _ptr(g)))
         return loss.value, (g.reshape(LB_FACES, LB_NP) if with_grad else None)
+
+    def set_train_noise(self, std):
+        check(lib().phc_lb_set_train_noise(self.h, std))
 
     def train(self, steps, batch_per_face=4096, lr=1e-3, seed=1):
         hist = np.zeros(max(steps, 1), np.float32)

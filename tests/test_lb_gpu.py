@@ -95,10 +95,20 @@ def test_collected_samples_match_oracle(gpu):
         assert np.array_equal(S[:, lo.NIN + 1].astype(int), d[idx])
         total += len(S)
     assert total > 1000
-    # subsampling keeps about the requested fraction
+    # the store fills in compact order (the same order on every run)
+    for f in range(lo.FACES):
+        assert np.all(np.diff(lb.samples(f)[:, 21]) > 0), f
+    # subsampling keeps about the requested fraction; a second collection of
+    # the same state appends an identical block
     lb.clear_samples()
     lb.collect(sol, sol.i2, o, n, keep=0.25, seed=9)
     assert 0.15 * total < lb.counts().sum() < 0.35 * total
+    first = [lb.samples(f).copy() for f in range(lo.FACES)]
+    lb.collect(sol, sol.i2, o, n, keep=0.25, seed=9)
+    for f in range(lo.FACES):
+        S = lb.samples(f)
+        k = len(first[f])
+        assert len(S) == 2 * k and np.array_equal(S[:k], first[f]) and np.array_equal(S[k:], first[f]), f
 
 
 def thin_solver(gpu, n, steps, seed=4, src=None):
@@ -219,3 +229,27 @@ def test_onpolicy_samples_match_oracle(gpu):
         assert np.array_equal(S[:, lo.NIN + 1].astype(int), d[idx])
         total += len(S)
     assert total > 1000
+
+
+@pytest.mark.parametrize("noise", [0.0, 0.02])
+def test_training_repeats_bit_for_bit(gpu, noise):
+    """Per-block gradient sums added in a fixed order: two trainings from the
+    same parameters, samples and seed end with identical parameters and loss
+    histories (with and without input noise)."""
+    banner("training determinism, noise %g" % noise)
+    rng = np.random.default_rng(5)
+    samples = random_samples(rng, [3000, 2500, 0, 1800, 900, 4000])
+    runs = []
+    for _ in range(2):
+        lb = gpu.LearnedBoundary(seed=9)
+        lb.set_capacity(4096)
+        for f, S in enumerate(samples):
+            if len(S):
+                lb.add_samples(f, S)
+        lb.set_train_noise(noise)
+        hist = lb.train(40, batch_per_face=1024, lr=1e-3, seed=3)
+        runs.append((hist, lb.params()))
+        lb.close()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+    assert np.all(np.isfinite(runs[0][0]))
