@@ -46,6 +46,21 @@ fd_step_kernel(FdStepArgs a)
     fd_tile<R, TY, NJ, ARITH, IT>(a, a, kt, rest % a.tiles_j, rest / a.tiles_j, tile);
 }
 
+// computeOneStepPML (FDTDkernels.hpp:87-104) in one launch: the march over
+// the whole grid, pml3D (FDTDkernels.hpp:36-73) on the slab points, inner3D
+// inside.
+template <int R, int TY, int NJ, int ARITH, typename IT>
+__global__ void __launch_bounds__(FD_TK *TY)
+fd_step_pml_kernel(FdStepArgs a)
+{
+    using SH = FdTileShape<R, TY, NJ>;
+    __shared__ float tile[2][SH::LH][SH::LW];
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int kt = b % a.tiles_k;
+    const int rest = b / a.tiles_k;
+    fd_tile<R, TY, NJ, ARITH, IT, true>(a, a, kt, rest % a.tiles_j, rest / a.tiles_j, tile);
+}
+
 // addRHS (FDTDkernels.hpp:121-128): one point.
 __global__ void fd_add_rhs_kernel(float *p, const float *vp, int64_t off, const float *rhs, int it)
 {
@@ -83,142 +98,20 @@ __global__ void fd_sample_kernel(const float *p, const int64_t *offs, int nrec, 
     out[(int64_t)r * stride + step] = (o == src_off) ? src_trace[step] : p[o];
 }
 
-// PML step on the six slabs of the grid (FDTDkernels.hpp:36-73 pml3D, the
-// regions of computeOneStepPML FDTDkernels.hpp:87-104): one thread per point,
-// stencil values from cache.  The slab points are walked as rows (blockIdx.y,
-// decoded once per block): i-slabs (all j, k along the row), j-slabs inside
-// the i range (k along the row), k-slabs inside both (j along the row).
-// eta(i,j,k) = ex[i+1] + ey[j+1] + ez[k+1] (profiles over -1..n).  With
-// ARITH_REF each value is rounded where the reference's expression rounds:
-// float stencil sums, lap formed in double and stored as float, the update
-// in double; phi in float with a double division.
-struct FdPmlArgs {
-    const float *pn;
-    float *pnm1;
-    const float *vp;
-    float *phi;
-    const float *ex, *ey, *ez;
-    FdLayout L;
-    float cx[FD_MAX_R + 1], cy[FD_MAX_R + 1], cz[FD_MAX_R + 1];
-    double coef0;
-    float c0f;
-    float hd2[3];
-    int x3, x4, y3, y4, z3, z4;  // inner box
-    int rowsA, rowsB, rowsC;
-    int kblocks;                 // blocks of 128 along k per x- or y-slab row
-    int nbAB;                    // blocks of the x and y slabs (rowsA + rowsB) * kblocks
-    int64_t src_off;
-    int inj;
-    float rhs_next;
-    float *trace;
-};
-
-template <int R, int ARITH>
-__global__ void __launch_bounds__(128) fd_pml_kernel(FdPmlArgs a)
-{
-#pragma clang fp contract(off)
-    const FdLayout &L = a.L;
-    const int nx = L.nx, ny = L.ny, nz = L.nz;
-    // 1D grid.  x slabs (rows A) and y slabs (rows B): full k rows, 128
-    // consecutive k per block.  z slabs (C): the points of all inner (i, j)
-    // columns flattened with the 2 nd slab values of k fastest, so consecutive
-    // threads read consecutive k.
-    const int b = blockIdx.x;
-    int i, j, k;
-    if (b < a.nbAB) {
-        const int row = b / a.kblocks;
-        k = (b - row * a.kblocks) * 128 + threadIdx.x;
-        if (k >= nz) return;
-        if (row < a.rowsA) {
-            const int q = row / ny;
-            j = row - q * ny;
-            i = q < a.x3 ? q : a.x4 + (q - a.x3);
-        } else {
-            const int r = row - a.rowsA;
-            const int nyb = a.y3 + ny - a.y4;
-            const int q = r % nyb;
-            i = a.x3 + r / nyb;
-            j = q < a.y3 ? q : a.y4 + (q - a.y3);
-        }
-    } else {
-        const int64_t g = (int64_t)(b - a.nbAB) * 128 + threadIdx.x;
-        const int nzb = a.z3 + nz - a.z4;
-        const int64_t col = g / nzb;
-        const int q = (int)(g - col * nzb);
-        const int nyi = a.y4 - a.y3;
-        if (col >= (int64_t)(a.x4 - a.x3) * nyi) return;
-        i = a.x3 + (int)(col / nyi);
-        j = a.y3 + (int)(col % nyi);
-        k = q < a.z3 ? q : a.z4 + (q - a.z3);
-    }
-    const int64_t c = fd_off(L, i, j, k);
-    const int64_t si = L.pitch_j, sj = L.pitch_k;
-    const float *pn = a.pn;
-    float lapx = 0.f, lapy = 0.f, lapz = 0.f;
-#pragma unroll
-    for (int l = 1; l <= R; ++l) lapx += a.cx[l] * (pn[c + l * si] + pn[c - l * si]);
-#pragma unroll
-    for (int l = 1; l <= R; ++l) lapy += a.cy[l] * (pn[c + l * sj] + pn[c - l * sj]);
-#pragma unroll
-    for (int l = 1; l <= R; ++l) lapz += a.cz[l] * (pn[c + l] + pn[c - l]);
-    const float pc = pn[c], pm = a.pnm1[c], v = a.vp[c], ph = a.phi[c];
-    const float fy0 = a.ey[j + 1], fz0 = a.ez[k + 1];
-    const float e = (a.ex[i + 1] + fy0) + fz0;
-    const float t1 = (((a.ex[i + 2] + fy0) + fz0) - ((a.ex[i] + fy0) + fz0)) * (pn[c + si] - pn[c - si]) * a.hd2[0];
-    const float t2 = (((a.ex[i + 1] + a.ey[j + 2]) + fz0) - ((a.ex[i + 1] + a.ey[j]) + fz0)) *
-                     (pn[c + sj] - pn[c - sj]) * a.hd2[1];
-    const float t3 = (((a.ex[i + 1] + fy0) + a.ez[k + 2]) - ((a.ex[i + 1] + fy0) + a.ez[k])) *
-                     (pn[c + 1] - pn[c - 1]) * a.hd2[2];
-    const float tg = (t1 + t2) + t3;
-    float val, phin;
-    if constexpr (ARITH == PHC_FD_ARITH_REF) {
-        const double dpc = (double)pc;
-        const float lap = (float)(((a.coef0 * dpc + (double)lapx) + (double)lapy) + (double)lapz);
-        const double de = (double)e;
-        const double num = (((2.0 - (double)(e * e)) + 2.0 * de) * dpc - (double)pm) + (double)(v * (ph + lap));
-        val = (float)(num / (1.0 + 2.0 * de));
-        phin = (float)((double)(ph - tg) / (1.0 + de));
-    } else {
-        const float lap = ((a.c0f * pc + lapx) + lapy) + lapz;
-        val = ((2.f - e * e + 2.f * e) * pc - pm + v * (ph + lap)) / (1.f + 2.f * e);
-        phin = (ph - tg) / (1.f + e);
-    }
-    a.phi[c] = phin;
-    if (c == a.src_off) {
-        if (a.trace) *a.trace = val;
-        if (a.inj) val = add_nc(val, mul_nc(v, a.rhs_next));
-    }
-    a.pnm1[c] = val;
-}
-
-using FdPmlFn = void (*)(FdPmlArgs);
-
-template <int R>
-static FdPmlFn pml_arith(int arith)
-{
-    return arith == PHC_FD_ARITH_REF ? fd_pml_kernel<R, PHC_FD_ARITH_REF> : fd_pml_kernel<R, PHC_FD_ARITH_FP32>;
-}
-
-static FdPmlFn fd_pick_pml(int R, int arith)
-{
-    switch (R) {
-    case 1: return pml_arith<1>(arith);
-    case 2: return pml_arith<2>(arith);
-    case 3: return pml_arith<3>(arith);
-    case 4: return pml_arith<4>(arith);
-    case 5: return pml_arith<5>(arith);
-    case 6: return pml_arith<6>(arith);
-    default: return nullptr;
-    }
-}
-
 // ------------------------------------------------------------ dispatching
 
 using FdKernelFn = void (*)(FdStepArgs);
 
 template <int R, int TY, int NJ>
-static FdKernelFn pick_arith(int arith, bool idx32)
+static FdKernelFn pick_arith(int arith, bool idx32, bool pml)
 {
+    if (pml) {
+        if (arith == PHC_FD_ARITH_REF)
+            return idx32 ? fd_step_pml_kernel<R, TY, NJ, PHC_FD_ARITH_REF, uint32_t>
+                         : fd_step_pml_kernel<R, TY, NJ, PHC_FD_ARITH_REF, int64_t>;
+        return idx32 ? fd_step_pml_kernel<R, TY, NJ, PHC_FD_ARITH_FP32, uint32_t>
+                     : fd_step_pml_kernel<R, TY, NJ, PHC_FD_ARITH_FP32, int64_t>;
+    }
     if (arith == PHC_FD_ARITH_REF)
         return idx32 ? fd_step_kernel<R, TY, NJ, PHC_FD_ARITH_REF, uint32_t>
                      : fd_step_kernel<R, TY, NJ, PHC_FD_ARITH_REF, int64_t>;
@@ -230,30 +123,30 @@ static FdKernelFn pick_arith(int arith, bool idx32)
 // Radius 4 (the 8th-order operator and order-4 SEM) gets the full set; the
 // other radii one shape.
 template <int R>
-static FdKernelFn pick_shape(int ty, int nj, int arith, bool idx32)
+static FdKernelFn pick_shape(int ty, int nj, int arith, bool idx32, bool pml)
 {
-    if (ty == 4 && nj == 2) return pick_arith<R, 4, 2>(arith, idx32);
+    if (ty == 4 && nj == 2) return pick_arith<R, 4, 2>(arith, idx32, pml);
     if constexpr (R == 4) {
-        if (ty == 4 && nj == 4) return pick_arith<R, 4, 4>(arith, idx32);
-        if (ty == 8 && nj == 1) return pick_arith<R, 8, 1>(arith, idx32);
-        if (ty == 8 && nj == 2) return pick_arith<R, 8, 2>(arith, idx32);
-        if (ty == 16 && nj == 1) return pick_arith<R, 16, 1>(arith, idx32);
-        if (ty == 16 && nj == 2) return pick_arith<R, 16, 2>(arith, idx32);
+        if (ty == 4 && nj == 4) return pick_arith<R, 4, 4>(arith, idx32, pml);
+        if (ty == 8 && nj == 1) return pick_arith<R, 8, 1>(arith, idx32, pml);
+        if (ty == 8 && nj == 2) return pick_arith<R, 8, 2>(arith, idx32, pml);
+        if (ty == 16 && nj == 1) return pick_arith<R, 16, 1>(arith, idx32, pml);
+        if (ty == 16 && nj == 2) return pick_arith<R, 16, 2>(arith, idx32, pml);
     }
     return nullptr;
 }
 
 // Kernel for half width R and block shape, arithmetic mode, 32-bit offsets
-// when every array is < 4 GiB.
-FdKernelFn fd_pick_kernel(int R, int ty, int nj, int arith, bool idx32)
+// when every array is < 4 GiB, with or without the PML slabs.
+FdKernelFn fd_pick_kernel(int R, int ty, int nj, int arith, bool idx32, bool pml)
 {
     switch (R) {
-    case 1: return pick_shape<1>(ty, nj, arith, idx32);
-    case 2: return pick_shape<2>(ty, nj, arith, idx32);
-    case 3: return pick_shape<3>(ty, nj, arith, idx32);
-    case 4: return pick_shape<4>(ty, nj, arith, idx32);
-    case 5: return pick_shape<5>(ty, nj, arith, idx32);
-    case 6: return pick_shape<6>(ty, nj, arith, idx32);
+    case 1: return pick_shape<1>(ty, nj, arith, idx32, pml);
+    case 2: return pick_shape<2>(ty, nj, arith, idx32, pml);
+    case 3: return pick_shape<3>(ty, nj, arith, idx32, pml);
+    case 4: return pick_shape<4>(ty, nj, arith, idx32, pml);
+    case 5: return pick_shape<5>(ty, nj, arith, idx32, pml);
+    case 6: return pick_shape<6>(ty, nj, arith, idx32, pml);
     default: return nullptr;
     }
 }
@@ -337,7 +230,7 @@ static phc_status fd_validate(const phc_fd_params *p)
     {
         const int ty = p->block_waves ? p->block_waves : ((p->lx == 4) ? 16 : 4);
         const int tj = p->tile_j ? p->tile_j : (p->block_waves ? ty * 2 : ((p->lx == 4) ? 16 : 8));
-        if (tj % ty != 0 || !fd_pick_kernel(p->lx, ty, tj / ty, p->arith, true))
+        if (tj % ty != 0 || !fd_pick_kernel(p->lx, ty, tj / ty, p->arith, true, false))
             return fail(PHC_ERR_INVALID, "no kernel for half width %d, tile_j %d, %d waves per block",
                         p->lx, tj, ty);
     }
@@ -402,10 +295,20 @@ static void fd_fill_args(const phc_fd *h, FdStepArgs &a, int ca, int cb)
     a.tiles_k = (h->nz + FD_TK - 1) / FD_TK;
     a.tiles_j = (h->ny + TJ - 1) / TJ;
     a.chunk_i = h->chunk_i;
-    // with a PML the stencil kernel updates the inner box only (FDTDinit.hpp:176-198)
-    a.ilo = h->pml_nd[0]; a.ihi = h->nx - h->pml_nd[0];
-    a.jlo = h->pml_nd[1]; a.jhi = h->ny - h->pml_nd[1];
-    a.klo = h->pml_nd[2]; a.khi = h->nz - h->pml_nd[2];
+    a.ilo = 0; a.ihi = h->nx;
+    a.jlo = 0; a.jhi = h->ny;
+    a.klo = 0; a.khi = h->nz;
+    if (h->pml_on) {
+        // slabs outside the inner box [x3,x4) x [y3,y4) x [z3,z4) (FDTDinit.hpp:176-198)
+        a.pml_ex = h->pml_ex.as<float>();
+        a.pml_ey = h->pml_ey.as<float>();
+        a.pml_ez = h->pml_ez.as<float>();
+        a.pml_phi = h->pml_phi.as<float>();
+        for (int d = 0; d < 3; ++d) a.pml_hd2[d] = h->pml_hd2[d];
+        a.px3 = h->pml_nd[0]; a.px4 = h->nx - h->pml_nd[0];
+        a.py3 = h->pml_nd[1]; a.py4 = h->ny - h->pml_nd[1];
+        a.pz3 = h->pml_nd[2]; a.pz4 = h->nz - h->pml_nd[2];
+    }
 }
 
 static int fd_num_blocks(const FdStepArgs &a)
@@ -430,7 +333,7 @@ static phc_status fd_materialize(phc_fd *h, int level)
 static phc_status fd_launch_step(phc_fd *h, FdStepArgs a, bool damp)
 {
     FdKernelFn fn = fd_pick_kernel(h->R, h->ty, h->nj, h->arith,
-                                   !h->index64 && h->L.total * 4 < ((int64_t)1 << 32));
+                                   !h->index64 && h->L.total * 4 < ((int64_t)1 << 32), h->pml_on != 0);
     if (!fn) return fail(PHC_ERR_INVALID, "no kernel for R=%d block_waves=%d rows/thread=%d", h->R, h->ty, h->nj);
     a.damp_prev = damp ? 1 : 0;
     const int nb = fd_num_blocks(a);
@@ -447,57 +350,14 @@ static void fd_drop_graph(phc_fd *h)
 }
 
 // One full step from levels cb (current) and ca (previous, overwritten by the
-// new level): with the learned boundary, its correction pass before and its
-// apply pass after the stencil launch.
-static phc_status fd_launch_pml(phc_fd *h, const FdStepArgs &a, int ca, int cb)
-{
-    FdPmlArgs P;
-    P.pn = h->lev[cb].as<float>();
-    P.pnm1 = h->lev[ca].as<float>();
-    P.vp = h->vp.as<float>();
-    P.phi = h->pml_phi.as<float>();
-    P.ex = h->pml_ex.as<float>();
-    P.ey = h->pml_ey.as<float>();
-    P.ez = h->pml_ez.as<float>();
-    P.L = h->L;
-    for (int l = 0; l <= FD_MAX_R; ++l) {
-        P.cx[l] = h->cx[l];
-        P.cy[l] = h->cy[l];
-        P.cz[l] = h->cz[l];
-    }
-    P.coef0 = h->coef0;
-    P.c0f = (float)h->coef0;
-    for (int d = 0; d < 3; ++d) P.hd2[d] = h->pml_hd2[d];
-    P.x3 = h->pml_nd[0]; P.x4 = h->nx - h->pml_nd[0];
-    P.y3 = h->pml_nd[1]; P.y4 = h->ny - h->pml_nd[1];
-    P.z3 = h->pml_nd[2]; P.z4 = h->nz - h->pml_nd[2];
-    P.rowsA = (P.x3 + h->nx - P.x4) * h->ny;
-    P.rowsB = (P.x4 - P.x3) * (P.y3 + h->ny - P.y4);
-    P.rowsC = (P.x4 - P.x3) * (P.z3 + h->nz - P.z4);
-    P.src_off = a.src_off;
-    P.inj = a.inj;
-    P.rhs_next = a.rhs_next;
-    P.trace = a.trace;
-    const int rows = P.rowsA + P.rowsB + P.rowsC;
-    if (rows == 0) return PHC_OK;
-    FdPmlFn fn = fd_pick_pml(h->R, h->arith);
-    if (!fn) return fail(PHC_ERR_INVALID, "no PML kernel for half width %d", h->R);
-    P.kblocks = (h->nz + 127) / 128;
-    P.nbAB = (P.rowsA + P.rowsB) * P.kblocks;
-    const int64_t nc = (int64_t)(P.x4 - P.x3) * (P.y4 - P.y3) * (P.z3 + h->nz - P.z4);
-    const int64_t nb = (int64_t)P.nbAB + (nc + 127) / 128;
-    if (nb > INT32_MAX) return fail(PHC_ERR_INVALID, "PML slabs too large for one launch");
-    hipLaunchKernelGGL(fn, dim3((unsigned)nb), dim3(128), 0, h->stream, P);
-    PHC_HIP(hipGetLastError());
-    return PHC_OK;
-}
-
+// new level): one marching launch (with the PML slabs when a PML is set);
+// with the learned boundary, its correction pass before and its apply pass
+// after the stencil launch.
 static phc_status fd_launch_full_step(phc_fd *h, const FdStepArgs &a, bool damp, int ca, int cb)
 {
     if (h->lb_on)
         PHC_HIP(lb_enqueue_correct(h->lbi, h->L, h->lev[cb].as<float>(), h->lev[ca].as<float>(),
                                    h->vp.as<float>(), h->stream));
-    if (h->pml_on) PHC_TRY(fd_launch_pml(h, a, ca, cb));
     PHC_TRY(fd_launch_step(h, a, damp));
     if (h->lb_on) PHC_HIP(lb_enqueue_apply(h->lbi, h->L, h->lev[ca].as<float>(), h->stream));
     return PHC_OK;
@@ -905,11 +765,6 @@ phc_status phc_fd_set_pml(phc_fd *h, const int *ndamp, const float *etax, const 
         return PHC_OK;
     }
     if (!etax || !etay || !etaz || !hd2) return fail(PHC_ERR_INVALID, "eta profile or hd2 is NULL");
-    {
-        const int64_t rows = (int64_t)2 * ndamp[0] * n[1] + (int64_t)(n[0] - 2 * ndamp[0]) * 2 * ndamp[1] +
-                             (int64_t)(n[0] - 2 * ndamp[0]) * 2 * ndamp[2];
-        if (rows > INT32_MAX) return fail(PHC_ERR_INVALID, "PML slabs of %lld rows exceed 2^31 - 1", (long long)rows);
-    }
     if (h->sw != 0) return fail(PHC_ERR_INVALID, "the PML replaces the sponge: sponge_width must be 0");
     if (h->lb_on) return fail(PHC_ERR_STATE, "a learned boundary is attached");
     const float *prof[3] = {etax, etay, etaz};
@@ -1012,7 +867,7 @@ static bool fd_tune_lookup(const char *path, const phc_fd *h, int *ty, int *tj, 
             hit = true;
         }
     std::fclose(f);
-    return hit && fd_pick_kernel(h->R, *ty, *tj / *ty, h->arith, true) && *ci > 0;
+    return hit && fd_pick_kernel(h->R, *ty, *tj / *ty, h->arith, true, false) && *ci > 0;
 }
 
 } // namespace phc
@@ -1032,6 +887,11 @@ extern "C" phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char
         PHC_TRY(s1.alloc(bytes));
         PHC_HIP(hipMemcpyAsync(s0.ptr, h->lev[0].ptr, bytes, hipMemcpyDeviceToDevice, h->stream));
         PHC_HIP(hipMemcpyAsync(s1.ptr, h->lev[1].ptr, bytes, hipMemcpyDeviceToDevice, h->stream));
+        DevBuf s2;  // the PML variant writes phi: its trials run on a copy
+        if (h->pml_on) {
+            PHC_TRY(s2.alloc(bytes));
+            PHC_HIP(hipMemcpyAsync(s2.ptr, h->pml_phi.ptr, bytes, hipMemcpyDeviceToDevice, h->stream));
+        }
         hipEvent_t e0, e1;
         PHC_HIP(hipEventCreate(&e0));
         PHC_HIP(hipEventCreate(&e1));
@@ -1041,7 +901,7 @@ extern "C" phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char
         int bty = h->ty, bnj = h->nj, bci = h->chunk_i;
         phc_status st = PHC_OK;
         for (auto &sh : shapes) {
-            if (!fd_pick_kernel(h->R, sh[0], sh[1] / sh[0], h->arith, true)) continue;
+            if (!fd_pick_kernel(h->R, sh[0], sh[1] / sh[0], h->arith, true, false)) continue;
             for (int c : chunks) {
                 if (c > h->nx && c != chunks[0]) continue;
                 h->ty = sh[0]; h->nj = sh[1] / sh[0]; h->chunk_i = std::min(c, h->nx);
@@ -1054,6 +914,7 @@ extern "C" phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char
                         a.pn = s1.as<float>();
                         a.pnm1 = s0.as<float>();
                         if (s & 1) { a.pn = s0.as<float>(); a.pnm1 = s1.as<float>(); }
+                        if (h->pml_on) a.pml_phi = s2.as<float>();
                         st = fd_launch_step(h, a, false);
                     }
                     if (st == PHC_OK && (hipEventRecord(e1, h->stream) != hipSuccess ||

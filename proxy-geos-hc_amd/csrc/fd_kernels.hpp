@@ -61,9 +61,15 @@ struct FdStepArgs {
     float *trace;       // if set: *trace = new value at src before injection
     int tiles_k, tiles_j, chunk_i;
     // box of points this launch updates: i in [ilo, ihi) (marched in chunks
-    // from ilo), j in [jlo, jhi), k in [klo, khi); the whole grid unless a
-    // PML takes the slabs around it
+    // from ilo), j in [jlo, jhi), k in [klo, khi)
     int ilo, ihi, jlo, jhi, klo, khi;
+    // PML (the marching kernel's PML variant): points outside the inner box
+    // [px3, px4) x [py3, py4) x [pz3, pz4) take the pml3D update with
+    // eta = pml_ex[i+1] + pml_ey[j+1] + pml_ez[k+1] (profiles over -1..n)
+    const float *pml_ex, *pml_ey, *pml_ez;
+    float *pml_phi;
+    float pml_hd2[3];
+    int px3, px4, py3, py4, pz3, pz4;
 };
 
 } // namespace phc

@@ -24,6 +24,45 @@ __device__ __forceinline__ float fd_update_ref(float pc, float pm, float v, doub
     return r * s;
 }
 
+// pml3D (FDTDkernels.hpp:36-73) at one point, every operation rounded where
+// the reference's expression rounds.  e = eta at the point; the three
+// first-difference terms use eta one step each way along their axis:
+//   tg = (d eta/dx)(dp/dx) hd2_x + (d eta/dy)(dp/dy) hd2_y + (d eta/dz)(dp/dz) hd2_z
+__device__ __forceinline__ void fd_pml_terms(float exm, float ex0, float exp_, float eym, float ey0, float eyp,
+                                             float ezm, float ez0, float ezp, float pxm, float pxp, float pym,
+                                             float pyp, float pzm, float pzp, const float *hd2, float &e, float &tg)
+{
+#pragma clang fp contract(off)
+    e = (ex0 + ey0) + ez0;
+    const float t1 = (((exp_ + ey0) + ez0) - ((exm + ey0) + ez0)) * (pxp - pxm) * hd2[0];
+    const float t2 = (((ex0 + eyp) + ez0) - ((ex0 + eym) + ez0)) * (pyp - pym) * hd2[1];
+    const float t3 = (((ex0 + ey0) + ezp) - ((ex0 + ey0) + ezm)) * (pzp - pzm) * hd2[2];
+    tg = (t1 + t2) + t3;
+}
+
+// New value and new phi of pml3D from the stencil's per-axis float sums.
+// ARITH_REF: lap formed in double and stored as float, the update in double,
+// phi in float with a double division (the reference's rounding).
+template <int ARITH>
+__device__ __forceinline__ void fd_pml_update(float pc, float pm, float v, float ph, double coef0, float c0f,
+                                              float lapx, float lapy, float lapz, float e, float tg, float &val,
+                                              float &phin)
+{
+#pragma clang fp contract(off)
+    if constexpr (ARITH == PHC_FD_ARITH_REF) {
+        const double dpc = (double)pc;
+        const float lap = (float)(((coef0 * dpc + (double)lapx) + (double)lapy) + (double)lapz);
+        const double de = (double)e;
+        const double num = (((2.0 - (double)(e * e)) + 2.0 * de) * dpc - (double)pm) + (double)(v * (ph + lap));
+        val = (float)(num / (1.0 + 2.0 * de));
+        phin = (float)((double)(ph - tg) / (1.0 + de));
+    } else {
+        const float lap = ((c0f * pc + lapx) + lapy) + lapz;
+        val = ((2.f - e * e + 2.f * e) * pc - pm + v * (ph + lap)) / (1.f + 2.f * e);
+        phin = (ph - tg) / (1.f + e);
+    }
+}
+
 // Face damping of one axis at index i of n (FDTDinit.hpp:343-395: the high
 // face loop runs last, so it wins where the faces overlap).
 __device__ __forceinline__ float fd_face_damping(const FdStepArgs &a, int i, int n)
@@ -54,7 +93,10 @@ struct FdTileShape {
 };
 
 // March of one tile column (k-tile kt, j-tile jt, i-chunk ic) of one grid.
-template <int R, int TY, int NJ, int ARITH, typename IT, class G>
+// PML: points outside the PML's inner box take the pml3D update (the
+// stencil values come from the same register ring and LDS tile), so one
+// launch covers the slabs and the inner box.
+template <int R, int TY, int NJ, int ARITH, typename IT, bool PML = false, class G>
 __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt, int jt, int ic,
                                         float (*tile)[FdTileShape<R, TY, NJ>::LH][FdTileShape<R, TY, NJ>::LW])
 {
@@ -93,6 +135,25 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
     }
     const bool inK = (k >= a.sw) && (k < L.nz - a.sw);
     const float fzv = (a.sw > 0 && kcomp) ? g.fz[k] : 1.f;
+    // PML profiles along k (per lane) and j (per row), fixed during the march
+    float ezm = 0.f, ez0 = 0.f, ezp = 0.f;
+    float eym[NJ], ey0[NJ], eyp[NJ];
+    bool pslab_jk[NJ];
+    if constexpr (PML) {
+        if (kcomp) {
+            ezm = a.pml_ez[k];
+            ez0 = a.pml_ez[k + 1];
+            ezp = a.pml_ez[k + 2];
+        }
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj) {
+            const bool jv = jcomp[jj];
+            eym[jj] = jv ? a.pml_ey[jr[jj]] : 0.f;
+            ey0[jj] = jv ? a.pml_ey[jr[jj] + 1] : 0.f;
+            eyp[jj] = jv ? a.pml_ey[jr[jj] + 2] : 0.f;
+            pslab_jk[jj] = jr[jj] < a.py3 || jr[jj] >= a.py4 || k < a.pz3 || k >= a.pz4;
+        }
+    }
     const bool khalo = tx < 2 * R;
     const int kk = (tx < R) ? (k0 - R + tx) : (k0 + FD_TK + tx - R);
     const int kcol = kk - k0 + R;
@@ -178,11 +239,52 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
             const bool inI = (i >= a.sw) && (i < L.nx - a.sw);
             const float fxv = (a.sw > 0) ? fd_face_damping(a, i, L.nx) : 1.f;
             const IT pi = (IT)i * pitch;
+            float exm = 0.f, ex0 = 0.f, exp_ = 0.f;
+            bool pslab_i = false;
+            if constexpr (PML) {
+                exm = a.pml_ex[i];
+                ex0 = a.pml_ex[i + 1];
+                exp_ = a.pml_ex[i + 2];
+                pslab_i = i < a.px3 || i >= a.px4;
+            }
 #pragma unroll
             for (int jj = 0; jj < NJ; ++jj) {
                 if (!jcomp[jj]) continue;
                 const int r = ty * NJ + jj + R;
                 const int c = tx + R;
+                if constexpr (PML) {
+                    if (pslab_i || pslab_jk[jj]) {
+                        float lapx, lapy, lapz;
+                        {
+#pragma clang fp contract(off)
+                            lapx = 0.f;
+#pragma unroll
+                            for (int l = 1; l <= R; ++l)
+                                lapx += a.cx[l] * (q[jj][(u + R + l) % Q] + q[jj][(u + R - l) % Q]);
+                            lapy = 0.f;
+#pragma unroll
+                            for (int l = 1; l <= R; ++l) lapy += a.cy[l] * (tl[r + l][c] + tl[r - l][c]);
+                            lapz = 0.f;
+#pragma unroll
+                            for (int l = 1; l <= R; ++l) lapz += a.cz[l] * (tl[r][c + l] + tl[r][c - l]);
+                        }
+                        const IT o = colq[jj] + pi;
+                        float e, tg, val, phin;
+                        fd_pml_terms(exm, ex0, exp_, eym[jj], ey0[jj], eyp[jj], ezm, ez0, ezp,
+                                     q[jj][(u + R - 1) % Q], q[jj][(u + R + 1) % Q], tl[r - 1][c], tl[r + 1][c],
+                                     tl[r][c - 1], tl[r][c + 1], a.pml_hd2, e, tg);
+                        const float vvj = vv[u & 1][jj];
+                        fd_pml_update<ARITH>(q[jj][(u + R) % Q], pm[u & 1][jj], vvj, ldo(a.pml_phi, o), a.coef0,
+                                             a.c0f, lapx, lapy, lapz, e, tg, val, phin);
+                        sto_nt(a.pml_phi, o, phin);
+                        if ((int64_t)(o / S) == g.src_off) {
+                            if (g.trace) *g.trace = val;
+                            if (g.inj) val = add_nc(val, mul_nc(vvj, g.rhs_next));
+                        }
+                        sto_nt(g.pnm1, o, val);
+                        continue;
+                    }
+                }
                 float s = 1.f;
                 if (a.sw > 0) {
                     if (inJ[jj] && inK && !inI) s = fxv;
