@@ -87,7 +87,7 @@ sem_sep_kernel(SemSepArgs a)
     const int rest = b / a.tiles_k;
     const int jt = rest % a.tiles_j;
     const int ic = rest / a.tiles_j;
-    const int k0 = kt * FD_TK, j0 = jt * TJ;
+    const int k0 = kt * FD_TK, j0 = 1 + jt * TJ;  // rows 0 and ny-1 are Dirichlet
     const int i0 = ic * a.chunk_i;
     const int i1 = min(i0 + a.chunk_i, L.nx);
     const int k = k0 + tx;
@@ -498,9 +498,10 @@ static phc_status sem_launch(phc_sem *h, int i1, int i2, int inj, float rhs_next
         a.inj_coef = h->inj_coef;
         a.rhs_next = rhs_next;
         a.trace = trace;
+        // tiles over the interior: k = 1..nz-2 (from k = 0), rows 1..ny-2
         const int TJ = h->ty * h->nj;
-        a.tiles_k = (h->L.nz + FD_TK - 1) / FD_TK;
-        a.tiles_j = (h->L.ny + TJ - 1) / TJ;
+        a.tiles_k = (std::max(h->L.nz - 1, 1) + FD_TK - 1) / FD_TK;
+        a.tiles_j = (std::max(h->L.ny - 2, 1) + TJ - 1) / TJ;
         a.chunk_i = h->chunk_i;
         // order 4: chunks start at multiples of 4 (the kernel's coefficient rows assume it)
         if (h->p == 4 && a.chunk_i < h->L.nx && a.chunk_i % 4) a.chunk_i += 4 - a.chunk_i % 4;
@@ -1026,17 +1027,20 @@ extern "C" phc_status phc_sem_autotune(phc_sem *h, int steps_per_trial, const ch
         if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
             st = fail(PHC_ERR_HIP, "event create failed");
         const int shapes[][2] = {{16, 16}, {16, 32}, {8, 16}, {4, 8}, {4, 4}};
-        const int chunks[] = {8, 16, 32, 64, 100, 128, 201, 256, 1 << 30};
+        // chunk lengths that split the march axis evenly (n chunks, multiples
+        // of 4 planes): a short last chunk would be mostly ring priming
+        std::vector<int> chunks;
+        for (int n : {1, 2, 3, 4, 5, 6, 8, 10, 12, 16}) {
+            int c = (h->L.nx + n - 1) / n;
+            c += (4 - c % 4) % 4;
+            if (std::find(chunks.begin(), chunks.end(), c) == chunks.end()) chunks.push_back(c);
+        }
         double best = 1e30;
         int bty = h->ty, bnj = h->nj, bci = h->chunk_i;
         for (auto &sh : shapes) {
             if (st != PHC_OK) break;
             if (!sem_pick_sep(h->p, sh[0], sh[1] / sh[0], true)) continue;
-            int last = -1;
-            for (int c0 : chunks) {
-                const int c = std::min(c0, h->ny);
-                if (c == last) continue;
-                last = c;
+            for (int c : chunks) {
                 h->ty = sh[0]; h->nj = sh[1] / sh[0]; h->chunk_i = c;
                 float ms = 0.f;
                 for (int rep = 0; rep < 2 && st == PHC_OK; ++rep) {

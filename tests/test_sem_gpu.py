@@ -73,6 +73,64 @@ def test_heterogeneous_model_and_receivers(gpu, oracle, kernel):
     assert np.array_equal(rec[0], src)
 
 
+# Tiles of 64 lanes along x cover k = 0..nx-2 and rows 1..nz-2 (the
+# boundary nodes are never updated); meshes whose last tile is nearly empty,
+# half full or full along x, and rows that fill the last row tile in part:
+#   order 4, ex 17: nx 69, last x tile holds 4 nodes
+#   order 4, ex 24: nx 97, 32 nodes
+#   order 4, ex 25: nx 101, 36 nodes
+#   order 2, ex 40: nx 81, 16 nodes
+TILINGS = [(17, 4, 9, 4), (24, 3, 5, 4), (25, 3, 4, 4), (40, 3, 11, 2)]
+
+
+@pytest.mark.parametrize("ex,ey,ez,order", TILINGS)
+def test_partial_tiles_match_oracle(gpu, oracle, ex, ey, ez, order):
+    banner("tiling ex=%d ey=%d ez=%d order=%d" % (ex, ey, ez, order))
+    h = 20.0
+    rng = np.random.default_rng(ex)
+    model = (1800 + 1500 * rng.random(ex * ey * ez)).astype(np.float32)
+    s, O = pair(gpu, oracle, 0, ex=ex, ey=ey, ez=ez, lx=h * ex, ly=h * ey, lz=h * ez, order=order,
+                dt=0.0004, f0=25.0, nsamples=130, model=model,
+                src=(h * (ex - 2) + 1, h * (ey // 2) + 1, h * (ez // 2) + 1))
+    nx, nz = s.nx, s.nz
+    # receivers on the source's row near the source (the source sits in the
+    # second-last element, so they lie in the last x tile and the one before)
+    sx = O.src_node % nx
+    row = O.src_node - sx
+    nodes = (row + np.clip(sx + np.array([-8, -4, -2, 2, 4, 6]), 1, nx - 2)).astype(np.int32)
+    s.set_receivers(nodes)
+    src, rec = s.run(120)
+    tr, orec = O.run(120, receivers=nodes)
+    assert rel_l2(s.current(), O.current()) < 1e-5
+    assert rel_l2(s.previous(), O.previous()) < 1e-5
+    assert rel_l2(src, tr) < 1e-5
+    for r in range(len(nodes)):
+        assert np.abs(orec[r]).max() > 1e-3 * np.abs(tr).max(), r
+        assert rel_l2(rec[r], orec[r]) < 1e-5, r
+    # boundary nodes are never written
+    f = s.current().reshape(s.ny, nz, nx)
+    assert not f[:, :, 0].any() and not f[:, :, -1].any() and not f[:, 0, :].any() and not f[:, -1, :].any()
+
+
+@pytest.mark.parametrize("ex,ey,ez,order", TILINGS[:2])
+def test_every_block_shape_gives_the_same_bits(gpu, tmp_path, ex, ey, ez, order):
+    # each node's arithmetic is the same whatever tile, block shape or march
+    # chunk covers it
+    kw = dict(ex=ex, ey=ey, ez=ez, lx=20.0 * ex, ly=20.0 * ey, lz=20.0 * ez, order=order, dt=0.0004,
+              f0=25.0, nsamples=40, src_point=(20.0 * (ex - 1) + 1, 21.0, 21.0))
+    ref = gpu.SemSolver(**kw)
+    ref.run(30)
+    for waves, tile_j, chunk in [(16, 16, 8), (16, 32, 12), (8, 16, 1 << 20), (4, 8, 20), (4, 4, 4)]:
+        banner("shape waves=%d tile_j=%d chunk=%d" % (waves, tile_j, chunk))
+        cache = tmp_path / ("s%d_%d_%d.txt" % (waves, tile_j, chunk))
+        cache.write_text("sem %d %d %d %d %d %d %d 1.0\n" % (ex, ey, ez, order, waves, tile_j, chunk))
+        s = gpu.SemSolver(**kw)
+        assert s.autotune(steps_per_trial=1, cache_path=str(cache)) == (tile_j, waves, chunk)
+        s.run(30)
+        assert np.array_equal(s.current(), ref.current())
+        assert np.array_equal(s.previous(), ref.previous())
+
+
 @pytest.mark.parametrize("order", [2, 4])
 def test_mass_and_operator(gpu, oracle, order):
     banner("mass/operator order=%d" % order)
