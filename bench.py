@@ -45,6 +45,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fd_step_r4_512_traffic.txt")
 BATCH_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fd_batch_64shots_traffic.txt")
 SEM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sem_p4_401_traffic.txt")
 SEM2D_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sem2d_p4_8001_traffic.txt")
+PML_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fd_pml_r4_512_kernel_stats.txt")
 
 
 def read_traffic(path=TRAFFIC_FILE):
@@ -241,9 +242,10 @@ def bench_sem2d(phc, steps=1000, big_ex=2000, big_order=4, big_steps=200):
 def bench_pml(phc, n=512, steps=200):
     """FD with the reference's PML (-usePML, FDTDkernels.hpp:36-104) on the
     headline grid: width (int)(4 vmin/(2.5 f0)/dx) = 24 at vmin 1500 m/s,
-    f0 10 Hz, dx 10 m; one PML launch on the slabs plus the stencil on the inner
-    box per step (HIP events around both).  Algorithmic bytes: 16 per inner
-    point, 24 per slab point (pn, pnm1, vp^2dt^2, phi read; phi, pnew written)."""
+    f0 10 Hz, dx 10 m; one marching launch per step (pml3D on the slab points,
+    inner3D on the rest; HIP events around it).  Algorithmic bytes: 16 per
+    inner point, 24 per slab point (pn, pnm1, vp^2dt^2, phi read; phi, pnew
+    written)."""
     setup = phc.FdSetup(nx=n, model=("layered", 6, 20240512), f0=10.0, cfl=0.4, nsamples=steps + 1,
                         sponge_width=0)
     sol = phc.FdSolver(setup, arith=phc.ARITH_FP32)
@@ -261,7 +263,7 @@ def bench_pml(phc, n=512, steps=200):
             "pml_width": nd, "shape": list(shape), "ms_per_step": round(kms, 5), "value": round(n ** 3 / (kms * 1e-3) / 1e6, 1),
             "unit": "Mpoint-updates/s", "finite": fin,
             "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}}
+                         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": read_traffic(PML_TRAFFIC_FILE)}}
 
 
 def bench_sem_elem(phc, steps=100):

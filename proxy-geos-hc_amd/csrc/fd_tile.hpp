@@ -177,6 +177,8 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
     float q[NJ][Q];                  // pn of plane p at slot (p - i0 + R) % Q
     float fk[NJ][F], fj[NH][F];      // halo of plane p at slot (p - i0) % F
     float pm[2][NJ], vv[2][NJ];      // pnm1 / vp of plane p in set (p - i0) % 2
+    float ph[2][NJ];                 // PML: phi of plane p (slab points) in set (p - i0) % 2
+    float ex[2][3];                  // PML: eta_x at p-1, p, p+1 (pml_ex[p..p+2]) in set (p - i0) % 2
 
     auto load_halo = [&](int slot, int p) {
         const IT po = (IT)p * pitch;
@@ -191,6 +193,17 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
         for (int jj = 0; jj < NJ; ++jj) {
             pm[set][jj] = ldo_nt(g.pnm1, colq[jj] + po);
             vv[set][jj] = ldo_nt(g.vp, colq[jj] + po);
+        }
+        if constexpr (PML) {
+            // phi only where plane p's point is in a slab; other points read
+            // element 0 (one cached line), so the load needs no branch
+            const bool sl = p < a.px3 || p >= a.px4;
+#pragma unroll
+            for (int jj = 0; jj < NJ; ++jj)
+                ph[set][jj] = ldo_nt(a.pml_phi, (sl || pslab_jk[jj]) ? colq[jj] + po : (IT)0);
+            ex[set][0] = a.pml_ex[p];
+            ex[set][1] = a.pml_ex[p + 1];
+            ex[set][2] = a.pml_ex[p + 2];
         }
     };
     auto stage = [&](int buf, int qslot, int hslot) {
@@ -242,9 +255,9 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
             float exm = 0.f, ex0 = 0.f, exp_ = 0.f;
             bool pslab_i = false;
             if constexpr (PML) {
-                exm = a.pml_ex[i];
-                ex0 = a.pml_ex[i + 1];
-                exp_ = a.pml_ex[i + 2];
+                exm = ex[u & 1][0];
+                ex0 = ex[u & 1][1];
+                exp_ = ex[u & 1][2];
                 pslab_i = i < a.px3 || i >= a.px4;
             }
 #pragma unroll
@@ -274,7 +287,7 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
                                      q[jj][(u + R - 1) % Q], q[jj][(u + R + 1) % Q], tl[r - 1][c], tl[r + 1][c],
                                      tl[r][c - 1], tl[r][c + 1], a.pml_hd2, e, tg);
                         const float vvj = vv[u & 1][jj];
-                        fd_pml_update<ARITH>(q[jj][(u + R) % Q], pm[u & 1][jj], vvj, ldo(a.pml_phi, o), a.coef0,
+                        fd_pml_update<ARITH>(q[jj][(u + R) % Q], pm[u & 1][jj], vvj, ph[u & 1][jj], a.coef0,
                                              a.c0f, lapx, lapy, lapz, e, tg, val, phin);
                         sto_nt(a.pml_phi, o, phin);
                         if ((int64_t)(o / S) == g.src_off) {
