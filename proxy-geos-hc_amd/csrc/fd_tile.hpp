@@ -28,25 +28,35 @@ __device__ __forceinline__ float fd_update_ref(float pc, float pm, float v, doub
 // the reference's expression rounds.  e = eta at the point; the three
 // first-difference terms use eta one step each way along their axis:
 //   tg = (d eta/dx)(dp/dx) hd2_x + (d eta/dy)(dp/dy) hd2_y + (d eta/dz)(dp/dz) hd2_z
+// ARITH_FP32 takes the eta differences as exp_ - exm etc. directly.
+template <int ARITH>
 __device__ __forceinline__ void fd_pml_terms(float exm, float ex0, float exp_, float eym, float ey0, float eyp,
                                              float ezm, float ez0, float ezp, float pxm, float pxp, float pym,
                                              float pyp, float pzm, float pzp, const float *hd2, float &e, float &tg)
 {
 #pragma clang fp contract(off)
     e = (ex0 + ey0) + ez0;
-    const float t1 = (((exp_ + ey0) + ez0) - ((exm + ey0) + ez0)) * (pxp - pxm) * hd2[0];
-    const float t2 = (((ex0 + eyp) + ez0) - ((ex0 + eym) + ez0)) * (pyp - pym) * hd2[1];
-    const float t3 = (((ex0 + ey0) + ezp) - ((ex0 + ey0) + ezm)) * (pzp - pzm) * hd2[2];
-    tg = (t1 + t2) + t3;
+    if constexpr (ARITH == PHC_FD_ARITH_REF) {
+        const float t1 = (((exp_ + ey0) + ez0) - ((exm + ey0) + ez0)) * (pxp - pxm) * hd2[0];
+        const float t2 = (((ex0 + eyp) + ez0) - ((ex0 + eym) + ez0)) * (pyp - pym) * hd2[1];
+        const float t3 = (((ex0 + ey0) + ezp) - ((ex0 + ey0) + ezm)) * (pzp - pzm) * hd2[2];
+        tg = (t1 + t2) + t3;
+    } else {
+        tg = (exp_ - exm) * (pxp - pxm) * hd2[0] + (eyp - eym) * (pyp - pym) * hd2[1] +
+             (ezp - ezm) * (pzp - pzm) * hd2[2];
+    }
 }
 
 // New value and new phi of pml3D from the stencil's per-axis float sums.
 // ARITH_REF: lap formed in double and stored as float, the update in double,
 // phi in float with a double division (the reference's rounding).
+// ARITH_FP32: lap32 is the whole Laplacian (the plain step's FMA chain); the
+// two divisions as products with the hardware reciprocal (1 ulp), a tenth of
+// the instructions of an IEEE division.
 template <int ARITH>
-__device__ __forceinline__ void fd_pml_update(float pc, float pm, float v, float ph, double coef0, float c0f,
-                                              float lapx, float lapy, float lapz, float e, float tg, float &val,
-                                              float &phin)
+__device__ __forceinline__ void fd_pml_update(float pc, float pm, float v, float ph, double coef0,
+                                              float lapx, float lapy, float lapz, float lap32, float e, float tg,
+                                              float &val, float &phin)
 {
 #pragma clang fp contract(off)
     if constexpr (ARITH == PHC_FD_ARITH_REF) {
@@ -57,9 +67,8 @@ __device__ __forceinline__ void fd_pml_update(float pc, float pm, float v, float
         val = (float)(num / (1.0 + 2.0 * de));
         phin = (float)((double)(ph - tg) / (1.0 + de));
     } else {
-        const float lap = ((c0f * pc + lapx) + lapy) + lapz;
-        val = ((2.f - e * e + 2.f * e) * pc - pm + v * (ph + lap)) / (1.f + 2.f * e);
-        phin = (ph - tg) / (1.f + e);
+        val = ((2.f - e * e + 2.f * e) * pc - pm + v * (ph + lap32)) * __builtin_amdgcn_rcpf(1.f + 2.f * e);
+        phin = (ph - tg) * __builtin_amdgcn_rcpf(1.f + e);
     }
 }
 
@@ -265,30 +274,39 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
                 if (!jcomp[jj]) continue;
                 const int r = ty * NJ + jj + R;
                 const int c = tx + R;
+                const float pc = q[jj][(u + R) % Q];
+                const float pmj = pm[u & 1][jj], vvj = vv[u & 1][jj];
+                // stencil sums, once per point whichever update follows:
+                // ARITH_REF per axis with the reference's rounding, ARITH_FP32
+                // one FMA chain
+                float lapx = 0.f, lapy = 0.f, lapz = 0.f, lap = 0.f;
+                if constexpr (ARITH == PHC_FD_ARITH_REF) {
+#pragma clang fp contract(off)
+#pragma unroll
+                    for (int l = 1; l <= R; ++l)
+                        lapx += a.cx[l] * (q[jj][(u + R + l) % Q] + q[jj][(u + R - l) % Q]);
+#pragma unroll
+                    for (int l = 1; l <= R; ++l) lapy += a.cy[l] * (tl[r + l][c] + tl[r - l][c]);
+#pragma unroll
+                    for (int l = 1; l <= R; ++l) lapz += a.cz[l] * (tl[r][c + l] + tl[r][c - l]);
+                } else {
+                    lap = a.c0f * pc;
+#pragma unroll
+                    for (int l = 1; l <= R; ++l) {
+                        lap = fmaf(a.cx[l], q[jj][(u + R + l) % Q] + q[jj][(u + R - l) % Q], lap);
+                        lap = fmaf(a.cy[l], tl[r + l][c] + tl[r - l][c], lap);
+                        lap = fmaf(a.cz[l], tl[r][c + l] + tl[r][c - l], lap);
+                    }
+                }
                 if constexpr (PML) {
                     if (pslab_i || pslab_jk[jj]) {
-                        float lapx, lapy, lapz;
-                        {
-#pragma clang fp contract(off)
-                            lapx = 0.f;
-#pragma unroll
-                            for (int l = 1; l <= R; ++l)
-                                lapx += a.cx[l] * (q[jj][(u + R + l) % Q] + q[jj][(u + R - l) % Q]);
-                            lapy = 0.f;
-#pragma unroll
-                            for (int l = 1; l <= R; ++l) lapy += a.cy[l] * (tl[r + l][c] + tl[r - l][c]);
-                            lapz = 0.f;
-#pragma unroll
-                            for (int l = 1; l <= R; ++l) lapz += a.cz[l] * (tl[r][c + l] + tl[r][c - l]);
-                        }
                         const IT o = colq[jj] + pi;
                         float e, tg, val, phin;
-                        fd_pml_terms(exm, ex0, exp_, eym[jj], ey0[jj], eyp[jj], ezm, ez0, ezp,
-                                     q[jj][(u + R - 1) % Q], q[jj][(u + R + 1) % Q], tl[r - 1][c], tl[r + 1][c],
-                                     tl[r][c - 1], tl[r][c + 1], a.pml_hd2, e, tg);
-                        const float vvj = vv[u & 1][jj];
-                        fd_pml_update<ARITH>(q[jj][(u + R) % Q], pm[u & 1][jj], vvj, ph[u & 1][jj], a.coef0,
-                                             a.c0f, lapx, lapy, lapz, e, tg, val, phin);
+                        fd_pml_terms<ARITH>(exm, ex0, exp_, eym[jj], ey0[jj], eyp[jj], ezm, ez0, ezp,
+                                            q[jj][(u + R - 1) % Q], q[jj][(u + R + 1) % Q], tl[r - 1][c],
+                                            tl[r + 1][c], tl[r][c - 1], tl[r][c + 1], a.pml_hd2, e, tg);
+                        fd_pml_update<ARITH>(pc, pmj, vvj, ph[u & 1][jj], a.coef0, lapx, lapy, lapz, lap, e, tg,
+                                             val, phin);
                         sto_nt(a.pml_phi, o, phin);
                         if ((int64_t)(o / S) == g.src_off) {
                             if (g.trace) *g.trace = val;
@@ -304,35 +322,12 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
                     if (inK && inI && !inJ[jj]) s = fyv[jj];
                     if (inJ[jj] && inI && !inK) s = fzv;
                 }
-                const float pc = q[jj][(u + R) % Q];
-                const float pmj = pm[u & 1][jj], vvj = vv[u & 1][jj];
                 float val;
                 if constexpr (ARITH == PHC_FD_ARITH_REF) {
-                    float lapx, lapy, lapz;
-                    {
-#pragma clang fp contract(off)
-                        lapx = 0.f;
-#pragma unroll
-                        for (int l = 1; l <= R; ++l)
-                            lapx += a.cx[l] * (q[jj][(u + R + l) % Q] + q[jj][(u + R - l) % Q]);
-                        lapy = 0.f;
-#pragma unroll
-                        for (int l = 1; l <= R; ++l) lapy += a.cy[l] * (tl[r + l][c] + tl[r - l][c]);
-                        lapz = 0.f;
-#pragma unroll
-                        for (int l = 1; l <= R; ++l) lapz += a.cz[l] * (tl[r][c + l] + tl[r][c - l]);
-                    }
                     // deferred damping of the previous level (x1 is exact)
                     const float pmv = mul_nc(pmj, a.damp_prev ? s : 1.f);
                     val = fd_update_ref(pc, pmv, vvj, a.coef0, lapx, lapy, lapz, s);
                 } else {
-                    float lap = a.c0f * pc;
-#pragma unroll
-                    for (int l = 1; l <= R; ++l) {
-                        lap = fmaf(a.cx[l], q[jj][(u + R + l) % Q] + q[jj][(u + R - l) % Q], lap);
-                        lap = fmaf(a.cy[l], tl[r + l][c] + tl[r - l][c], lap);
-                        lap = fmaf(a.cz[l], tl[r][c + l] + tl[r][c - l], lap);
-                    }
                     const float pmv = pmj * (a.damp_prev ? s : 1.f);
                     val = s * fmaf(vvj, lap, 2.f * pc - pmv);
                 }
