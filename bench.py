@@ -266,11 +266,13 @@ def bench_pml(phc, n=512, steps=200):
                          "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": read_traffic(PML_TRAFFIC_FILE)}}
 
 
-def bench_sem_elem(phc, steps=100):
+def bench_sem_elem(phc, steps=100, steps_p4=20):
     """The reference's CPU-runnable SEM case (50^3 elements of order 2, 101^3
     nodes, c = 1500, f0 = 5 Hz, dt = 1e-3) through the three SEM step kernels:
     separable, element with atomic scatter, element with the 8-colour
-    scatter."""
+    scatter; then the same three on 100^3 elements of order 4 (401^3 nodes,
+    dt = 2e-4, f0 = 8 Hz), where the element path reads 24 B of geometry
+    factors per quadrature point."""
     out = {"workload": "SEM 50^3 elements of order 2 (101^3 nodes), %d steps, FP32" % steps}
     for name, kern in (("separable", 0), ("element_atomic", 1), ("element_colour", 2)):
         s = phc.SemSolver(ex=50, lx=2000.0, order=2, dt=1e-3, f0=5.0, nsamples=steps + 1, kernel=kern)
@@ -282,6 +284,19 @@ def bench_sem_elem(phc, steps=100):
         s.close()
         out[name] = {"ms_per_step": round(ms, 5), "value": round(101 ** 3 / (ms * 1e-3) / 1e6, 1),
                      "unit": "Mnode-updates/s"}
+    p4 = {"workload": "SEM 100^3 elements of order 4 (401^3 nodes), %d steps, FP32" % steps_p4}
+    for name, kern in (("separable", 0), ("element_atomic", 1), ("element_colour", 2)):
+        s = phc.SemSolver(ex=100, lx=2000.0, order=4, dt=2e-4, f0=8.0, nsamples=steps_p4 + 6, kernel=kern)
+        if kern == 0:
+            s.autotune(steps_per_trial=5)
+        s.set_timing(True)
+        s.run(5)
+        s.run(steps_p4)
+        ms = s.last_kernel_ms()
+        s.close()
+        p4[name] = {"ms_per_step": round(ms, 5), "value": round(401 ** 3 / (ms * 1e-3) / 1e6, 1),
+                    "unit": "Mnode-updates/s"}
+    out["order4"] = p4
     return out
 
 
