@@ -56,6 +56,10 @@ struct SemSepArgs {
     float rhs_next;
     float *trace;
     int tiles_k, tiles_j, chunk_i;
+    // per-XCD schedule (null: one block per (tile, chunk of chunk_i planes)
+    // in XCD block order): hardware block b marches tile column sched[b].x
+    // over planes [sched[b].y, sched[b].z); an empty range does nothing
+    const int4 *sched;
 };
 
 // Separable SEM step.  Same march as fd_step_kernel (register rings unrolled
@@ -82,14 +86,22 @@ sem_sep_kernel(SemSepArgs a)
 
     const FdLayout &L = a.L;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int kt = b % a.tiles_k;
-    const int rest = b / a.tiles_k;
-    const int jt = rest % a.tiles_j;
-    const int ic = rest / a.tiles_j;
+    int w, i0, i1;
+    if (a.sched) {
+        const int4 e = a.sched[blockIdx.x];
+        w = e.x;
+        i0 = e.y;
+        i1 = e.z;
+        if (i0 >= i1) return;  // padding of a shorter XCD list (whole block)
+    } else {
+        const int b = xcd_block(blockIdx.x, gridDim.x);
+        const int ic = b / (a.tiles_k * a.tiles_j);
+        w = b - ic * (a.tiles_k * a.tiles_j);
+        i0 = ic * a.chunk_i;
+        i1 = min(i0 + a.chunk_i, L.nx);
+    }
+    const int kt = w % a.tiles_k, jt = w / a.tiles_k;
     const int k0 = kt * FD_TK, j0 = 1 + jt * TJ;  // rows 0 and ny-1 are Dirichlet
-    const int i0 = ic * a.chunk_i;
-    const int i1 = min(i0 + a.chunk_i, L.nx);
     const int k = k0 + tx;
     const IT pitch = (IT)L.pitch_j * S;
 
@@ -414,6 +426,10 @@ struct phc_sem {
     // separable path
     FdLayout L;
     DevBuf lev[2], qf;
+    // chunk_i < 0: the per-XCD schedule, rebuilt when the block shape changes
+    DevBuf sched;
+    int sched_nb = 0, sched_tj = -1;
+    int cus = 256;
     float cx[SEM_MAX_P][2 * SEM_MAX_P + 1], cy[SEM_MAX_P][2 * SEM_MAX_P + 1],
         cz[SEM_MAX_P][2 * SEM_MAX_P + 1];
     int ty = 16, nj = 1, chunk_i = 0;
@@ -478,6 +494,52 @@ static phc_status sem_get(phc_sem *h, const DevBuf &src, float *host)
     return PHC_OK;
 }
 
+// Per-XCD schedule of the separable kernel (chunk_i < 0).  The hardware
+// deals block b to XCD b % 8, and each XCD starts its blocks in order as its
+// CUs free up, one 16-wave block per CU at a time.  XCD x takes plane range x
+// of 8 (cut between 4-plane groups) over every tile column, so the tiles of
+// one L2 march the same planes.  The columns that would leave a last round
+// part-empty are cut into shorter pieces that fill one round instead.  The
+// lists are padded to one length with empty ranges; table entry b belongs to
+// hardware block b.  Coverage does not depend on where a block runs.
+static phc_status sem_prepare_sched(phc_sem *h)
+{
+    const int TJ = h->ty * h->nj;
+    if (h->kernel != PHC_SEM_SEPARABLE || h->chunk_i >= 0 || h->sched_tj == TJ) return PHC_OK;
+    const int nx = h->L.nx;
+    const int ncol = ((std::max(h->L.nz - 1, 1) + FD_TK - 1) / FD_TK) * ((std::max(h->L.ny - 2, 1) + TJ - 1) / TJ);
+    const int cpx = std::max(h->cus / 8, 1);  // CUs per XCD
+    std::vector<std::vector<int4>> lists(8);
+    size_t longest = 0;
+    const int G = (nx + 3) / 4;  // 4-plane groups of the march axis, dealt evenly over the XCDs
+    for (int x = 0; x < 8; ++x) {
+        const int a = 4 * (G * x / 8), b = std::min(4 * (G * (x + 1) / 8), nx);
+        auto &li = lists[x];
+        if (a < b) {
+            const int rem = ncol % cpx, full = ncol - rem;
+            for (int w = 0; w < full; ++w) li.push_back(make_int4(w, a, b, 0));
+            const int g = (b - a + 3) / 4;  // 4-plane groups of the range
+            const int pieces = rem ? std::max(1, std::min(cpx / rem, g)) : 0;
+            for (int w = full; w < ncol; ++w)
+                for (int q = 0; q < pieces; ++q) {
+                    const int g0 = g * q / pieces, g1 = g * (q + 1) / pieces;
+                    li.push_back(make_int4(w, a + 4 * g0, std::min(a + 4 * g1, b), 0));
+                }
+        }
+        longest = std::max(longest, li.size());
+    }
+    std::vector<int4> tab(8 * longest, make_int4(0, 0, 0, 0));
+    for (int x = 0; x < 8; ++x)
+        for (size_t l = 0; l < lists[x].size(); ++l) tab[8 * l + x] = lists[x][l];
+    PHC_HIP(hipStreamSynchronize(h->stream));  // queued launches may still read the old table
+    sem_drop_graph(h);
+    PHC_TRY(h->sched.alloc(sizeof(int4) * tab.size()));
+    PHC_HIP(hipMemcpy(h->sched.ptr, tab.data(), sizeof(int4) * tab.size(), hipMemcpyHostToDevice));
+    h->sched_nb = (int)tab.size();
+    h->sched_tj = TJ;
+    return PHC_OK;
+}
+
 // One step: level i1 <- update(level i1, level i2); optional fused injection
 // of the next sample and source trace.
 static phc_status sem_launch(phc_sem *h, int i1, int i2, int inj, float rhs_next, float *trace,
@@ -502,10 +564,19 @@ static phc_status sem_launch(phc_sem *h, int i1, int i2, int inj, float rhs_next
         const int TJ = h->ty * h->nj;
         a.tiles_k = (std::max(h->L.nz - 1, 1) + FD_TK - 1) / FD_TK;
         a.tiles_j = (std::max(h->L.ny - 2, 1) + TJ - 1) / TJ;
-        a.chunk_i = h->chunk_i;
-        // order 4: chunks start at multiples of 4 (the kernel's coefficient rows assume it)
-        if (h->p == 4 && a.chunk_i < h->L.nx && a.chunk_i % 4) a.chunk_i += 4 - a.chunk_i % 4;
-        const int nb = a.tiles_k * a.tiles_j * ((h->L.nx + a.chunk_i - 1) / a.chunk_i);
+        int nb;
+        if (h->chunk_i < 0) {
+            // the table is built outside graph capture (phc_sem_run prepares it first)
+            PHC_TRY(sem_prepare_sched(h));
+            a.sched = h->sched.as<int4>();
+            a.chunk_i = 0;
+            nb = h->sched_nb;
+        } else {
+            a.chunk_i = h->chunk_i;
+            // order 4: chunks start at multiples of 4 (the kernel's coefficient rows assume it)
+            if (h->p == 4 && a.chunk_i < h->L.nx && a.chunk_i % 4) a.chunk_i += 4 - a.chunk_i % 4;
+            nb = a.tiles_k * a.tiles_j * ((h->L.nx + a.chunk_i - 1) / a.chunk_i);
+        }
         SemSepFn fn = sem_pick_sep(h->p, h->ty, h->nj, h->L.total * 4 < ((int64_t)1 << 32));
         if (!fn) return fail(PHC_ERR_INVALID, "no separable kernel for p=%d", h->p);
         if (e0) PHC_HIP(hipEventRecord(e0, h->stream));
@@ -696,9 +767,17 @@ phc_status phc_sem_create(const phc_sem_params *p, phc_sem **out)
         h->ty = p->block_waves ? p->block_waves : 16;
         h->nj = p->tile_j ? p->tile_j / h->ty : 1;
         h->chunk_i = p->chunk_i ? p->chunk_i : h->ny;
+        if (h->chunk_i < -1)
+            return bail(fail(PHC_ERR_INVALID, "chunk_i %d: planes per chunk, or -1 for the per-XCD schedule",
+                             h->chunk_i));
         if (!sem_pick_sep(P, h->ty, h->nj, true))
             return bail(fail(PHC_ERR_INVALID, "no separable kernel for p=%d with %d waves x %d rows",
                              P, h->ty, h->nj));
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+            prop.multiProcessorCount > 0)
+            h->cus = prop.multiProcessorCount;
     } else {
         const size_t bytes = sizeof(float) * (size_t)h->nn;
         if ((st = h->lev[0].alloc(bytes)) || (st = h->lev[1].alloc(bytes)) || (st = h->y.alloc(bytes)) ||
@@ -810,6 +889,7 @@ phc_status phc_sem_run(phc_sem *h, int it0, int nsteps, int *i1p, int *i2p)
     if (it0 < 0 || it0 + nsteps > h->nsamples)
         return fail(PHC_ERR_INVALID, "steps %d..%d outside the %d source samples", it0,
                     it0 + nsteps - 1, h->nsamples);
+    PHC_TRY(sem_prepare_sched(h));  // before any capture
     if (h->trace.bytes < sizeof(float) * nsteps) {
         PHC_HIP(hipStreamSynchronize(h->stream));
         sem_drop_graph(h);
@@ -1006,7 +1086,7 @@ extern "C" phc_status phc_sem_autotune(phc_sem *h, int steps_per_trial, const ch
             double ms;
             while (std::fscanf(f, "%7s %d %d %d %d %d %d %d %lf", tag, &ex, &ey, &ez, &pp, &a, &b, &c, &ms) == 9)
                 if (!std::strcmp(tag, "sem") && ex == h->ex && ey == h->ey && ez == h->ez && pp == h->p &&
-                    a > 0 && b % a == 0 && c > 0 && sem_pick_sep(h->p, a, b / a, true)) {
+                    a > 0 && b % a == 0 && (c > 0 || c == -1) && sem_pick_sep(h->p, a, b / a, true)) {
                     h->ty = a; h->nj = b / a; h->chunk_i = c;
                     hit = true;
                 }
@@ -1035,6 +1115,7 @@ extern "C" phc_status phc_sem_autotune(phc_sem *h, int steps_per_trial, const ch
             c += (4 - c % 4) % 4;
             if (std::find(chunks.begin(), chunks.end(), c) == chunks.end()) chunks.push_back(c);
         }
+        chunks.push_back(-1);  // the per-XCD schedule
         double best = 1e30;
         int bty = h->ty, bnj = h->nj, bci = h->chunk_i;
         for (auto &sh : shapes) {

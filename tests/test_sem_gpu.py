@@ -120,7 +120,9 @@ def test_every_block_shape_gives_the_same_bits(gpu, tmp_path, ex, ey, ez, order)
               f0=25.0, nsamples=40, src_point=(20.0 * (ex - 1) + 1, 21.0, 21.0))
     ref = gpu.SemSolver(**kw)
     ref.run(30)
-    for waves, tile_j, chunk in [(16, 16, 8), (16, 32, 12), (8, 16, 1 << 20), (4, 8, 20), (4, 4, 4)]:
+    # chunk -1: the per-XCD schedule of plane ranges
+    for waves, tile_j, chunk in [(16, 16, 8), (16, 32, 12), (8, 16, 1 << 20), (4, 8, 20), (4, 4, 4),
+                                 (16, 16, -1), (4, 8, -1)]:
         banner("shape waves=%d tile_j=%d chunk=%d" % (waves, tile_j, chunk))
         cache = tmp_path / ("s%d_%d_%d.txt" % (waves, tile_j, chunk))
         cache.write_text("sem %d %d %d %d %d %d %d 1.0\n" % (ex, ey, ez, order, waves, tile_j, chunk))
@@ -129,6 +131,27 @@ def test_every_block_shape_gives_the_same_bits(gpu, tmp_path, ex, ey, ez, order)
         s.run(30)
         assert np.array_equal(s.current(), ref.current())
         assert np.array_equal(s.previous(), ref.previous())
+
+
+def test_xcd_schedule_rounds_and_pieces_bit_exact(gpu):
+    # 81 x 61 x 161 nodes, order 2.  With 4 x 4 tiles there are 2 x 40 tile
+    # columns: on 32 CUs per XCD, 64 whole-range blocks per XCD and 16 columns
+    # cut into 2 pieces; with 16 x 16 tiles 20 columns, each cut into pieces
+    kw = dict(ex=40, ey=30, ez=80, lx=800.0, ly=600.0, lz=1600.0, order=2, dt=0.0008, f0=20.0, nsamples=30,
+              src_point=(401.0, 301.0, 801.0))
+    ref = gpu.SemSolver(block_waves=16, tile_j=16, chunk_i=1 << 20, **kw)
+    ref.run(25)
+    assert np.abs(ref.current()).max() > 0
+    for waves, tile_j in [(4, 4), (16, 16), (8, 16)]:
+        banner("per-XCD schedule waves=%d tile_j=%d" % (waves, tile_j))
+        s = gpu.SemSolver(block_waves=waves, tile_j=tile_j, chunk_i=-1, **kw)
+        s.run(25)
+        assert np.array_equal(s.current(), ref.current())
+        assert np.array_equal(s.previous(), ref.previous())
+        s.compute_one_step(25, s.i1, s.i2)  # the per-step surface uses the same table
+        s.close()
+    with pytest.raises(RuntimeError, match="chunk_i"):
+        gpu.SemSolver(chunk_i=-2, **kw)
 
 
 @pytest.mark.parametrize("order", [2, 4])
