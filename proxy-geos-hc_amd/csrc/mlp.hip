@@ -85,8 +85,12 @@ struct LbGridView {
     int n[3];  // region size
 };
 
-__device__ inline float lb_inputs(const LbGridView &G, const LbFaceGeom &g, int d, int u, int v, float h, float c1,
-                                  float x[LB_NIN], int64_t *center)
+// The loads of lb_inputs: raw pn / pnm1 along the normal (x[0..17]) and vp at
+// the point (x[18]); every offset is read (beyond the region's edge the grid
+// still holds values: its halo, or the wide grid around the region), so the
+// loads are not inside lane-divergent branches.
+__device__ inline void lb_load_raw(const LbGridView &G, const LbFaceGeom &g, int d, int u, int v,
+                                   float x[LB_NIN], int64_t *center)
 {
     int c[3];
     lb_point(G.n, g, d, u, v, c);
@@ -97,16 +101,19 @@ __device__ inline float lb_inputs(const LbGridView &G, const LbFaceGeom &g, int 
     else step = 1;
     const int64_t base = fd_off(G.L, G.o[0] + c[0], G.o[1] + c[1], G.o[2] + c[2]);
     *center = base;
-    float s = 0.f;
-    // every offset is read (beyond the region's edge the grid still holds
-    // values: its halo, or the wide grid around the region), then masked, so
-    // the loads are not inside lane-divergent branches
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
         const int64_t o = base + (int64_t)(m - 4) * dir * step;
         x[m] = G.pn[o];
         x[9 + m] = G.pnm1[o];
     }
+    x[18] = G.vp[base];
+}
+
+// The arithmetic of lb_inputs on raw values from lb_load_raw (in place).
+__device__ inline float lb_finish_inputs(int d, float h, float c1, float x[LB_NIN])
+{
+    float s = 0.f;
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
         const bool inside = 4 - m <= d;  // outward offset 4-m within the region
@@ -119,8 +126,15 @@ __device__ inline float lb_inputs(const LbGridView &G, const LbFaceGeom &g, int 
     const float inv = s > LB_S_TINY ? 1.f / s : 0.f;
 #pragma unroll
     for (int m = 0; m < 18; ++m) x[m] *= inv;
-    x[18] = sqrtf(G.vp[base] * c1);
+    x[18] = sqrtf(x[18] * c1);
     return s;
+}
+
+__device__ inline float lb_inputs(const LbGridView &G, const LbFaceGeom &g, int d, int u, int v, float h, float c1,
+                                  float x[LB_NIN], int64_t *center)
+{
+    lb_load_raw(G, g, d, u, v, x, center);
+    return lb_finish_inputs(d, h, c1, x);
 }
 
 // ------------------------------------------------------------ collection
@@ -581,6 +595,15 @@ __device__ inline void lb_lane_inputs(const float x[LB_NIN], float xk[12])
 
 constexpr int LB_GPW = 4;  // groups of 32 layer points per wavefront
 
+// Groups of 32 lanes of the correction pass on one face: 32 consecutive
+// compact indices on the i- and j-faces; on the k-faces 8 consecutive j of
+// one i times the LB_W depths.
+static_assert(LB_W == 4, "k-face groups are 8 rows of LB_W = 4 depths");
+__host__ __device__ inline int64_t lb_correct_groups(const LbFaceGeom &g, int64_t cnt)
+{
+    return g.axis == 2 ? (int64_t)g.nu * ((g.nv + 7) / 8) : (cnt + 31) / 32;
+}
+
 struct LbCorrectArgs {
     LbGridView G;
     const uint16_t *w;
@@ -593,7 +616,7 @@ struct LbCorrectArgs {
 // Correction pass: corr[face][d][u][v] = vp2 * c1 * s * y for every layer
 // point.  A block works on one face: its biases b1 go to LDS, each wavefront
 // keeps the face's other weights in registers and evaluates LB_GPW groups of
-// 32 consecutive points.
+// 32 points (lb_correct_groups).
 __global__ void __launch_bounds__(256) lb_correct_kernel(LbCorrectArgs A)
 {
     __shared__ float b1s[LB_W * LB_H];
@@ -606,27 +629,50 @@ __global__ void __launch_bounds__(256) lb_correct_kernel(LbCorrectArgs A)
     lb_load_wave_weights(w, W);
     const LbFaceGeom g = lb_face(A.G.n, f);
     const int64_t cnt = A.off[f + 1] - A.off[f];
+    const int64_t ngroups = lb_correct_groups(g, cnt);
     const int64_t g0 = ((int64_t)(blockIdx.x - A.blk0[f]) * 4 + (threadIdx.x >> 6)) * LB_GPW;
+    const int p = threadIdx.x & 31;  // this lane's point of the group (both half-waves)
+    const int nvb = (g.nv + 7) >> 3;
     for (int gi = 0; gi < LB_GPW; ++gi) {
-        const int64_t base = (g0 + gi) * 32;
-        if (base >= cnt) break;  // uniform per wavefront
-        const int64_t idx = base + (threadIdx.x & 31);
-        const int ci = (int)(idx < cnt ? idx : cnt - 1);  // layers hold < 2^31 points
-        // (d, u, v) of the group's first point, then this lane's offset with
-        // carries (nv >= 16, so at most two)
-        const int b0 = (int)base;
-        int v = b0 % g.nv, u = (b0 / g.nv) % g.nu, d = b0 / (g.nv * g.nu);
-        v += ci - b0;
-        while (v >= g.nv) {
-            v -= g.nv;
-            if (++u == g.nu) { u = 0; ++d; }
+        const int64_t gidx = g0 + gi;
+        if (gidx >= ngroups) break;  // uniform per wavefront
+        int d, u, v;
+        bool ok;
+        if (g.axis == 2) {
+            // k-faces: 8 consecutive j (v) x the 4 depths, so the lanes of a
+            // row read its normal stencils from one or two 64-byte segments
+            u = (int)(gidx / nvb);
+            v = (int)(gidx - (int64_t)u * nvb) * 8 + (p >> 2);
+            d = p & 3;
+            ok = v < g.nv;
+            if (!ok) v = g.nv - 1;
+        } else {
+            // i- and j-faces: 32 consecutive compact indices, v (= k) fastest;
+            // the group's first point, then this lane's offset with carries
+            // (nv >= 16, so at most two)
+            const int64_t base = gidx * 32;
+            const int64_t idx = base + p;
+            ok = idx < cnt;
+            const int ci = (int)(ok ? idx : cnt - 1);  // layers hold < 2^31 points
+            const int b0 = (int)base;
+            v = b0 % g.nv;
+            u = (b0 / g.nv) % g.nu;
+            d = b0 / (g.nv * g.nu);
+            v += ci - b0;
+            while (v >= g.nv) {
+                v -= g.nv;
+                if (++u == g.nu) { u = 0; ++d; }
+            }
         }
         float x[LB_NIN], xk[12];
         int64_t center;
-        const float s = lb_inputs(A.G, g, d, u, v, A.h, A.c1, x, &center);
+        lb_load_raw(A.G, g, d, u, v, x, &center);
+        const float vraw = x[18];
+        const float s = lb_finish_inputs(d, A.h, A.c1, x);
         lb_lane_inputs(x, xk);
         const float y = lb_mlp_wave(W, b1s, xk, d);
-        if ((threadIdx.x & 32) == 0 && idx < cnt) A.corr[A.off[f] + idx] = A.G.vp[center] * A.c1 * s * y;
+        if ((threadIdx.x & 32) == 0 && ok)
+            A.corr[A.off[f] + ((int64_t)d * g.nu + u) * g.nv + v] = vraw * A.c1 * s * y;
     }
 }
 
@@ -654,13 +700,15 @@ __global__ void __launch_bounds__(256) lb_infer_kernel(const uint16_t *w, const 
 // corrections of every face whose layer holds the point.  The union is walked
 // as rows (blockIdx.y, decoded once per block): 2*LB_W*ny rows along k for
 // the i-layers, 2*LB_W*(nx-2*LB_W) rows along k for the j-layers (i
-// interior), (nx-2*LB_W)*2*LB_W rows along j for the k-layers (i, j interior).
+// interior); a second launch covers the k-layers (i, j interior) with one row
+// per i and 2*LB_W lanes per j, so lanes run along k through both layers.
 struct LbApplyArgs {
     float *p;
     FdLayout L;
     const float *corr;
     int64_t off[LB_FACES + 1];
-    int rowsA, rowsB, rowsC;
+    int rowsA, rowsB;
+    int kface;          // 1: the k-layers, i and j interior (the second launch)
     float ldamp[LB_W];  // damping factor of the new value per layer depth (1: none)
 };
 
@@ -670,26 +718,25 @@ __global__ void __launch_bounds__(256) lb_apply_kernel(LbApplyArgs A)
     const int row = blockIdx.y;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int i, j, k;
-    if (row < A.rowsA) {  // i in layers, all j, k along the row
+    if (A.kface) {  // row = interior i; 2 LB_W lanes per j run along k through both k-layers
+        const int q = t % (2 * LB_W);
+        i = LB_W + row;
+        j = LB_W + t / (2 * LB_W);
+        k = q < LB_W ? q : nz - 2 * LB_W + q;
+        if (j >= ny - LB_W) return;
+    } else if (row < A.rowsA) {  // i in layers, all j, k along the row
         const int q = row / ny;
         j = row - q * ny;
         i = q < LB_W ? q : nx - 2 * LB_W + q;
         k = t;
         if (k >= nz) return;
-    } else if (row < A.rowsA + A.rowsB) {  // j in layers, i interior, k along the row
+    } else {  // j in layers, i interior, k along the row
         const int r = row - A.rowsA;
         const int q = r % (2 * LB_W);
         i = LB_W + r / (2 * LB_W);
         j = q < LB_W ? q : ny - 2 * LB_W + q;
         k = t;
         if (k >= nz) return;
-    } else {  // k in layers, i and j interior, j along the row
-        const int r = row - A.rowsA - A.rowsB;
-        const int q = r % (2 * LB_W);
-        i = LB_W + r / (2 * LB_W);
-        k = q < LB_W ? q : nz - 2 * LB_W + q;
-        j = LB_W + t;
-        if (j >= ny - LB_W) return;
     }
     const int n[3] = {nx, ny, nz};
     const int c[3] = {i, j, k};
@@ -729,8 +776,11 @@ hipError_t lb_enqueue_correct(const LbInfer &lb, const FdLayout &L, const float 
     A.c1 = lb.c1;
     A.blk0[0] = 0;
     for (int f = 0; f <= LB_FACES; ++f) A.off[f] = lb.corr_off[f];
-    const int64_t per_block = 4 * LB_GPW * 32;  // points per block
-    for (int f = 0; f < LB_FACES; ++f) A.blk0[f + 1] = A.blk0[f] + (A.off[f + 1] - A.off[f] + per_block - 1) / per_block;
+    const int64_t per_block = 4 * LB_GPW;  // groups per block
+    const int nn[3] = {L.nx, L.ny, L.nz};
+    for (int f = 0; f < LB_FACES; ++f)
+        A.blk0[f + 1] = A.blk0[f] + (lb_correct_groups(lb_face(nn, f), A.off[f + 1] - A.off[f]) + per_block - 1) /
+                                        per_block;
     hipLaunchKernelGGL(lb_correct_kernel, dim3((unsigned)A.blk0[LB_FACES]), dim3(256), 0, stream, A);
     return hipGetLastError();
 }
@@ -745,11 +795,17 @@ hipError_t lb_enqueue_apply(const LbInfer &lb, const FdLayout &L, float *pnew, h
     for (int d = 0; d < LB_W; ++d) A.ldamp[d] = lb.ldamp[d];
     A.rowsA = 2 * LB_W * L.ny;
     A.rowsB = 2 * LB_W * (L.nx - 2 * LB_W);
-    A.rowsC = (L.nx - 2 * LB_W) * 2 * LB_W;
-    const int len = std::max(L.nz, L.ny - 2 * LB_W);
-    hipLaunchKernelGGL(lb_apply_kernel, dim3((len + 255) / 256, A.rowsA + A.rowsB + A.rowsC), dim3(256), 0, stream,
-                       A);
-    return hipGetLastError();
+    A.kface = 0;
+    hipLaunchKernelGGL(lb_apply_kernel, dim3((L.nz + 255) / 256, A.rowsA + A.rowsB), dim3(256), 0, stream, A);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int klen = 2 * LB_W * (L.ny - 2 * LB_W);  // lanes per interior i
+    if (L.nx > 2 * LB_W && klen > 0) {
+        A.kface = 1;
+        hipLaunchKernelGGL(lb_apply_kernel, dim3((klen + 255) / 256, L.nx - 2 * LB_W), dim3(256), 0, stream, A);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 } // namespace phc
