@@ -305,6 +305,41 @@ def bench_sem_elem(phc, steps=100, steps_p4=20):
 LB_TRAIN_NOISE = 0.02
 
 
+def bench_learned_step(phc, n=256, steps=300):
+    """Time per step of the learned layer's grid (n + 8)^3 (correction pass,
+    step, apply pass; an untrained layer costs the same) against the 40-point
+    sponge's grid (n + 80)^3, graph-replayed, with the 28 receivers of the
+    learned workload.  No training and no FP64 reference run."""
+    import learned
+    lb = phc.LearnedBoundary(seed=11)
+    rec = learned.receivers_in_region(n)
+    dt = learned.fd_dt()
+    inter = np.array([15, 34, 78, 121, 215])
+    out = {"workload": "FD %d^3 region, learned 4-point layer vs 40-point sponge, %d steps" % (n, steps)}
+    for name, pad, sw, use_lb in (("learned", 4, 0, True), ("sponge40", 40, 40, False)):
+        N = n + 2 * pad
+        vp = learned.layered_vp2dt2((N, N, N), pad, inter, np.linspace(1500.0, 4500.0, 6), dt)
+        s = phc.FdSetup(N, N, N, L=4, dx=learned.DX, xs=n // 2 + pad, ys=n // 2 + pad, zs=n // 2 + pad,
+                        f0=10.0, cfl=learned.CFL, sponge_width=sw, model=vp, nsamples=steps + 1)
+        sol = phc.FdSolver(s, arith=phc.ARITH_FP32, init_value=0.0)
+        if use_lb:
+            sol.set_learned_boundary(lb, halo=0.0)
+        shape = sol.autotune(steps_per_trial=5)
+        sol.set_receivers(rec + pad)
+        sol.run(0, steps)  # capture + first replay
+        sol.synchronize()
+        sol.i1, sol.i2 = 0, 1
+        t0 = time.perf_counter()
+        sol.run(0, steps)
+        sol.synchronize()
+        out[name] = {"grid": N, "shape": list(shape),
+                     "ms_per_step": round((time.perf_counter() - t0) / steps * 1e3, 4)}
+        sol.close()
+    lb.close()
+    out["speedup_learned_vs_sponge40"] = round(out["sponge40"]["ms_per_step"] / out["learned"]["ms_per_step"], 3)
+    return out
+
+
 def bench_learned(phc, n=256, steps=2000):
     """Learned thin boundary: trained from scratch on the GPU (200 wide-sponge
     shots, Adam), then one shot on an n^3 region: the 4-point learned layer,
@@ -335,7 +370,8 @@ def main():
     ap.add_argument("--arith", default="fp32", choices=["fp32", "ref"])
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched multi-shot measurement reported beside the headline")
-    ap.add_argument("--workload", default="fd", choices=["fd", "batch", "sem", "sem2d", "pml", "semelem", "learned"],
+    ap.add_argument("--workload", default="fd", choices=["fd", "batch", "sem", "sem2d", "pml", "semelem", "learned",
+                                                        "learned-step"],
                     help="fd: the 512^3 headline; batch: only the batched multi-shot workload "
                          "(--steps steps, default 800); sem: only the SEM order-4 workload")
     ap.add_argument("--batch-shape", default=None,
@@ -354,6 +390,15 @@ def main():
     if phc.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "learned-step":
+        r = bench_learned_step(phc)
+        out = {"metric": METRIC, "value": r["speedup_learned_vs_sponge40"], "unit": "x (sponge step / learned step)",
+               "n_gpus": 1, "steps": 300, "warmup": 0, "ms_per_step": r["learned"]["ms_per_step"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+               "data": "synthetic: seeded layered vp, untrained layer", "config": {"workload": r["workload"]},
+               "learned_step": r}
+        print(json.dumps(out), flush=True)
+        return
     if args.workload == "learned":
         r = bench_learned(phc)
         out = {"metric": METRIC, "value": r["learned"]["trace_rel_l2"], "unit": "receiver-trace rel. L2",
