@@ -198,8 +198,9 @@ typedef struct {
     int src_element;       /* element holding the source (SEMproxy.cpp:101-106) */
     int kernel;            /* PHC_SEM_* */
     int weights_ref;       /* 1: the reference's 8-decimal GLL weights; 0: exact */
-    int tile_j, block_waves, chunk_i; /* separable kernel tiling (0: default); chunk_i = -1:
-                                         per-XCD schedule of plane ranges */
+    int tile_j, block_waves, chunk_i; /* separable kernel tiling (0: default); chunk_i = -m,
+                                         m = 1..4: per-XCD schedule of 8 >> (m-1) plane
+                                         ranges x 2^(m-1) column bands */
 } phc_sem_params;
 
 phc_status phc_sem_create(const phc_sem_params *p, phc_sem **out);
@@ -231,8 +232,8 @@ phc_status phc_sem_set_timing(phc_sem *h, int enable);
 phc_status phc_sem_last_run_kernel_ms(phc_sem *h, double *ms_per_step);
 phc_status phc_sem_synchronize(phc_sem *h);
 /* Autotuner of the separable kernel (no reference counterpart): block shape
- * and chunk length (or -1, the per-XCD schedule) timed on scratch copies of
- * the levels; optional text cache.
+ * and chunk length (or -1..-4, the per-XCD schedules) timed on scratch copies
+ * of the levels; optional text cache.
  * A no-op returning zeros for the element kernel. */
 phc_status phc_sem_autotune(phc_sem *h, int steps_per_trial, const char *cache_path, int *tile_j,
                             int *block_waves, int *chunk_i);

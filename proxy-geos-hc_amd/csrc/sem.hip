@@ -426,7 +426,7 @@ struct phc_sem {
     // separable path
     FdLayout L;
     DevBuf lev[2], qf;
-    // chunk_i < 0: the per-XCD schedule, rebuilt when the block shape changes
+    // chunk_i < 0: the per-XCD schedule, rebuilt when it or the block shape changes
     DevBuf sched;
     int sched_nb = 0, sched_tj = -1;
     int cus = 256;
@@ -494,33 +494,39 @@ static phc_status sem_get(phc_sem *h, const DevBuf &src, float *host)
     return PHC_OK;
 }
 
-// Per-XCD schedule of the separable kernel (chunk_i < 0).  The hardware
-// deals block b to XCD b % 8, and each XCD starts its blocks in order as its
-// CUs free up, one 16-wave block per CU at a time.  XCD x takes plane range x
-// of 8 (cut between 4-plane groups) over every tile column, so the tiles of
-// one L2 march the same planes.  The columns that would leave a last round
-// part-empty are cut into shorter pieces that fill one round instead.  The
-// lists are padded to one length with empty ranges; table entry b belongs to
-// hardware block b.  Coverage does not depend on where a block runs.
+// Per-XCD schedule of the separable kernel (chunk_i = -m, m = 1..4).  The
+// hardware deals block b to XCD b % 8, and each XCD starts its blocks in order
+// as its CUs free up, one 16-wave block per CU at a time.  The march axis is
+// cut into R = 8 >> (m - 1) plane ranges (between 4-plane groups) and the tile
+// columns into 8 / R bands; XCD x takes range x % R of band x / R, so the
+// tiles of one L2 march the same planes (longer ranges: less ring priming
+// per march).  The columns that would leave a last round part-empty are cut
+// into shorter pieces that fill one round instead.  The lists are padded to
+// one length with empty ranges; table entry b belongs to hardware block b.
+// Coverage does not depend on where a block runs.
 static phc_status sem_prepare_sched(phc_sem *h)
 {
     const int TJ = h->ty * h->nj;
-    if (h->kernel != PHC_SEM_SEPARABLE || h->chunk_i >= 0 || h->sched_tj == TJ) return PHC_OK;
+    const int key = TJ * 8 - h->chunk_i;
+    if (h->kernel != PHC_SEM_SEPARABLE || h->chunk_i >= 0 || h->sched_tj == key) return PHC_OK;
+    const int R = 8 >> (-h->chunk_i - 1), B = 8 / R;
     const int nx = h->L.nx;
     const int ncol = ((std::max(h->L.nz - 1, 1) + FD_TK - 1) / FD_TK) * ((std::max(h->L.ny - 2, 1) + TJ - 1) / TJ);
     const int cpx = std::max(h->cus / 8, 1);  // CUs per XCD
     std::vector<std::vector<int4>> lists(8);
     size_t longest = 0;
-    const int G = (nx + 3) / 4;  // 4-plane groups of the march axis, dealt evenly over the XCDs
+    const int G = (nx + 3) / 4;  // 4-plane groups of the march axis, dealt evenly over the ranges
     for (int x = 0; x < 8; ++x) {
-        const int a = 4 * (G * x / 8), b = std::min(4 * (G * (x + 1) / 8), nx);
+        const int r = x % R, band = x / R;
+        const int a = 4 * (G * r / R), b = std::min(4 * (G * (r + 1) / R), nx);
+        const int c0 = ncol * band / B, c1 = ncol * (band + 1) / B;
         auto &li = lists[x];
-        if (a < b) {
-            const int rem = ncol % cpx, full = ncol - rem;
-            for (int w = 0; w < full; ++w) li.push_back(make_int4(w, a, b, 0));
+        if (a < b && c0 < c1) {
+            const int rem = (c1 - c0) % cpx, full = c1 - rem;
+            for (int w = c0; w < full; ++w) li.push_back(make_int4(w, a, b, 0));
             const int g = (b - a + 3) / 4;  // 4-plane groups of the range
             const int pieces = rem ? std::max(1, std::min(cpx / rem, g)) : 0;
-            for (int w = full; w < ncol; ++w)
+            for (int w = full; w < c1; ++w)
                 for (int q = 0; q < pieces; ++q) {
                     const int g0 = g * q / pieces, g1 = g * (q + 1) / pieces;
                     li.push_back(make_int4(w, a + 4 * g0, std::min(a + 4 * g1, b), 0));
@@ -536,7 +542,7 @@ static phc_status sem_prepare_sched(phc_sem *h)
     PHC_TRY(h->sched.alloc(sizeof(int4) * tab.size()));
     PHC_HIP(hipMemcpy(h->sched.ptr, tab.data(), sizeof(int4) * tab.size(), hipMemcpyHostToDevice));
     h->sched_nb = (int)tab.size();
-    h->sched_tj = TJ;
+    h->sched_tj = key;
     return PHC_OK;
 }
 
@@ -767,8 +773,8 @@ phc_status phc_sem_create(const phc_sem_params *p, phc_sem **out)
         h->ty = p->block_waves ? p->block_waves : 16;
         h->nj = p->tile_j ? p->tile_j / h->ty : 1;
         h->chunk_i = p->chunk_i ? p->chunk_i : h->ny;
-        if (h->chunk_i < -1)
-            return bail(fail(PHC_ERR_INVALID, "chunk_i %d: planes per chunk, or -1 for the per-XCD schedule",
+        if (h->chunk_i < -4)
+            return bail(fail(PHC_ERR_INVALID, "chunk_i %d: planes per chunk, or -1..-4 for the per-XCD schedule",
                              h->chunk_i));
         if (!sem_pick_sep(P, h->ty, h->nj, true))
             return bail(fail(PHC_ERR_INVALID, "no separable kernel for p=%d with %d waves x %d rows",
@@ -1086,7 +1092,7 @@ extern "C" phc_status phc_sem_autotune(phc_sem *h, int steps_per_trial, const ch
             double ms;
             while (std::fscanf(f, "%7s %d %d %d %d %d %d %d %lf", tag, &ex, &ey, &ez, &pp, &a, &b, &c, &ms) == 9)
                 if (!std::strcmp(tag, "sem") && ex == h->ex && ey == h->ey && ez == h->ez && pp == h->p &&
-                    a > 0 && b % a == 0 && (c > 0 || c == -1) && sem_pick_sep(h->p, a, b / a, true)) {
+                    a > 0 && b % a == 0 && (c > 0 || c >= -4) && sem_pick_sep(h->p, a, b / a, true)) {
                     h->ty = a; h->nj = b / a; h->chunk_i = c;
                     hit = true;
                 }
@@ -1115,7 +1121,7 @@ extern "C" phc_status phc_sem_autotune(phc_sem *h, int steps_per_trial, const ch
             c += (4 - c % 4) % 4;
             if (std::find(chunks.begin(), chunks.end(), c) == chunks.end()) chunks.push_back(c);
         }
-        chunks.push_back(-1);  // the per-XCD schedule
+        for (int m = 1; m <= 4; ++m) chunks.push_back(-m);  // the per-XCD schedules
         double best = 1e30;
         int bty = h->ty, bnj = h->nj, bci = h->chunk_i;
         for (auto &sh : shapes) {

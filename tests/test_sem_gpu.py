@@ -120,9 +120,9 @@ def test_every_block_shape_gives_the_same_bits(gpu, tmp_path, ex, ey, ez, order)
               f0=25.0, nsamples=40, src_point=(20.0 * (ex - 1) + 1, 21.0, 21.0))
     ref = gpu.SemSolver(**kw)
     ref.run(30)
-    # chunk -1: the per-XCD schedule of plane ranges
+    # chunk -m: the per-XCD schedules (8 >> (m-1) plane ranges x 2^(m-1) column bands)
     for waves, tile_j, chunk in [(16, 16, 8), (16, 32, 12), (8, 16, 1 << 20), (4, 8, 20), (4, 4, 4),
-                                 (16, 16, -1), (4, 8, -1)]:
+                                 (16, 16, -1), (4, 8, -1), (16, 16, -2), (4, 8, -3), (16, 16, -4)]:
         banner("shape waves=%d tile_j=%d chunk=%d" % (waves, tile_j, chunk))
         cache = tmp_path / ("s%d_%d_%d.txt" % (waves, tile_j, chunk))
         cache.write_text("sem %d %d %d %d %d %d %d 1.0\n" % (ex, ey, ez, order, waves, tile_j, chunk))
@@ -142,16 +142,16 @@ def test_xcd_schedule_rounds_and_pieces_bit_exact(gpu):
     ref = gpu.SemSolver(block_waves=16, tile_j=16, chunk_i=1 << 20, **kw)
     ref.run(25)
     assert np.abs(ref.current()).max() > 0
-    for waves, tile_j in [(4, 4), (16, 16), (8, 16)]:
-        banner("per-XCD schedule waves=%d tile_j=%d" % (waves, tile_j))
-        s = gpu.SemSolver(block_waves=waves, tile_j=tile_j, chunk_i=-1, **kw)
+    for waves, tile_j, m in [(4, 4, 1), (16, 16, 1), (8, 16, 1), (4, 4, 2), (4, 4, 3), (4, 4, 4), (16, 16, 3)]:
+        banner("per-XCD schedule waves=%d tile_j=%d split %d" % (waves, tile_j, m))
+        s = gpu.SemSolver(block_waves=waves, tile_j=tile_j, chunk_i=-m, **kw)
         s.run(25)
         assert np.array_equal(s.current(), ref.current())
         assert np.array_equal(s.previous(), ref.previous())
         s.compute_one_step(25, s.i1, s.i2)  # the per-step surface uses the same table
         s.close()
     with pytest.raises(RuntimeError, match="chunk_i"):
-        gpu.SemSolver(chunk_i=-2, **kw)
+        gpu.SemSolver(chunk_i=-5, **kw)
 
 
 @pytest.mark.parametrize("order", [2, 4])
