@@ -271,8 +271,7 @@ def bench_sem_elem(phc, steps=100, steps_p4=20):
     nodes, c = 1500, f0 = 5 Hz, dt = 1e-3) through the three SEM step kernels:
     separable, element with atomic scatter, element with the 8-colour
     scatter; then the same three on 100^3 elements of order 4 (401^3 nodes,
-    dt = 2e-4, f0 = 8 Hz), where the element path reads 24 B of geometry
-    factors per quadrature point."""
+    dt = 2e-4, f0 = 8 Hz)."""
     out = {"workload": "SEM 50^3 elements of order 2 (101^3 nodes), %d steps, FP32" % steps}
     for name, kern in (("separable", 0), ("element_atomic", 1), ("element_colour", 2)):
         s = phc.SemSolver(ex=50, lx=2000.0, order=2, dt=1e-3, f0=5.0, nsamples=steps + 1, kernel=kern)
