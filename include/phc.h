@@ -231,6 +231,14 @@ phc_status phc_sem_check_finite(phc_sem *h, int level);
 phc_status phc_sem_set_timing(phc_sem *h, int enable);
 phc_status phc_sem_last_run_kernel_ms(phc_sem *h, double *ms_per_step);
 phc_status phc_sem_synchronize(phc_sem *h);
+/* Per-XCD schedule of the separable kernel (no reference counterpart; host
+ * only, for inspection and tests): for a march axis of march_planes planes,
+ * tile_columns tile columns, cus CUs and split m (chunk_i = -m), the table
+ * hardware block b reads: entries[3b..3b+2] = tile column, first plane, end
+ * plane (end <= first: no work).  With entries NULL only *n_entries is set;
+ * otherwise *n_entries is the room in entries (in triples) on entry. */
+phc_status phc_sem_schedule_table(int march_planes, int tile_columns, int cus, int split, int *entries,
+                                  int *n_entries);
 /* Autotuner of the separable kernel (no reference counterpart): block shape
  * and chunk length (or -1..-4, the per-XCD schedules) timed on scratch copies
  * of the levels; optional text cache.

@@ -504,15 +504,10 @@ static phc_status sem_get(phc_sem *h, const DevBuf &src, float *host)
 // into shorter pieces that fill one round instead.  The lists are padded to
 // one length with empty ranges; table entry b belongs to hardware block b.
 // Coverage does not depend on where a block runs.
-static phc_status sem_prepare_sched(phc_sem *h)
+static void sem_build_sched(int nx, int ncol, int cus, int m, std::vector<int4> &tab)
 {
-    const int TJ = h->ty * h->nj;
-    const int key = TJ * 8 - h->chunk_i;
-    if (h->kernel != PHC_SEM_SEPARABLE || h->chunk_i >= 0 || h->sched_tj == key) return PHC_OK;
-    const int R = 8 >> (-h->chunk_i - 1), B = 8 / R;
-    const int nx = h->L.nx;
-    const int ncol = ((std::max(h->L.nz - 1, 1) + FD_TK - 1) / FD_TK) * ((std::max(h->L.ny - 2, 1) + TJ - 1) / TJ);
-    const int cpx = std::max(h->cus / 8, 1);  // CUs per XCD
+    const int R = 8 >> (m - 1), B = 8 / R;
+    const int cpx = std::max(cus / 8, 1);  // CUs per XCD
     std::vector<std::vector<int4>> lists(8);
     size_t longest = 0;
     const int G = (nx + 3) / 4;  // 4-plane groups of the march axis, dealt evenly over the ranges
@@ -534,9 +529,19 @@ static phc_status sem_prepare_sched(phc_sem *h)
         }
         longest = std::max(longest, li.size());
     }
-    std::vector<int4> tab(8 * longest, make_int4(0, 0, 0, 0));
+    tab.assign(8 * longest, make_int4(0, 0, 0, 0));
     for (int x = 0; x < 8; ++x)
         for (size_t l = 0; l < lists[x].size(); ++l) tab[8 * l + x] = lists[x][l];
+}
+
+static phc_status sem_prepare_sched(phc_sem *h)
+{
+    const int TJ = h->ty * h->nj;
+    const int key = TJ * 8 - h->chunk_i;
+    if (h->kernel != PHC_SEM_SEPARABLE || h->chunk_i >= 0 || h->sched_tj == key) return PHC_OK;
+    const int ncol = ((std::max(h->L.nz - 1, 1) + FD_TK - 1) / FD_TK) * ((std::max(h->L.ny - 2, 1) + TJ - 1) / TJ);
+    std::vector<int4> tab;
+    sem_build_sched(h->L.nx, ncol, h->cus, -h->chunk_i, tab);
     PHC_HIP(hipStreamSynchronize(h->stream));  // queued launches may still read the old table
     sem_drop_graph(h);
     PHC_TRY(h->sched.alloc(sizeof(int4) * tab.size()));
@@ -656,6 +661,28 @@ int phc_sem_element_from_point(int ex, int ey, int ez, float lx, float ly, float
     for (int i = 0; i < ez; ++i)
         if (z >= i * hz && z < (i + 1) * hz) eZ = i;
     return eX + eZ * ex + eY * ex * ez;
+}
+
+phc_status phc_sem_schedule_table(int march_planes, int tile_columns, int cus, int split, int *entries,
+                                  int *n_entries)
+{
+    if (!n_entries) return fail(PHC_ERR_INVALID, "n_entries is NULL");
+    if (march_planes < 1 || tile_columns < 1 || cus < 1 || split < 1 || split > 4)
+        return fail(PHC_ERR_INVALID, "planes %d, columns %d, CUs %d, split %d", march_planes, tile_columns, cus,
+                    split);
+    std::vector<int4> tab;
+    sem_build_sched(march_planes, tile_columns, cus, split, tab);
+    if (entries) {
+        if (*n_entries < (int)tab.size())
+            return fail(PHC_ERR_INVALID, "table has %d entries, room for %d", (int)tab.size(), *n_entries);
+        for (size_t b = 0; b < tab.size(); ++b) {
+            entries[3 * b] = tab[b].x;
+            entries[3 * b + 1] = tab[b].y;
+            entries[3 * b + 2] = tab[b].z;
+        }
+    }
+    *n_entries = (int)tab.size();
+    return PHC_OK;
 }
 
 phc_status phc_sem_create(const phc_sem_params *p, phc_sem **out)

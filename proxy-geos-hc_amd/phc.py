@@ -701,6 +701,7 @@ SIGNATURES.update({
     "phc_sem_set_timing": (c_int, [c_void_p, c_int]),
     "phc_sem_last_run_kernel_ms": (c_int, [c_void_p, ctypes.POINTER(c_double)]),
     "phc_sem_synchronize": (c_int, [c_void_p]),
+    "phc_sem_schedule_table": (c_int, [c_int, c_int, c_int, c_int, c_void_p, ctypes.POINTER(c_int)]),
     "phc_sem_autotune": (c_int, [c_void_p, c_int, ctypes.c_char_p, ctypes.POINTER(c_int),
                                  ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
     "phc_sem_element_from_point": (c_int, [c_int, c_int, c_int, c_float, c_float, c_float,
@@ -946,6 +947,16 @@ class Sem64Solver:
         out = np.zeros(self.nn)
         check(lib().phc_sem64_get_field(self.h, 1 if current else 0, out))
         return out
+
+
+def sem_schedule_table(march_planes, tile_columns, cus=256, split=1):
+    """The per-XCD schedule table of the separable SEM kernel (host only):
+    rows (tile column, first plane, end plane), row b read by hardware block b."""
+    n = c_int(0)
+    check(lib().phc_sem_schedule_table(march_planes, tile_columns, cus, split, None, ctypes.byref(n)))
+    out = np.zeros((n.value, 3), np.int32)
+    check(lib().phc_sem_schedule_table(march_planes, tile_columns, cus, split, _ptr(out), ctypes.byref(n)))
+    return out
 
 
 class SemSolver:
