@@ -64,11 +64,11 @@ def test_oracle_pml_absorbs(oracle):
     assert ea < 0.05 * eb, (ea, eb)
 
 
-def pair(phc, oracle, shape, f0=30.0, xs=None, ys=None, zs=None, nsamples=60, ndamp=None, model="two_layer"):
+def pair(phc, oracle, shape, f0=30.0, xs=None, ys=None, zs=None, nsamples=60, ndamp=None, model="two_layer", L=4):
     nx, ny, nz = shape
-    s = phc.FdSetup(nx=nx, ny=ny, nz=nz, L=4, f0=f0, xs=xs, ys=ys, zs=zs, sponge_width=0, model=model,
+    s = phc.FdSetup(nx=nx, ny=ny, nz=nz, L=L, f0=f0, xs=xs, ys=ys, zs=zs, sponge_width=0, model=model,
                     nsamples=nsamples)
-    P = oracle.FdProblem(nx=nx, ny=ny, nz=nz, L=4, xs=s.xs, ys=s.ys, zs=s.zs, f0=f0, model=s.vp.copy(),
+    P = oracle.FdProblem(nx=nx, ny=ny, nz=nz, L=L, xs=s.xs, ys=s.ys, zs=s.zs, f0=f0, model=s.vp.copy(),
                          nsamples=nsamples, sponge_width=0, pml=True, ndamp=ndamp)
     assert np.array_equal(P.rhs, s.rhs)
     return s, P
@@ -101,6 +101,24 @@ def test_gpu_pml_bit_exact(gpu, oracle, shape, ndamp, src):
     otr, orv = P.run(20, receivers=rec)
     assert np.array_equal(src_t, otr) and np.array_equal(rv, orv)
     assert np.array_equal(sol.current(), P.current())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L", [1, 2, 3, 5, 6])
+def test_gpu_pml_bit_exact_other_orders(gpu, oracle, L):
+    # -usePML with the stencil of half width L (the march's ring, halo and the
+    # pml3D first differences all change with it)
+    print("\n[pml] half width %d" % L, flush=True)
+    s, P = pair(gpu, oracle, (30, 27, 41), nsamples=50, L=L)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF)
+    nd = sol.set_pml(ndamp=int(P.ndamp[0]), f0=s.f0)
+    assert nd == int(P.ndamp[0])
+    src_t, _ = sol.run(0, 45)
+    tr, _ = P.run(45)
+    assert np.abs(tr).max() > 0
+    assert np.array_equal(src_t, tr)
+    assert np.array_equal(sol.current(), P.current())
+    assert np.array_equal(sol.previous(), P.previous())
 
 
 @pytest.mark.gpu
