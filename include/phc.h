@@ -201,6 +201,8 @@ typedef struct {
     int tile_j, block_waves, chunk_i; /* separable kernel tiling (0: default); chunk_i = -m,
                                          m = 1..4: per-XCD schedule of 8 >> (m-1) plane
                                          ranges x 2^(m-1) column bands */
+    int index64;           /* 1: 64-bit offsets in the separable and 2D kernels even
+                              when every array is < 4 GiB (testing) */
 } phc_sem_params;
 
 phc_status phc_sem_create(const phc_sem_params *p, phc_sem **out);

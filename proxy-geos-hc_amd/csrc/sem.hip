@@ -433,6 +433,7 @@ struct phc_sem {
     float cx[SEM_MAX_P][2 * SEM_MAX_P + 1], cy[SEM_MAX_P][2 * SEM_MAX_P + 1],
         cz[SEM_MAX_P][2 * SEM_MAX_P + 1];
     int ty = 16, nj = 1, chunk_i = 0;
+    int index64 = 0;  // 64-bit offsets even for arrays < 4 GiB
     // element path
     DevBuf y, dt2m, geo, rhs;
     float Dm[(SEM_MAX_P + 1) * (SEM_MAX_P + 1)];
@@ -588,7 +589,7 @@ static phc_status sem_launch(phc_sem *h, int i1, int i2, int inj, float rhs_next
             if (h->p == 4 && a.chunk_i < h->L.nx && a.chunk_i % 4) a.chunk_i += 4 - a.chunk_i % 4;
             nb = a.tiles_k * a.tiles_j * ((h->L.nx + a.chunk_i - 1) / a.chunk_i);
         }
-        SemSepFn fn = sem_pick_sep(h->p, h->ty, h->nj, h->L.total * 4 < ((int64_t)1 << 32));
+        SemSepFn fn = sem_pick_sep(h->p, h->ty, h->nj, !h->index64 && h->L.total * 4 < ((int64_t)1 << 32));
         if (!fn) return fail(PHC_ERR_INVALID, "no separable kernel for p=%d", h->p);
         if (e0) PHC_HIP(hipEventRecord(e0, h->stream));
         hipLaunchKernelGGL(fn, dim3(nb), dim3(FD_TK, h->ty), 0, h->stream, a);
@@ -800,6 +801,7 @@ phc_status phc_sem_create(const phc_sem_params *p, phc_sem **out)
         h->ty = p->block_waves ? p->block_waves : 16;
         h->nj = p->tile_j ? p->tile_j / h->ty : 1;
         h->chunk_i = p->chunk_i ? p->chunk_i : h->ny;
+        h->index64 = p->index64 ? 1 : 0;
         if (h->chunk_i < -4)
             return bail(fail(PHC_ERR_INVALID, "chunk_i %d: planes per chunk, or -1..-4 for the per-XCD schedule",
                              h->chunk_i));

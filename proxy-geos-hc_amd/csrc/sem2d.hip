@@ -239,6 +239,7 @@ struct phc_sem2d {
     hipStream_t stream = nullptr;
     Sem2Layout L;
     DevBuf lev[2], qf, sig, rhs;   // sig: z0[nx], z1[nx], x0[nz], x1[nz]
+    int index64 = 0;               // 64-bit offsets even for arrays < 4 GiB
     float cx[S2_MAX_P + 2][S2_W], cz[S2_MAX_P + 2][S2_W];
     std::vector<float> hmass, hsh;  // FP32 lumped mass and boundary term per node
     int chunk = 0;
@@ -316,7 +317,7 @@ static phc_status s2_launch(phc_sem2d *h, int i1, int i2, int inj, float rhs_nex
     a.chunk = h->chunk > 0 ? h->chunk : s2_default_chunk(h);
     a.nchunks = (h->nz + a.chunk - 1) / a.chunk;
     const int waves = h->L.tiles_x * a.nchunks;
-    Sem2dFn fn = sem2d_pick(h->p, h->L.total * 4 < ((int64_t)1 << 32));
+    Sem2dFn fn = sem2d_pick(h->p, !h->index64 && h->L.total * 4 < ((int64_t)1 << 32));
     if (!fn) return fail(PHC_ERR_INVALID, "no 2D SEM kernel for p=%d", h->p);
     if (e0) PHC_HIP(hipEventRecord(e0, h->stream));
     hipLaunchKernelGGL(fn, dim3((waves + S2_WAVES - 1) / S2_WAVES), dim3(64 * S2_WAVES), 0, h->stream, a);
@@ -473,6 +474,7 @@ phc_status phc_sem2d_create(const phc_sem_params *p, phc_sem2d **out)
     rows(ax, h->cx);
     rows(az, h->cz);
     h->chunk = p->chunk_i > 0 ? std::min(p->chunk_i, h->nz) : 0;
+    h->index64 = p->index64 ? 1 : 0;
     *out = h;
     return PHC_OK;
 }

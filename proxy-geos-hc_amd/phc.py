@@ -682,7 +682,7 @@ class SemParams(ctypes.Structure):
                 ("order", c_int), ("model", c_void_p), ("dt", c_float),
                 ("rhs", c_void_p), ("nsamples", c_int), ("src_element", c_int),
                 ("kernel", c_int), ("weights_ref", c_int),
-                ("tile_j", c_int), ("block_waves", c_int), ("chunk_i", c_int)]
+                ("tile_j", c_int), ("block_waves", c_int), ("chunk_i", c_int), ("index64", c_int)]
 
 
 SIGNATURES.update({
@@ -736,7 +736,7 @@ class Sem2dSolver:
 
     def __init__(self, ex=65, lx=2000.0, order=3, dt=0.001, f0=10.0, model=None, nsamples=1000,
                  src_point=(1001.0, 1001.0, 1001.0), ez=None, lz=None, src_element=None,
-                 weights_ref=True, source_order=1, rhs=None, chunk=0):
+                 weights_ref=True, source_order=1, rhs=None, chunk=0, index64=False):
         ez = ex if ez is None else ez
         lz = lx if lz is None else lz
         nE = ex * ez
@@ -761,6 +761,7 @@ class Sem2dSolver:
         p.kernel = SEM_SEPARABLE
         p.weights_ref = 1 if weights_ref else 0
         p.chunk_i = chunk
+        p.index64 = 1 if index64 else 0
         h = c_void_p()
         check(lib().phc_sem2d_create(ctypes.byref(p), ctypes.byref(h)))
         self.h = h
@@ -967,7 +968,7 @@ class SemSolver:
     def __init__(self, ex=10, lx=2000.0, order=2, dt=0.001, f0=10.0, model=None, nsamples=100,
                  src_point=(1001.0, 1001.0, 1001.0), ey=None, ez=None, ly=None, lz=None,
                  kernel=SEM_SEPARABLE, weights_ref=True, source_order=1, rhs=None,
-                 tile_j=0, block_waves=0, chunk_i=0):
+                 tile_j=0, block_waves=0, chunk_i=0, index64=False):
         ey = ex if ey is None else ey
         ez = ex if ez is None else ez
         ly = lx if ly is None else ly
@@ -991,6 +992,7 @@ class SemSolver:
         p.kernel = kernel
         p.weights_ref = 1 if weights_ref else 0
         p.tile_j, p.block_waves, p.chunk_i = tile_j, block_waves, chunk_i
+        p.index64 = 1 if index64 else 0
         h = c_void_p()
         check(lib().phc_sem_create(ctypes.byref(p), ctypes.byref(h)))
         self.h = h

@@ -77,6 +77,16 @@ def test_rows_per_wave_do_not_change_results(gpu, oracle, chunk):
     assert np.array_equal(G.previous(), ref.previous())
 
 
+def test_64bit_offsets_give_the_same_bits(gpu):
+    kw = dict(ex=21, ez=17, order=4, lx=840.0, lz=680.0, dt=5e-4, nsamples=60, src_element=3 + 5 * 21)
+    A = gpu.Sem2dSolver(**kw)
+    B = gpu.Sem2dSolver(index64=True, **kw)
+    a_src, _ = A.run(50)
+    b_src, _ = B.run(50)
+    assert np.array_equal(a_src, b_src)
+    assert np.array_equal(A.current(), B.current()) and np.array_equal(A.previous(), B.previous())
+
+
 def test_mass_and_boundary_term_match_oracle(gpu, oracle):
     ex, ez, p = 9, 7, 4
     G, O = pair(gpu, oracle, ex, ez, p, 1, src_element=0)

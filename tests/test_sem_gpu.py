@@ -154,6 +154,23 @@ def test_xcd_schedule_rounds_and_pieces_bit_exact(gpu):
         gpu.SemSolver(chunk_i=-5, **kw)
 
 
+@pytest.mark.parametrize("order", [2, 4, 5])
+def test_64bit_offsets_give_the_same_bits(gpu, order):
+    # the kernels built for arrays of 4 GiB and more (64-bit offsets), forced
+    # on a small mesh: same values as the 32-bit offsets, chunked and per-XCD
+    kw = dict(ex=7, ey=5, ez=6, lx=140.0, ly=100.0, lz=120.0, order=order, dt=0.0003, f0=25.0, nsamples=40,
+              src_point=(71.0, 51.0, 61.0))
+    ref = gpu.SemSolver(**kw)
+    src_ref, _ = ref.run(30)
+    for chunk in (0, 12, -1):
+        s = gpu.SemSolver(index64=True, chunk_i=chunk, **kw)
+        src, _ = s.run(30)
+        assert np.array_equal(src, src_ref)
+        assert np.array_equal(s.current(), ref.current())
+        assert np.array_equal(s.previous(), ref.previous())
+        s.close()
+
+
 @pytest.mark.parametrize("order", [2, 4])
 def test_mass_and_operator(gpu, oracle, order):
     banner("mass/operator order=%d" % order)
