@@ -104,13 +104,14 @@ def test_gpu_pml_bit_exact(gpu, oracle, shape, ndamp, src):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("L", [1, 2, 3, 5, 6])
-def test_gpu_pml_bit_exact_other_orders(gpu, oracle, L):
+@pytest.mark.parametrize("L,index64", [(1, False), (2, False), (3, False), (5, False), (6, False), (4, True)])
+def test_gpu_pml_bit_exact_other_orders(gpu, oracle, L, index64):
     # -usePML with the stencil of half width L (the march's ring, halo and the
-    # pml3D first differences all change with it)
-    print("\n[pml] half width %d" % L, flush=True)
+    # pml3D first differences all change with it); and the 64-bit-offset
+    # kernel (used for arrays of 4 GiB and more) forced on a small grid
+    print("\n[pml] half width %d index64 %d" % (L, index64), flush=True)
     s, P = pair(gpu, oracle, (30, 27, 41), nsamples=50, L=L)
-    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF, index64=index64)
     nd = sol.set_pml(ndamp=int(P.ndamp[0]), f0=s.f0)
     assert nd == int(P.ndamp[0])
     src_t, _ = sol.run(0, 45)
