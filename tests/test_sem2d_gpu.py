@@ -172,3 +172,30 @@ def test_rejects_bad_parameters(gpu):
         gpu.check(gpu.lib().phc_sem2d_create(ctypes.byref(p), ctypes.byref(h)))
     with pytest.raises(RuntimeError, match="order"):
         gpu.Sem2dSolver(ex=4, order=6, nsamples=2, src_element=0)
+
+
+def test_full_size_8001_rows_per_wave_and_offsets(gpu):
+    # the benchmark's large mesh (2000 x 2000 elements of order 4, 8001^2
+    # nodes), source in the element beside a corner so that the pulse reaches
+    # both paraxial edges near it: marches of 2, 32 and all rows per
+    # wavefront and the 64-bit-offset kernel give the same bits
+    kw = dict(ex=2000, order=4, lx=40000.0, dt=2e-4, f0=25.0, nsamples=301, src_element=1 + 1 * 2000)
+    ref = gpu.Sem2dSolver(chunk=2, **kw)
+    # the source node is (x, z) = (4, 4): receivers 6 nodes along +x and +z,
+    # a corner and two edge nodes
+    nodes = [ref.src_node, ref.src_node + 6, ref.src_node + 6 * ref.nx, 0, ref.nx * 20, ref.nx + 20]
+    ref.set_receivers(nodes)
+    src0, rec0 = ref.run(300)
+    cur, prev = ref.current(), ref.previous()
+    ref.close()
+    f = cur.reshape(ref.nz, ref.nx)
+    assert np.isfinite(f).all() and np.abs(f[0, :40]).max() > 0 and np.abs(f[:40, 0]).max() > 0
+    assert np.abs(rec0[1:3]).max() > 1e-3 * np.abs(src0).max()
+    for chunk, i64 in [(32, False), (1 << 20, False), (32, True)]:
+        print("\n[sem2d] 8001^2 chunk %d index64 %d" % (chunk, i64), flush=True)
+        G = gpu.Sem2dSolver(chunk=chunk, index64=i64, **kw)
+        G.set_receivers(nodes)
+        src, rec = G.run(300)
+        assert np.array_equal(src, src0) and np.array_equal(rec, rec0)
+        assert np.array_equal(G.current(), cur) and np.array_equal(G.previous(), prev)
+        G.close()
