@@ -124,3 +124,34 @@ def test_source_on_tile_and_grid_edges(gpu, oracle):
         tr, _ = P.run(25)
         assert np.array_equal(src[k], tr), "shot %d" % k
         assert np.array_equal(b.field(k, True), P.current()), "shot %d" % k
+
+
+def test_full_size_survey_every_shot_exact(gpu, oracle):
+    # the benchmark survey (64 shots, axes U[128, 384], 14 GB of fields):
+    # every shot of the FP32 batch = the single-grid FP32 path bit for bit,
+    # and three of its shots batched in reference rounding = the oracle
+    setups = gpu.multi_shot_setups(nshots=64, nsteps=10, seed=2024)
+    n = 6
+    banner("64-shot survey, FP32, %d steps" % n)
+    b = gpu.FdBatch(setups, arith=gpu.ARITH_FP32)
+    src, _ = b.run(n)
+    for k, s in enumerate(setups):
+        one = gpu.FdSolver(s, arith=gpu.ARITH_FP32)
+        tr, _ = one.run(0, n)
+        assert np.array_equal(src[k], tr), "shot %d" % k
+        assert np.array_equal(b.field(k, True), one.current()), "shot %d current" % k
+        one.close()
+    b.close()
+    banner("3 survey shots, reference rounding, vs the oracle")
+    sub = setups[:3]
+    r = gpu.FdBatch(sub, arith=gpu.ARITH_REF)
+    rsrc, _ = r.run(n)
+    for k, s in enumerate(sub):
+        P = oracle.FdProblem(nx=s.nx, ny=s.ny, nz=s.nz, L=4, xs=s.xs, ys=s.ys, zs=s.zs, f0=s.f0, dt=s.dt,
+                             model=s.vp.copy(), nsamples=s.nsamples, sponge_width=s.sponge_width)
+        assert np.array_equal(P.rhs, s.rhs)
+        tr, _ = P.run(n)
+        assert np.abs(tr).max() > 0
+        assert np.array_equal(rsrc[k], tr), "shot %d" % k
+        assert np.array_equal(r.field(k, True), P.current()), "shot %d current" % k
+    r.close()
