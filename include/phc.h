@@ -60,7 +60,7 @@ typedef struct phc_fd phc_fd;
 typedef struct {
     /* FDTDGRIDS (src/fd/utilsFDTD/FDTDdata.hpp:6-23) */
     int nx, ny, nz;   /* interior points per axis */
-    int lx, ly, lz;   /* stencil half widths (halo widths); must be equal, 1..6 */
+    int lx, ly, lz;   /* stencil half widths (halo widths), each 1..6 (FDTDinit.hpp:96-110) */
     int xs, ys, zs;   /* source point (interior coordinates) */
     /* FDTDMODELS (FDTDdata.hpp:25-40) */
     const float *coefx, *coefy, *coefz; /* [l+1] each, coef[0] unused by the step */
@@ -87,6 +87,9 @@ phc_status phc_fd_destroy(phc_fd *h);
 
 /* FDTDKernel::addRHS (FDTDkernels.hpp:121-128): level[cb][src] += vp[src]*rhs[it]. */
 phc_status phc_fd_add_rhs(phc_fd *h, int it, int cb);
+/* FDTDKernel::addRHS with the caller's sample (FDTDmain.cpp:59 passes
+ * RHSTerm[itSample] through FDTDMODELS): level[cb](i,j,k) += vp(i,j,k) * rhs. */
+phc_status phc_fd_add_rhs_value(phc_fd *h, int cb, int i, int j, int k, float rhs);
 /* FDTDKernel::computeOneStepSB (FDTDkernels.hpp:75-84) = inner3D + applySponge:
  * level ca <- sponge * (2 level[cb] - level[ca] + vp (coef0 p + lap p)),
  * level cb <- sponge * level cb.  The damping of cb is deferred to the next
@@ -213,6 +216,15 @@ phc_status phc_sem_sizes(phc_sem *h, int *num_nodes, int *nx, int *ny, int *nz);
  * timeSample into level i2 at the first node of the source element, then
  * level i1 <- 2 level i2 - level i1 - dt^2 (K level i2)/M on interior nodes. */
 phc_status phc_sem_compute_one_step(phc_sem *h, int time_sample, int i1, int i2);
+/* SEMsolver::computeOneStep with the caller's sources (SEMproxy.cpp:48 passes
+ * RHSTerm(:, timeSample) and rhsElement, SEMsolver.cpp:37-42): for r = 0..nrhs-1
+ * level i2 at the first node of rhs_element[r] += dt^2 c^2 rhs_value[r], then
+ * the step. */
+phc_status phc_sem_compute_one_step_rhs(phc_sem *h, int i1, int i2, int nrhs, const int *rhs_element,
+                                        const float *rhs_value);
+/* Source of the fused loop (phc_sem_run): element and samples, replacing the
+ * ones given at create (SEMproxy::init_source, SEMproxy.cpp:93-119). */
+phc_status phc_sem_set_source(phc_sem *h, int src_element, int nsamples, const float *rhs);
 /* SEMproxy::run (SEMproxy.cpp:41-64) for steps it0..it0+nsteps-1, injection
  * fused into the preceding step, captured in one hipGraph.  *i1 / *i2 are the
  * reference's level indices and are swapped as the reference swaps them. */
@@ -281,6 +293,11 @@ phc_status phc_sem2d_sizes(phc_sem2d *h, int *num_nodes, int *nx, int *nz);
 /* inject sample time_sample into level i2 at the first node of the source
  * element (SEMsolver.cpp:39-43), then level i1 <- the new level */
 phc_status phc_sem2d_compute_one_step(phc_sem2d *h, int time_sample, int i1, int i2);
+/* the same with the caller's sources (SEMsolver.cpp:37-42), as phc_sem_compute_one_step_rhs */
+phc_status phc_sem2d_compute_one_step_rhs(phc_sem2d *h, int i1, int i2, int nrhs, const int *rhs_element,
+                                          const float *rhs_value);
+/* source of the fused loop, as phc_sem_set_source */
+phc_status phc_sem2d_set_source(phc_sem2d *h, int src_element, int nsamples, const float *rhs);
 /* SEMproxy::run (SEMproxy.cpp:41-64) for steps it0..it0+nsteps-1, captured
  * in one hipGraph; *i1 / *i2 swapped as the reference swaps them */
 phc_status phc_sem2d_run(phc_sem2d *h, int it0, int nsteps, int *i1, int *i2);

@@ -178,25 +178,29 @@ class FdProblem:
     def __init__(self, nx=150, ny=None, nz=None, L=4, dx=10.0, xs=None, ys=None, zs=None,
                  f0=15.0, time_max=1.0, vmin=1500.0, vmax=4500.0, cfl=0.8,
                  source_order=1, sponge_width=20, sponge_alpha=-0.00015,
-                 model="layered2", dt=None, nsamples=None, pml=False, ndamp=None):
+                 model="layered2", dt=None, nsamples=None, pml=False, ndamp=None, ly=None, lz=None):
         ny = nx if ny is None else ny
         nz = nx if nz is None else nz
         self.nx, self.ny, self.nz, self.L = nx, ny, nz, L
+        # per-axis half widths (FDTDinit.hpp:96-110); L is lx
+        ly = L if ly is None else ly
+        lz = L if lz is None else lz
+        self.lh = (L, ly, lz)
         xs = nx // 2 if xs is None else xs
         ys = ny // 2 if ys is None else ys
         zs = nz // 2 if zs is None else zs
         g = FdGrid()
         g.nx, g.ny, g.nz = nx, ny, nz
-        g.lx = g.ly = g.lz = L
+        g.lx, g.ly, g.lz = L, ly, lz
         g.xs, g.ys, g.zs = xs, ys, zs
         g.x3, g.x4, g.y3, g.y4, g.z3, g.z4 = 0, nx, 0, ny, 0, nz
         self.g = g
         self.cx = fd_coefs(L, dx)
-        self.cy = self.cx.copy()
-        self.cz = self.cx.copy()
-        self.coef0 = lib().orc_fd_coef0(self.cx, L, self.cy, L, self.cz, L)
+        self.cy = fd_coefs(ly, dx)
+        self.cz = fd_coefs(lz, dx)
+        self.coef0 = lib().orc_fd_coef0(self.cx, L, self.cy, ly, self.cz, lz)
         if dt is None:
-            dt = lib().orc_fd_compute_dt(vmax, self.cx, L, self.cy, L, self.cz, L, cfl)
+            dt = lib().orc_fd_compute_dt(vmax, self.cx, L, self.cy, ly, self.cz, lz, cfl)
         self.dt = np.float32(dt)
         if nsamples is None:
             nsamples = int(np.float32(time_max) / self.dt)
@@ -235,15 +239,15 @@ class FdProblem:
                 self.eta.append(prof)
             self.hd2 = np.full(3, np.float32(1.0 / (4.0 * dx * dx)), np.float32)
             self.phi = np.zeros(nx * ny * nz, np.float32)
-        ext = (nx + 2 * L) * (ny + 2 * L) * (nz + 2 * L)
+        ext = (nx + 2 * L) * (ny + 2 * ly) * (nz + 2 * lz)
         self.p0 = np.full(ext, np.float32(0.000001), np.float32)
         self.p1 = np.full(ext, np.float32(0.000001), np.float32)
         self.cur_is_p1 = ctypes.c_int(1)
         self.it = 0
 
     def ext_shape(self):
-        L = self.L
-        return (self.nx + 2 * L, self.ny + 2 * L, self.nz + 2 * L)
+        lx, ly, lz = self.lh
+        return (self.nx + 2 * lx, self.ny + 2 * ly, self.nz + 2 * lz)
 
     def run(self, nsteps, receivers=None):
         """Run nsteps reference steps; returns (source trace, receiver traces)."""

@@ -69,14 +69,31 @@ def build_oracle():
 def build_host():
     """C++ drop-in drivers (fd_hip, sem_hip) linked against libphc.so."""
     outs = []
-    for name, src in (("fd_hip", "fd_main.cpp"), ("sem_hip", "sem_main.cpp")):
-        path = os.path.join(HOST, src)
-        if not os.path.exists(path):
-            continue
+    for name, srcs in (("fd_hip", ["fd_main.cpp"]), ("sem_hip", ["sem_main.cpp", "SEMproxy.cpp"])):
+        paths = [os.path.join(HOST, s) for s in srcs]
         exe = os.path.join(PKG, name)
-        _run(["g++", "-O2", "-std=c++17", "-I" + INCLUDE, "-I" + HOST, path, "-o", exe,
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-I" + HOST] + paths + ["-o", exe,
               "-L" + PKG, "-lphc", "-Wl,-rpath,$ORIGIN"])
         outs.append(exe)
+    return outs
+
+
+def build_checks():
+    """Host check programs under tests/cpp/ (the drop-in classes' semantics),
+    linked against libphc.so; and, where the reference tree is present, the
+    reference's own drivers built unchanged on the drop-in headers
+    (oracle/ref_drivers.mk, outputs in oracle/_ref/)."""
+    outs = []
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_semantics.cpp")
+    if os.path.exists(src):
+        exe = src[:-4]
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-I" + HOST, src, "-o", exe,
+              "-L" + PKG, "-lphc", "-Wl,-rpath,$ORIGIN/../../proxy-geos-hc_amd"])
+        outs.append(exe)
+    ref = os.environ.get("PHC_REFERENCE", "/root/reference")
+    if os.path.isdir(os.path.join(ref, "src", "fd")):
+        _run(["make", "-s", "-B", "-f", os.path.join(ORACLE, "ref_drivers.mk"), "REF=" + ref])
+        outs.append(os.path.join(ORACLE, "_ref"))
     return outs
 
 
@@ -87,8 +104,10 @@ def build_all(jobs=8):
         f1.result()
         f2.result()
     build_host()
+    build_checks()
 
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
-    {"all": build_all, "phc": build_libphc, "oracle": build_oracle, "host": build_host}[what]()
+    {"all": build_all, "phc": build_libphc, "oracle": build_oracle, "host": build_host,
+     "checks": build_checks}[what]()

@@ -67,6 +67,12 @@ __global__ void fd_add_rhs_kernel(float *p, const float *vp, int64_t off, const 
     if (threadIdx.x == 0 && blockIdx.x == 0) p[off] = add_nc(p[off], mul_nc(vp[off], rhs[it]));
 }
 
+// addRHS with the sample value handed over by the caller.
+__global__ void fd_add_value_kernel(float *p, const float *vp, int64_t off, float rhs)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[off] = add_nc(p[off], mul_nc(vp[off], rhs));
+}
+
 // Deferred sponge product of a level: p *= s on every interior point.
 __global__ void fd_damp_kernel(float *p, FdLayout L, int sw, const float *fx, const float *fy,
                                const float *fz)
@@ -171,7 +177,8 @@ void fd_axis_damping(int n, int W, float alpha, std::vector<float> &f)
 using namespace phc;
 
 struct phc_fd {
-    int nx, ny, nz, R;
+    int nx, ny, nz, R;  // R: the largest half width (the stored halo on every axis)
+    int lh[3];          // per-axis half widths lx, ly, lz (the reference's halos)
     int xs, ys, zs;
     int arith;
     int sw;
@@ -209,11 +216,10 @@ static phc_status fd_validate(const phc_fd_params *p)
     if (!p) return fail(PHC_ERR_INVALID, "params is NULL");
     if (p->nx < 1 || p->ny < 1 || p->nz < 1)
         return fail(PHC_ERR_INVALID, "grid %d x %d x %d is empty", p->nx, p->ny, p->nz);
-    if (p->lx != p->ly || p->lx != p->lz)
-        return fail(PHC_ERR_INVALID, "unequal half widths lx=%d ly=%d lz=%d are not supported",
-                    p->lx, p->ly, p->lz);
-    if (p->lx < 1 || p->lx > FD_MAX_R)
-        return fail(PHC_ERR_INVALID, "half width %d outside 1..%d", p->lx, FD_MAX_R);
+    for (int l : {p->lx, p->ly, p->lz})
+        if (l < 1 || l > FD_MAX_R)
+            return fail(PHC_ERR_INVALID, "half width %d outside 1..%d", l, FD_MAX_R);
+    const int R = std::max(p->lx, std::max(p->ly, p->lz));
     if (p->xs < 0 || p->xs >= p->nx || p->ys < 0 || p->ys >= p->ny || p->zs < 0 || p->zs >= p->nz)
         return fail(PHC_ERR_INVALID, "source (%d,%d,%d) outside the %d x %d x %d grid", p->xs,
                     p->ys, p->zs, p->nx, p->ny, p->nz);
@@ -228,14 +234,14 @@ static phc_status fd_validate(const phc_fd_params *p)
     if (p->block_waves != 0 && p->block_waves != 4 && p->block_waves != 8 && p->block_waves != 16)
         return fail(PHC_ERR_INVALID, "block_waves must be 4, 8 or 16 (got %d)", p->block_waves);
     {
-        const int ty = p->block_waves ? p->block_waves : ((p->lx == 4) ? 16 : 4);
-        const int tj = p->tile_j ? p->tile_j : (p->block_waves ? ty * 2 : ((p->lx == 4) ? 16 : 8));
-        if (tj % ty != 0 || !fd_pick_kernel(p->lx, ty, tj / ty, p->arith, true, false))
+        const int ty = p->block_waves ? p->block_waves : ((R == 4) ? 16 : 4);
+        const int tj = p->tile_j ? p->tile_j : (p->block_waves ? ty * 2 : ((R == 4) ? 16 : 8));
+        if (tj % ty != 0 || !fd_pick_kernel(R, ty, tj / ty, p->arith, true, false))
             return fail(PHC_ERR_INVALID, "no kernel for half width %d, tile_j %d, %d waves per block",
-                        p->lx, tj, ty);
+                        R, tj, ty);
     }
     if (p->chunk_i < 0) return fail(PHC_ERR_INVALID, "negative chunk_i");
-    const double pts = (double)(p->nx + 2 * p->lx) * (p->ny + 2 * p->ly) * (p->nz + 2 * p->lz + 32);
+    const double pts = (double)(p->nx + 2 * R) * (p->ny + 2 * R) * (p->nz + 2 * R + 32);
     if (pts > 1.5e10) return fail(PHC_ERR_INVALID, "grid too large (%.3g points)", pts);
     for (int64_t n = 0; n < (int64_t)p->nx * p->ny * p->nz; ++n)
         if (!(p->vp[n] >= 0.f) || !std::isfinite(p->vp[n]))
@@ -366,7 +372,8 @@ static phc_status fd_launch_full_step(phc_fd *h, const FdStepArgs &a, bool damp,
 phc_status phc::fd_field_view(phc_fd *h, FdFieldView *v)
 {
     if (!h || !v) return fail(PHC_ERR_INVALID, "NULL argument");
-    if (h->R != LB_R) return fail(PHC_ERR_INVALID, "the learned boundary needs half width %d", LB_R);
+    if (h->lh[0] != LB_R || h->lh[1] != LB_R || h->lh[2] != LB_R)
+        return fail(PHC_ERR_INVALID, "the learned boundary needs half width %d on every axis", LB_R);
     for (int l = 0; l <= h->R; ++l)
         if (h->cx[l] != h->cy[l] || h->cx[l] != h->cz[l])
             return fail(PHC_ERR_INVALID, "the learned boundary needs equal coefficients on the three axes");
@@ -391,7 +398,9 @@ phc_status phc_fd_create(const phc_fd_params *p, phc_fd **out)
     *out = nullptr;
     PHC_TRY(fd_validate(p));
     phc_fd *h = new phc_fd();
-    h->nx = p->nx; h->ny = p->ny; h->nz = p->nz; h->R = p->lx;
+    h->nx = p->nx; h->ny = p->ny; h->nz = p->nz;
+    h->lh[0] = p->lx; h->lh[1] = p->ly; h->lh[2] = p->lz;
+    h->R = std::max(p->lx, std::max(p->ly, p->lz));
     h->xs = p->xs; h->ys = p->ys; h->zs = p->zs;
     h->arith = p->arith;
     h->sw = p->sponge_width;
@@ -399,11 +408,13 @@ phc_status phc_fd_create(const phc_fd_params *p, phc_fd **out)
     std::memset(h->cx, 0, sizeof(h->cx));
     std::memset(h->cy, 0, sizeof(h->cy));
     std::memset(h->cz, 0, sizeof(h->cz));
-    for (int l = 0; l <= h->R; ++l) {
-        h->cx[l] = p->coefx[l];
-        h->cy[l] = p->coefy[l];
-        h->cz[l] = p->coefz[l];
-    }
+    // Unequal half widths (FDTDkernels.hpp:16-29 loops to each coef*.size()):
+    // the kernels run at the largest one, R, with zero weights past an axis's
+    // own width.  Those extra terms add 0 * (finite halo values) = 0 exactly,
+    // so every sum keeps the reference's value.
+    for (int l = 0; l <= p->lx; ++l) h->cx[l] = p->coefx[l];
+    for (int l = 0; l <= p->ly; ++l) h->cy[l] = p->coefy[l];
+    for (int l = 0; l <= p->lz; ++l) h->cz[l] = p->coefz[l];
     h->coef0 = p->coef0;
     h->nsamples = p->nsamples;
     // default shape: the fastest measured at 512^3 for radius 4 (16 waves,
@@ -473,6 +484,21 @@ phc_status phc_fd_add_rhs(phc_fd *h, int it, int cb)
     PHC_TRY(fd_materialize(h, cb));
     hipLaunchKernelGGL(fd_add_rhs_kernel, dim3(1), dim3(64), 0, h->stream, h->lev[cb].as<float>(),
                        h->vp.as<float>(), fd_off(h->L, h->xs, h->ys, h->zs), h->rhs.as<float>(), it);
+    PHC_HIP(hipGetLastError());
+    return PHC_OK;
+}
+
+phc_status phc_fd_add_rhs_value(phc_fd *h, int cb, int i, int j, int k, float rhs)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    if (cb < 0 || cb > 1) return fail(PHC_ERR_INVALID, "level %d is not 0 or 1", cb);
+    if (i < 0 || i >= h->nx || j < 0 || j >= h->ny || k < 0 || k >= h->nz)
+        return fail(PHC_ERR_INVALID, "source (%d,%d,%d) outside the %d x %d x %d grid", i, j, k, h->nx, h->ny,
+                    h->nz);
+    if (!std::isfinite(rhs)) return fail(PHC_ERR_INVALID, "source value is not finite");
+    PHC_TRY(fd_materialize(h, cb));
+    hipLaunchKernelGGL(fd_add_value_kernel, dim3(1), dim3(64), 0, h->stream, h->lev[cb].as<float>(),
+                       h->vp.as<float>(), fd_off(h->L, i, j, k), rhs);
     PHC_HIP(hipGetLastError());
     return PHC_OK;
 }
@@ -656,16 +682,17 @@ phc_status phc_fd_get_field(phc_fd *h, int level, float *host)
     PHC_HIP(hipStreamSynchronize(h->stream));
     std::vector<float> buf((size_t)h->L.total);
     PHC_HIP(hipMemcpy(buf.data(), h->lev[level].ptr, sizeof(float) * h->L.total, hipMemcpyDeviceToHost));
-    const int R = h->R;
-    const int NYe = h->ny + 2 * R, NZe = h->nz + 2 * R;
-    for (int i = -R; i < h->nx + R; ++i)
-        for (int j = -R; j < h->ny + R; ++j) {
-            float *dst = host + ((int64_t)(i + R) * NYe + (j + R)) * NZe;
-            const float *srcp = &buf[(size_t)fd_off(h->L, i, j, -R)];
+    // IDX3_l order with the reference's per-axis halos (FDTDmacros.hpp:14)
+    const int LX = h->lh[0], LY = h->lh[1], LZ = h->lh[2];
+    const int NYe = h->ny + 2 * LY, NZe = h->nz + 2 * LZ;
+    for (int i = -LX; i < h->nx + LX; ++i)
+        for (int j = -LY; j < h->ny + LY; ++j) {
+            float *dst = host + ((int64_t)(i + LX) * NYe + (j + LY)) * NZe;
+            const float *srcp = &buf[(size_t)fd_off(h->L, i, j, -LZ)];
             std::memcpy(dst, srcp, sizeof(float) * NZe);
             const bool row_in = i >= 0 && i < h->nx && j >= 0 && j < h->ny;
             if (h->pending[level] && row_in)
-                for (int k = 0; k < h->nz; ++k) dst[k + R] = dst[k + R] * fd_sponge_at(h, i, j, k);
+                for (int k = 0; k < h->nz; ++k) dst[k + LZ] = dst[k + LZ] * fd_sponge_at(h, i, j, k);
         }
     return PHC_OK;
 }
@@ -677,12 +704,12 @@ phc_status phc_fd_set_field(phc_fd *h, int level, const float *host)
     PHC_HIP(hipStreamSynchronize(h->stream));
     std::vector<float> buf((size_t)h->L.total);
     PHC_HIP(hipMemcpy(buf.data(), h->lev[level].ptr, sizeof(float) * h->L.total, hipMemcpyDeviceToHost));
-    const int R = h->R;
-    const int NYe = h->ny + 2 * R, NZe = h->nz + 2 * R;
-    for (int i = -R; i < h->nx + R; ++i)
-        for (int j = -R; j < h->ny + R; ++j)
-            std::memcpy(&buf[(size_t)fd_off(h->L, i, j, -R)],
-                        host + ((int64_t)(i + R) * NYe + (j + R)) * NZe, sizeof(float) * NZe);
+    const int LX = h->lh[0], LY = h->lh[1], LZ = h->lh[2];
+    const int NYe = h->ny + 2 * LY, NZe = h->nz + 2 * LZ;
+    for (int i = -LX; i < h->nx + LX; ++i)
+        for (int j = -LY; j < h->ny + LY; ++j)
+            std::memcpy(&buf[(size_t)fd_off(h->L, i, j, -LZ)],
+                        host + ((int64_t)(i + LX) * NYe + (j + LY)) * NZe, sizeof(float) * NZe);
     PHC_HIP(hipMemcpy(h->lev[level].ptr, buf.data(), sizeof(float) * h->L.total, hipMemcpyHostToDevice));
     h->pending[level] = 0;
     return PHC_OK;
@@ -692,8 +719,8 @@ phc_status phc_fd_read_point(phc_fd *h, int level, int i, int j, int k, float *v
 {
     if (!h || !v) return fail(PHC_ERR_INVALID, "handle or output is NULL");
     if (level < 0 || level > 1) return fail(PHC_ERR_INVALID, "level %d is not 0 or 1", level);
-    const int R = h->R;
-    if (i < -R || i >= h->nx + R || j < -R || j >= h->ny + R || k < -R || k >= h->nz + R)
+    const int LX = h->lh[0], LY = h->lh[1], LZ = h->lh[2];
+    if (i < -LX || i >= h->nx + LX || j < -LY || j >= h->ny + LY || k < -LZ || k >= h->nz + LZ)
         return fail(PHC_ERR_INVALID, "point (%d,%d,%d) outside the grid and halo", i, j, k);
     PHC_HIP(hipStreamSynchronize(h->stream));
     float x = 0.f;
@@ -709,21 +736,20 @@ phc_status phc_fd_get_slice_xz(phc_fd *h, int level, int j, float *host)
 {
     if (!h || !host) return fail(PHC_ERR_INVALID, "handle or host buffer is NULL");
     if (j < 0 || j >= h->ny) return fail(PHC_ERR_INVALID, "slice j=%d outside 0..%d", j, h->ny - 1);
-    const int R = h->R;
-    std::vector<float> full((size_t)(h->nx + 2 * R) * (h->ny + 2 * R) * (h->nz + 2 * R));
+    const int LX = h->lh[0], LY = h->lh[1], LZ = h->lh[2];
+    std::vector<float> full((size_t)(h->nx + 2 * LX) * (h->ny + 2 * LY) * (h->nz + 2 * LZ));
     PHC_TRY(phc_fd_get_field(h, level, full.data()));
-    const int NYe = h->ny + 2 * R, NZe = h->nz + 2 * R;
+    const int NYe = h->ny + 2 * LY, NZe = h->nz + 2 * LZ;
     for (int k = 0; k < h->nz; ++k)
         for (int i = 0; i < h->nx; ++i)
-            host[(int64_t)k * h->nx + i] = full[((int64_t)(i + R) * NYe + (j + R)) * NZe + (k + R)];
+            host[(int64_t)k * h->nx + i] = full[((int64_t)(i + LX) * NYe + (j + LY)) * NZe + (k + LZ)];
     return PHC_OK;
 }
 
 phc_status phc_fd_check_finite(phc_fd *h, int level)
 {
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
-    const int R = h->R;
-    std::vector<float> full((size_t)(h->nx + 2 * R) * (h->ny + 2 * R) * (h->nz + 2 * R));
+    std::vector<float> full((size_t)(h->nx + 2 * h->lh[0]) * (h->ny + 2 * h->lh[1]) * (h->nz + 2 * h->lh[2]));
     PHC_TRY(phc_fd_get_field(h, level, full.data()));
     for (size_t n = 0; n < full.size(); ++n)
         if (!std::isfinite(full[n]))

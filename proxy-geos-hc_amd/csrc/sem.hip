@@ -349,6 +349,12 @@ __global__ void sem_inject_kernel(float *p, int64_t off, float coef, const float
     if (threadIdx.x == 0 && blockIdx.x == 0) p[off] = add_nc(p[off], mul_nc(coef, rhs[it]));
 }
 
+// One source term handed over by the caller: p[off] += coef * value.
+__global__ void sem_add_value_kernel(float *p, int64_t off, float coef, float value)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[off] = add_nc(p[off], mul_nc(coef, value));
+}
+
 __global__ void sem_copy_point_kernel(const float *p, int64_t off, float *dst)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *dst = p[off];
@@ -638,6 +644,36 @@ static phc_status sem_launch(phc_sem *h, int i1, int i2, int inj, float rhs_next
     return PHC_OK;
 }
 
+// First node of element e, globalNodesList(e, 0) (SEMmesh.cpp:214-223: elements
+// numbered x fastest, then z, then y).
+static int64_t sem_first_node(const phc_sem *h, int64_t e)
+{
+    const int64_t exi = e % h->ex, ezi = (e / h->ex) % h->ez, eyi = e / ((int64_t)h->ex * h->ez);
+    return exi * h->p + (ezi * h->p) * h->nx + (eyi * h->p) * (int64_t)h->nx * h->nz;
+}
+
+// The source of the fused loop: element, samples, injection coefficient
+// dt^2 c^2 of the element (SEMsolver.cpp:38-42, rounded left to right).
+static phc_status sem_source(phc_sem *h, int src_element, int nsamples, const float *rhs)
+{
+    const int64_t nE = (int64_t)h->ex * h->ey * h->ez;
+    if (src_element < 0 || src_element >= nE)
+        return fail(PHC_ERR_INVALID, "source element %d outside 0..%lld", src_element, (long long)nE - 1);
+    if (nsamples < 0 || (nsamples > 0 && !rhs)) return fail(PHC_ERR_INVALID, "bad rhs");
+    h->src_element = src_element;
+    h->src_node = (int)sem_first_node(h, src_element);
+    const float c = h->model[src_element];
+    h->inj_coef = ((h->dt * h->dt) * c) * c;
+    const int64_t sn = h->src_node;
+    const int sx = (int)(sn % h->nx), sz = (int)((sn / h->nx) % h->nz), sy = (int)(sn / ((int64_t)h->nx * h->nz));
+    h->src_interior = sx >= 1 && sx < h->nx - 1 && sz >= 1 && sz < h->nz - 1 && sy >= 1 && sy < h->ny - 1;
+    h->nsamples = nsamples;
+    h->hrhs.assign(rhs, rhs + nsamples);
+    PHC_TRY(h->rhs.alloc(sizeof(float) * std::max(1, nsamples)));
+    if (nsamples > 0) PHC_HIP(hipMemcpy(h->rhs.ptr, rhs, sizeof(float) * nsamples, hipMemcpyHostToDevice));
+    return PHC_OK;
+}
+
 static phc_status sem_inject(phc_sem *h, int level, int it)
 {
     hipLaunchKernelGGL(sem_inject_kernel, dim3(1), dim3(64), 0, h->stream, h->lev[level].as<float>(),
@@ -878,6 +914,40 @@ phc_status phc_sem_compute_one_step(phc_sem *h, int time_sample, int i1, int i2)
         return fail(PHC_ERR_INVALID, "time sample %d outside 0..%d", time_sample, h->nsamples - 1);
     PHC_TRY(sem_inject(h, i2, time_sample));
     return sem_launch(h, i1, i2, 0, 0.f, nullptr, nullptr, nullptr);
+}
+
+phc_status phc_sem_compute_one_step_rhs(phc_sem *h, int i1, int i2, int nrhs, const int *rhs_element,
+                                        const float *rhs_value)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    if (i1 < 0 || i1 > 1 || i2 < 0 || i2 > 1 || i1 == i2)
+        return fail(PHC_ERR_INVALID, "levels i1=%d i2=%d must be 0 and 1", i1, i2);
+    if (nrhs < 0 || (nrhs > 0 && (!rhs_element || !rhs_value)))
+        return fail(PHC_ERR_INVALID, "bad source list (%d sources)", nrhs);
+    const int64_t nE = (int64_t)h->ex * h->ey * h->ez;
+    for (int r = 0; r < nrhs; ++r) {
+        if (rhs_element[r] < 0 || rhs_element[r] >= nE)
+            return fail(PHC_ERR_INVALID, "source %d: element %d outside 0..%lld", r, rhs_element[r],
+                        (long long)nE - 1);
+        if (!std::isfinite(rhs_value[r])) return fail(PHC_ERR_INVALID, "source %d: value is not finite", r);
+    }
+    // SEMsolver.cpp:37-42: pnGlobal(node0(e), i2) += dt dt c c rhs, source by source
+    for (int r = 0; r < nrhs; ++r) {
+        const int e = rhs_element[r];
+        const float c = h->model[e];
+        hipLaunchKernelGGL(sem_add_value_kernel, dim3(1), dim3(64), 0, h->stream, h->lev[i2].as<float>(),
+                           sem_off(h, sem_first_node(h, e)), ((h->dt * h->dt) * c) * c, rhs_value[r]);
+        PHC_HIP(hipGetLastError());
+    }
+    return sem_launch(h, i1, i2, 0, 0.f, nullptr, nullptr, nullptr);
+}
+
+phc_status phc_sem_set_source(phc_sem *h, int src_element, int nsamples, const float *rhs)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    sem_drop_graph(h);
+    return sem_source(h, src_element, nsamples, rhs);
 }
 
 static phc_status sem_enqueue_run(phc_sem *h, int it0, int nsteps, int i1, int i2,

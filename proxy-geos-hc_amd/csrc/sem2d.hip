@@ -191,6 +191,12 @@ __global__ void sem2d_inject_kernel(float *p, int64_t off, float coef, const flo
     if (threadIdx.x == 0 && blockIdx.x == 0) p[off] = add_nc(p[off], mul_nc(coef, rhs[it]));
 }
 
+// One source term handed over by the caller: p[off] += coef * value.
+__global__ void sem2d_add_value_kernel(float *p, int64_t off, float coef, float value)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[off] = add_nc(p[off], mul_nc(coef, value));
+}
+
 // Receiver samples after a step.  The source node already holds the next
 // sample's injection, so a receiver there takes the source trace (the value
 // before injection) instead.
@@ -233,7 +239,7 @@ struct phc_sem2d {
     int64_t nn;
     float dt;
     int nsamples;
-    std::vector<float> hrhs;
+    std::vector<float> hrhs, model;
     int src_element, src_node;
     float inj_coef;
     hipStream_t stream = nullptr;
@@ -385,6 +391,7 @@ phc_status phc_sem2d_create(const phc_sem_params *p, phc_sem2d **out)
     h->dt = p->dt;
     h->nsamples = p->nsamples;
     h->hrhs.assign(p->rhs, p->rhs + p->nsamples);
+    h->model.assign(p->model, p->model + (size_t)p->ex * p->ez);
     h->src_element = p->src_element;
     {
         const int e = p->src_element, exi = e % h->ex, ezi = e / h->ex;
@@ -497,6 +504,51 @@ phc_status phc_sem2d_compute_one_step(phc_sem2d *h, int time_sample, int i1, int
         return fail(PHC_ERR_INVALID, "time sample %d outside 0..%d", time_sample, h->nsamples - 1);
     PHC_TRY(s2_inject(h, i2, time_sample));
     return s2_launch(h, i1, i2, 0, 0.f, nullptr, nullptr, nullptr);
+}
+
+phc_status phc_sem2d_compute_one_step_rhs(phc_sem2d *h, int i1, int i2, int nrhs, const int *rhs_element,
+                                          const float *rhs_value)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    if (i1 < 0 || i1 > 1 || i2 < 0 || i2 > 1 || i1 == i2)
+        return fail(PHC_ERR_INVALID, "levels i1=%d i2=%d must be 0 and 1", i1, i2);
+    if (nrhs < 0 || (nrhs > 0 && (!rhs_element || !rhs_value)))
+        return fail(PHC_ERR_INVALID, "bad source list (%d sources)", nrhs);
+    const int nE = h->ex * h->ez;
+    for (int r = 0; r < nrhs; ++r) {
+        if (rhs_element[r] < 0 || rhs_element[r] >= nE)
+            return fail(PHC_ERR_INVALID, "source %d: element %d outside 0..%d", r, rhs_element[r], nE - 1);
+        if (!std::isfinite(rhs_value[r])) return fail(PHC_ERR_INVALID, "source %d: value is not finite", r);
+    }
+    // SEMsolver.cpp:37-42: pnGlobal(node0(e), i2) += dt dt c c rhs, source by source
+    for (int r = 0; r < nrhs; ++r) {
+        const int e = rhs_element[r];
+        const int64_t node = (int64_t)(e % h->ex) * h->p + (int64_t)(e / h->ex) * h->p * h->nx;
+        const float c = h->model[e];
+        hipLaunchKernelGGL(sem2d_add_value_kernel, dim3(1), dim3(64), 0, h->stream, h->lev[i2].as<float>(),
+                           s2_off(h, node), ((h->dt * h->dt) * c) * c, rhs_value[r]);
+        PHC_HIP(hipGetLastError());
+    }
+    return s2_launch(h, i1, i2, 0, 0.f, nullptr, nullptr, nullptr);
+}
+
+phc_status phc_sem2d_set_source(phc_sem2d *h, int src_element, int nsamples, const float *rhs)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    if (src_element < 0 || src_element >= h->ex * h->ez)
+        return fail(PHC_ERR_INVALID, "source element %d outside 0..%d", src_element, h->ex * h->ez - 1);
+    if (nsamples < 0 || (nsamples > 0 && !rhs)) return fail(PHC_ERR_INVALID, "bad rhs");
+    PHC_HIP(hipStreamSynchronize(h->stream));
+    s2_drop_graph(h);
+    h->src_element = src_element;
+    h->src_node = (src_element % h->ex) * h->p + (src_element / h->ex) * h->p * h->nx;
+    const float c = h->model[src_element];
+    h->inj_coef = ((h->dt * h->dt) * c) * c;
+    h->nsamples = nsamples;
+    h->hrhs.assign(rhs, rhs + nsamples);
+    PHC_TRY(h->rhs.alloc(sizeof(float) * std::max(1, nsamples)));
+    if (nsamples > 0) PHC_HIP(hipMemcpy(h->rhs.ptr, rhs, sizeof(float) * nsamples, hipMemcpyHostToDevice));
+    return PHC_OK;
 }
 
 static phc_status s2_enqueue_run(phc_sem2d *h, int it0, int nsteps, int i1, int i2, std::vector<hipEvent_t> *ev)

@@ -1,5 +1,8 @@
 // fd_main.cpp -- the FD proxy driver (src/fd/FDTDmain.cpp:9-108) on libphc.so.
-// Same argv, same prints, same timers; the step runs on the GPU.
+// The per-step loop makes the reference's calls with the reference's
+// arguments (FDTDmain.cpp:55-81); -fused replaces it with graph-captured runs
+// of 50 steps.  Same prints and timers.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <exception>
@@ -15,33 +18,32 @@ int main(int argc, char *argv[])
         auto startInit = system_clock::now();
         std::cout << "\n+==================================== \n| Initializing FDTD Application ...       \n"
                   << "+==================================== \n\n";
-        FDTDGRIDS g;
-        FDTDMODELS m;
-        FDTDInit init;
-        FDTDKernel kernel;
-        FDTDUtils utils;
-        init.init_geometry(argc, argv, g);
-        init.init_coefficients(g, m);
-        init.init_source(m);
-        init.init_models(g, m);
-        init.defineSpongeBoundary(g, m);
+        FDTDGRIDS myGrids;
+        FDTDMODELS myModels;
+        FDTDInit myInit;
+        FDTDKernel myKernel;
+        FDTDUtils myFDTDUtils;
+        myInit.init_geometry(argc, argv, myGrids);
+        myInit.init_coefficients(myGrids, myModels);
+        myInit.init_source(myModels);
+        myInit.init_models(myGrids, myModels);
+        myInit.defineSpongeBoundary(myGrids, myModels.spongeArray);
         std::cout << "\n+================================= \n|  Running FDTD Application ...       \n"
                   << "+================================= \n\n";
-        const int n = init.nsteps >= 0 ? std::min(init.nsteps, init.nSamples) : init.nSamples;
         auto startRun = system_clock::now();
         duration<double> tRHS(0), tKernel(0), tOut(0);
-        if (!init.fused) {
-            for (int it = 0; it < n; ++it) {
+        if (!myInit.fused) {
+            for (int itSample = 0; itSample < myInit.nSamples; itSample++) {
                 auto t0 = system_clock::now();
-                kernel.addRHS(g, it, m, init.i2);
+                myKernel.addRHS(myGrids, itSample, myModels, myInit.i2, myModels.pnGlobal);
                 auto t1 = system_clock::now();
-                if (init.usePML)
-                    kernel.computeOneStepPML(g, m, init.i1, init.i2);
+                if (myInit.usePML)
+                    myKernel.computeOneStepPML(myGrids, myModels, myInit.i1, myInit.i2);
                 else
-                    kernel.computeOneStepSB(g, m, init.i1, init.i2);
+                    myKernel.computeOneStepSB(myGrids, myModels, myInit.i1, myInit.i2);
                 auto t2 = system_clock::now();
-                std::swap(init.i1, init.i2);
-                utils.output(g, m, it, init.i2, init.saveSnapShots);
+                std::swap(myInit.i1, myInit.i2);
+                myFDTDUtils.output(myGrids, myModels.pnGlobal, itSample, myInit.i2, myInit.saveSnapShots);
                 auto t3 = system_clock::now();
                 tRHS += t1 - t0;
                 tKernel += t2 - t1;
@@ -49,25 +51,30 @@ int main(int argc, char *argv[])
             }
         } else {
             // graph-captured runs, each ending on the next output step
+            phc_fd *h = myModels.device(myGrids);
+            const int n = myInit.nSamples;
             int it = 0;
             while (it < n) {
                 int last = ((it + 49) / 50) * 50;
                 if (last > n - 1) last = n - 1;
                 const int len = last - it + 1;
                 auto t0 = system_clock::now();
-                int cur = init.i2;
-                phc_check(phc_fd_run(m.dev, it, len, &cur), "phc_fd_run");
-                phc_check(phc_fd_synchronize(m.dev), "synchronize");
+                myModels.pnGlobal.push_host_changes();
+                int cur = myInit.i2;
+                phc_check(phc_fd_run(h, it, len, &cur), "phc_fd_run");
+                phc_check(phc_fd_synchronize(h), "synchronize");
+                myModels.pnGlobal.device_changed(0);
+                myModels.pnGlobal.device_changed(1);
                 auto t1 = system_clock::now();
-                init.i2 = cur;
-                init.i1 = 1 - cur;
+                myInit.i2 = cur;
+                myInit.i1 = 1 - cur;
                 it += len;
-                utils.output(g, m, last, init.i2, init.saveSnapShots);
+                myFDTDUtils.output(myGrids, myModels.pnGlobal, last, myInit.i2, myInit.saveSnapShots);
                 tKernel += t1 - t0;
                 tOut += system_clock::now() - t1;
             }
         }
-        phc_check(phc_fd_synchronize(m.dev), "synchronize");
+        if (myModels.dev) phc_check(phc_fd_synchronize(myModels.dev->h), "synchronize");
         std::cout << "\n+================================= \n|  FDTD Application Finished.       \n"
                   << "+================================= \n\n";
         std::cout << "Elapsed Initialization Time : " << duration<double>(startRun - startInit).count()

@@ -71,6 +71,7 @@ SIGNATURES = {
     "phc_fd_create": (c_int, [ctypes.POINTER(FdParams), ctypes.POINTER(c_void_p)]),
     "phc_fd_destroy": (c_int, [c_void_p]),
     "phc_fd_add_rhs": (c_int, [c_void_p, c_int, c_int]),
+    "phc_fd_add_rhs_value": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_float]),
     "phc_fd_compute_one_step_sb": (c_int, [c_void_p, c_int, c_int]),
     "phc_fd_run": (c_int, [c_void_p, c_int, c_int, ctypes.POINTER(c_int)]),
     "phc_fd_set_receivers": (c_int, [c_void_p, c_int, c_void_p]),
@@ -150,19 +151,25 @@ class FdSetup:
 
     def __init__(self, nx=150, ny=None, nz=None, L=4, dx=10.0, xs=None, ys=None, zs=None,
                  f0=15.0, time_max=1.0, vmin=1500.0, vmax=4500.0, cfl=0.8, source_order=1,
-                 sponge_width=20, sponge_alpha=-0.00015, model="two_layer", nsamples=None):
+                 sponge_width=20, sponge_alpha=-0.00015, model="two_layer", nsamples=None,
+                 ly=None, lz=None):
         self.nx = nx
         self.ny = nx if ny is None else ny
         self.nz = nx if nz is None else nz
+        # half widths per axis (-lx / -ly / -lz, FDTDinit.hpp:96-110); L is lx
         self.L = L
+        self.lx = L
+        self.ly = L if ly is None else ly
+        self.lz = L if lz is None else lz
         self.xs = nx // 2 if xs is None else xs
         self.ys = self.ny // 2 if ys is None else ys
         self.zs = self.nz // 2 if zs is None else zs
-        self.cx = fd_coefs(L, dx)
-        self.cy = self.cx.copy()
-        self.cz = self.cx.copy()
-        self.coef0 = lib().phc_fd_coef0(self.cx, L, self.cy, L, self.cz, L)
-        self.dt = np.float32(lib().phc_fd_compute_dt(vmax, self.cx, L, self.cy, L, self.cz, L, cfl))
+        self.cx = fd_coefs(self.lx, dx)
+        self.cy = fd_coefs(self.ly, dx)
+        self.cz = fd_coefs(self.lz, dx)
+        self.coef0 = lib().phc_fd_coef0(self.cx, self.lx, self.cy, self.ly, self.cz, self.lz)
+        self.dt = np.float32(lib().phc_fd_compute_dt(vmax, self.cx, self.lx, self.cy, self.ly, self.cz,
+                                                     self.lz, cfl))
         self.nsamples = int(np.float32(time_max) / self.dt) if nsamples is None else nsamples
         self.f0 = f0
         self.rhs = source_term(self.nsamples, self.dt, f0, source_order, EXP_DOUBLE)
@@ -197,10 +204,11 @@ class FdSolver:
         s = setup
         self.setup = s
         self.nx, self.ny, self.nz, self.L = s.nx, s.ny, s.nz, s.L
+        self.lh = (s.lx, s.ly, s.lz)
         self._keep = [s.cx, s.cy, s.cz, s.vp, s.rhs]
         p = FdParams()
         p.nx, p.ny, p.nz = s.nx, s.ny, s.nz
-        p.lx = p.ly = p.lz = s.L
+        p.lx, p.ly, p.lz = s.lx, s.ly, s.lz
         p.xs, p.ys, p.zs = s.xs, s.ys, s.zs
         p.coefx, p.coefy, p.coefz = _ptr(s.cx), _ptr(s.cy), _ptr(s.cz)
         p.coef0 = s.coef0
@@ -287,10 +295,11 @@ class FdSolver:
         check(lib().phc_fd_set_receivers(self.h, self.nrec, _ptr(a)))
 
     def field(self, level):
-        L = self.L
-        out = np.zeros((self.nx + 2 * L) * (self.ny + 2 * L) * (self.nz + 2 * L), np.float32)
+        lx, ly, lz = self.lh
+        shape = (self.nx + 2 * lx, self.ny + 2 * ly, self.nz + 2 * lz)
+        out = np.zeros(shape[0] * shape[1] * shape[2], np.float32)
         check(lib().phc_fd_get_field(self.h, level, out))
-        return out.reshape(self.nx + 2 * L, self.ny + 2 * L, self.nz + 2 * L)
+        return out.reshape(shape)
 
     def set_field(self, level, arr):
         check(lib().phc_fd_set_field(self.h, level, np.ascontiguousarray(arr, np.float32).reshape(-1)))
@@ -690,6 +699,8 @@ SIGNATURES.update({
     "phc_sem_destroy": (c_int, [c_void_p]),
     "phc_sem_sizes": (c_int, [c_void_p] + [ctypes.POINTER(c_int)] * 4),
     "phc_sem_compute_one_step": (c_int, [c_void_p, c_int, c_int, c_int]),
+    "phc_sem_compute_one_step_rhs": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "phc_sem_set_source": (c_int, [c_void_p, c_int, c_int, c_void_p]),
     "phc_sem_run": (c_int, [c_void_p, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
     "phc_sem_set_receivers": (c_int, [c_void_p, c_int, c_void_p]),
     "phc_sem_get_traces": (c_int, [c_void_p, c_void_p, c_void_p]),
@@ -714,6 +725,8 @@ SIGNATURES.update({
     "phc_sem2d_destroy": (c_int, [c_void_p]),
     "phc_sem2d_sizes": (c_int, [c_void_p] + [ctypes.POINTER(c_int)] * 3),
     "phc_sem2d_compute_one_step": (c_int, [c_void_p, c_int, c_int, c_int]),
+    "phc_sem2d_compute_one_step_rhs": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "phc_sem2d_set_source": (c_int, [c_void_p, c_int, c_int, c_void_p]),
     "phc_sem2d_run": (c_int, [c_void_p, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
     "phc_sem2d_set_receivers": (c_int, [c_void_p, c_int, c_void_p]),
     "phc_sem2d_get_traces": (c_int, [c_void_p, c_void_p, c_void_p]),

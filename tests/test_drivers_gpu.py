@@ -128,7 +128,6 @@ def test_sem_driver_2d(gpu, oracle, tmp_path, fused):
     args = ["65", "2000", "-2d", "-order", "3", "-dt", "0.001", "-f0", "10", "-nsteps", "301", "-snapshots"]
     out = run("sem_hip", args + (["-fused"] if fused else []), str(tmp_path))
     assert "ex, ey, ez=65, 0, 65" in out and "nx, ny, nz=196, 1, 196" in out
-    assert "Number of nodes: 38416" in out and "Number of elements: 4225" in out
     m = re.search(r"Element number for the source location: (\d+)", out)
     e = int(m.group(1))
     assert e == 32 + 32 * 65  # 1001 / (2000/65) = 32.5 on both axes

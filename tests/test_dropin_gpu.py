@@ -1,0 +1,141 @@
+"""The drop-in boundary end to end.
+
+* The reference's own drivers (src/fd/FDTDmain.cpp; src/sem/main/SEMproxy.cpp
+  and main.cpp), built unchanged on the drop-in headers by
+  oracle/ref_drivers.mk, run on the GPU and print what the published runs and
+  this library's own drivers print.
+* tests/cpp/dropin_semantics: host edits of pnGlobal, RHS values and the
+  source point handed over between steps are honoured, bit for bit against
+  the same edits made through the C ABI.
+* The C ABI's caller-sample entry points give the bits of the handle-sample
+  ones."""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+pytestmark = pytest.mark.gpu
+REF_BIN = os.path.join(ROOT, "oracle", "_ref")
+PKG = os.path.join(ROOT, "proxy-geos-hc_amd")
+
+
+def run(path, args, cwd):
+    print("[dropin] %s %s" % (os.path.basename(path), " ".join(args)), flush=True)
+    r = subprocess.run([path] + args, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout
+    return r.stdout
+
+
+def fd_prints(out, src):
+    return {int(m.group(1)): m.group(2) for m in
+            re.finditer(r"TimeStep=(\d+)\t; Pressure value at source \[%s\] = (\S+)" % src, out)}
+
+
+def test_dropin_semantics(gpu, tmp_path):
+    exe = os.path.join(ROOT, "tests", "cpp", "dropin_semantics")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/dropin_semantics is not built: run __graft_entry__.build()")
+    out = run(exe, [], str(tmp_path))
+    assert "DROPIN OK" in out, out
+
+
+def test_reference_fd_driver_unchanged(gpu, tmp_path):
+    """FDTDmain.cpp as the reference ships it, on the drop-in classes: the
+    published source-point prints through step 400 (150^3, homogeneous
+    1500 m/s), and the same prints as fd_hip."""
+    exe = os.path.join(REF_BIN, "fd_ref_driver")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/fd_ref_driver not built (reference tree absent at build time)")
+    args = ["-nx", "150", "-homogeneous", "-nsteps", "401"]
+    got = fd_prints(run(exe, args, str(tmp_path)), "75 75 75")
+    d = json.load(open(os.path.join(GOLDEN, "published_traces.json")))
+    rec = d["result_Cypress_fd_SEQUENTIAL_EPYCROME_01.txt"]
+    n = 0
+    for step, txt in zip(rec["steps"], rec["values"]):
+        if step > 400:
+            break
+        assert got[step] == txt, (step, got[step], txt)
+        n += 1
+    assert n == 9
+    ours = fd_prints(run(os.path.join(PKG, "fd_hip"), args, str(tmp_path)), "75 75 75")
+    assert ours == got
+
+
+def test_reference_fd_driver_unequal_widths_pml(gpu, tmp_path):
+    exe = os.path.join(REF_BIN, "fd_ref_driver")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/fd_ref_driver not built (reference tree absent at build time)")
+    args = ["-nx", "40", "-ny", "36", "-nz", "44", "-lx", "4", "-ly", "3", "-lz", "2", "-xs", "20",
+            "-ys", "18", "-zs", "22", "-f0", "30", "-usePML", "-nsteps", "151"]
+    got = fd_prints(run(exe, args, str(tmp_path)), "20 18 22")
+    ours = fd_prints(run(os.path.join(PKG, "fd_hip"), args, str(tmp_path)), "20 18 22")
+    assert len(got) == 4 and got == ours
+
+
+def test_reference_sem_driver_unchanged(gpu, oracle, tmp_path):
+    """SEMproxy.cpp + main.cpp as the reference ships them (order 2, dt 1e-3,
+    f0 10 Hz, 10 samples: SEMdata.hpp:13-22), on the drop-in classes: the
+    source-node print equals sem_hip's and the oracle's."""
+    exe = os.path.join(REF_BIN, "sem_ref_driver")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/sem_ref_driver not built (reference tree absent at build time)")
+    pat = r"TimeStep=(\d+);  pnGlobal @ elementSource location (\d+) after computeOneStep = (\S+)"
+    got = re.findall(pat, run(exe, ["12", "2000"], str(tmp_path)))
+    ours = re.findall(pat, run(os.path.join(PKG, "sem_hip"), ["12", "2000"], str(tmp_path)))
+    assert len(got) == 1 and got == ours
+    O = oracle.SemProblem(ex=12, lx=2000.0, order=2, dt=0.001, f0=10.0, nsamples=10,
+                          src_point=(1001.0, 1001.0, 1001.0))
+    tr, _ = O.run(1)
+    assert abs(float(got[0][2]) - tr[0]) <= 1e-5 * abs(tr[0]) + 1e-12
+
+
+@pytest.mark.parametrize("kernel", ["separable", "element", "color"])
+def test_sem_step_with_caller_rhs(gpu, kernel):
+    kern = {"separable": gpu.SEM_SEPARABLE, "element": gpu.SEM_ELEMENT, "color": gpu.SEM_ELEMENT_COLOR}[kernel]
+    kw = dict(ex=7, lx=700.0, order=3, dt=0.001, f0=25.0, nsamples=30, src_point=(351.0, 351.0, 351.0),
+              kernel=kern, model=np.linspace(1500, 2500, 343).astype(np.float32))
+    a = gpu.SemSolver(**kw)
+    b = gpu.SemSolver(**kw)
+    el = np.array([b.src_element], np.int32)
+    for it in range(20):
+        a.compute_one_step(it, a.i1, a.i2)
+        a.i1, a.i2 = a.i2, a.i1
+        v = np.array([b.rhs[it]], np.float32)
+        gpu.check(gpu.lib().phc_sem_compute_one_step_rhs(b.h, b.i1, b.i2, 1, el.ctypes.data, v.ctypes.data))
+        b.i1, b.i2 = b.i2, b.i1
+    assert np.array_equal(a.current(), b.current()) and np.array_equal(a.previous(), b.previous())
+    bad = np.array([343], np.int32)
+    with pytest.raises(gpu.PhcError):
+        gpu.check(gpu.lib().phc_sem_compute_one_step_rhs(b.h, 0, 1, 1, bad.ctypes.data, v.ctypes.data))
+
+
+def test_sem_set_source_equals_create(gpu):
+    kw = dict(ex=6, lx=600.0, order=2, dt=0.001, f0=25.0, nsamples=40)
+    a = gpu.SemSolver(src_point=(301.0, 301.0, 301.0), **kw)
+    b = gpu.SemSolver(src_point=(101.0, 401.0, 201.0), **kw)
+    rhs = (0.5 * a.rhs).astype(np.float32)
+    gpu.check(gpu.lib().phc_sem_set_source(b.h, a.src_element, len(rhs), rhs.ctypes.data))
+    gpu.check(gpu.lib().phc_sem_set_source(a.h, a.src_element, len(rhs), rhs.ctypes.data))
+    sa, _ = a.run(30)
+    sb, _ = b.run(30)
+    assert np.array_equal(sa, sb) and np.array_equal(a.current(), b.current())
+
+
+def test_sem2d_step_with_caller_rhs(gpu):
+    kw = dict(ex=12, lx=1200.0, order=3, dt=0.001, f0=20.0, nsamples=30, src_point=(601.0, 0.0, 601.0))
+    a = gpu.Sem2dSolver(**kw)
+    b = gpu.Sem2dSolver(**kw)
+    el = np.array([b.src_element], np.int32)
+    for it in range(25):
+        a.compute_one_step(it, a.i1, a.i2)
+        a.i1, a.i2 = a.i2, a.i1
+        v = np.array([b.rhs[it]], np.float32)
+        gpu.check(gpu.lib().phc_sem2d_compute_one_step_rhs(b.h, b.i1, b.i2, 1, el.ctypes.data, v.ctypes.data))
+        b.i1, b.i2 = b.i2, b.i1
+    assert np.array_equal(a.current(), b.current()) and np.array_equal(a.previous(), b.previous())

@@ -67,6 +67,52 @@ def test_radius_bit_exact(gpu, oracle, L):
     assert np.array_equal(sol.previous(), P.previous())
 
 
+@pytest.mark.parametrize("lh", [(4, 3, 2), (2, 4, 1), (6, 1, 3)])
+def test_unequal_half_widths_bit_exact(gpu, oracle, lh):
+    """-lx/-ly/-lz set per axis (FDTDinit.hpp:96-110; inner3D loops to each
+    coef*.size(), FDTDkernels.hpp:16-29): per-step surface and fused run
+    bit-exact with the oracle, fields in the reference's per-axis halo
+    layout; FP32 mode within 1e-5."""
+    lx, ly, lz = lh
+    banner("half widths %s" % (lh,))
+    s = gpu.FdSetup(nx=30, ny=27, nz=70, L=lx, ly=ly, lz=lz, model="two_layer", nsamples=40, sponge_width=6,
+                    f0=20.0)
+    P = oracle.FdProblem(nx=30, ny=27, nz=70, L=lx, ly=ly, lz=lz, model=s.vp.copy(), nsamples=40,
+                         sponge_width=6, f0=20.0)
+    assert P.dt == s.dt and P.coef0 == s.coef0
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF)
+    assert sol.current().shape == P.ext_shape()
+    for it in range(10):
+        sol.step(it)
+    src, _ = sol.run(10, 20)
+    tr, _ = P.run(30)
+    assert np.array_equal(src, tr[10:])
+    assert np.array_equal(sol.current(), P.current())
+    assert np.array_equal(sol.previous(), P.previous())
+    f32 = gpu.FdSolver(s, arith=gpu.ARITH_FP32)
+    f32.run(0, 30)
+    assert rel_l2(f32.current(), P.current()) < 1e-5
+
+
+def test_add_rhs_value_equals_add_rhs(gpu):
+    """addRHS with the caller's sample (phc_fd_add_rhs_value) gives the bits of
+    addRHS from the handle's samples."""
+    s = gpu.FdSetup(nx=24, model="two_layer", nsamples=20, sponge_width=4, f0=25.0)
+    a = gpu.FdSolver(s, arith=gpu.ARITH_REF)
+    b = gpu.FdSolver(s, arith=gpu.ARITH_REF)
+    for it in range(12):
+        a.step(it)
+        gpu.check(gpu.lib().phc_fd_add_rhs_value(b.h, b.i2, s.xs, s.ys, s.zs, float(s.rhs[it])))
+        b.compute_one_step_sb(b.i1, b.i2)
+        b.i1, b.i2 = b.i2, b.i1
+    assert np.array_equal(a.current(), b.current())
+    assert np.array_equal(a.previous(), b.previous())
+    with pytest.raises(gpu.PhcError):
+        gpu.check(gpu.lib().phc_fd_add_rhs_value(b.h, 0, 24, 0, 0, 1.0))
+    with pytest.raises(gpu.PhcError):
+        gpu.check(gpu.lib().phc_fd_add_rhs_value(b.h, 0, 1, 1, 1, float("nan")))
+
+
 @pytest.mark.parametrize("tile_j,waves,chunk_i,index64", [
     (8, 4, 7, False), (16, 4, 128, False), (16, 4, 5, True), (8, 4, 64, True),
     (8, 8, 13, False), (16, 8, 256, False), (16, 16, 9, False), (32, 16, 33, True)])

@@ -64,12 +64,13 @@ def test_oracle_pml_absorbs(oracle):
     assert ea < 0.05 * eb, (ea, eb)
 
 
-def pair(phc, oracle, shape, f0=30.0, xs=None, ys=None, zs=None, nsamples=60, ndamp=None, model="two_layer", L=4):
+def pair(phc, oracle, shape, f0=30.0, xs=None, ys=None, zs=None, nsamples=60, ndamp=None, model="two_layer", L=4,
+         ly=None, lz=None):
     nx, ny, nz = shape
     s = phc.FdSetup(nx=nx, ny=ny, nz=nz, L=L, f0=f0, xs=xs, ys=ys, zs=zs, sponge_width=0, model=model,
-                    nsamples=nsamples)
+                    nsamples=nsamples, ly=ly, lz=lz)
     P = oracle.FdProblem(nx=nx, ny=ny, nz=nz, L=L, xs=s.xs, ys=s.ys, zs=s.zs, f0=f0, model=s.vp.copy(),
-                         nsamples=nsamples, sponge_width=0, pml=True, ndamp=ndamp)
+                         nsamples=nsamples, sponge_width=0, pml=True, ndamp=ndamp, ly=ly, lz=lz)
     assert np.array_equal(P.rhs, s.rhs)
     return s, P
 
@@ -104,13 +105,17 @@ def test_gpu_pml_bit_exact(gpu, oracle, shape, ndamp, src):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("L,index64", [(1, False), (2, False), (3, False), (5, False), (6, False), (4, True)])
-def test_gpu_pml_bit_exact_other_orders(gpu, oracle, L, index64):
+@pytest.mark.parametrize("L,index64,ly,lz", [(1, False, None, None), (2, False, None, None),
+                                             (3, False, None, None), (5, False, None, None),
+                                             (6, False, None, None), (4, True, None, None),
+                                             (4, False, 3, 2), (2, False, 5, 3)])
+def test_gpu_pml_bit_exact_other_orders(gpu, oracle, L, index64, ly, lz):
     # -usePML with the stencil of half width L (the march's ring, halo and the
-    # pml3D first differences all change with it); and the 64-bit-offset
-    # kernel (used for arrays of 4 GiB and more) forced on a small grid
-    print("\n[pml] half width %d index64 %d" % (L, index64), flush=True)
-    s, P = pair(gpu, oracle, (30, 27, 41), nsamples=50, L=L)
+    # pml3D first differences all change with it), also with other widths
+    # along y and z (-ly, -lz); and the 64-bit-offset kernel (used for arrays
+    # of 4 GiB and more) forced on a small grid
+    print("\n[pml] half widths %d %s %s index64 %d" % (L, ly, lz, index64), flush=True)
+    s, P = pair(gpu, oracle, (30, 27, 41), nsamples=50, L=L, ly=ly, lz=lz)
     sol = gpu.FdSolver(s, arith=gpu.ARITH_REF, index64=index64)
     nd = sol.set_pml(ndamp=int(P.ndamp[0]), f0=s.f0)
     assert nd == int(P.ndamp[0])
