@@ -80,6 +80,10 @@ typedef struct {
     int chunk_i;       /* planes per block along the marching axis i */
     int index64;       /* 1: force 64-bit element offsets (default: 32-bit when
                         * every array is below 4 GiB) */
+    int k_remainder;   /* a last, partial 64-point tile along k: 0 (default)
+                        * updated point by point when it holds at most 32
+                        * points, as a whole tile otherwise; 1 always point by
+                        * point; -1 always as a tile.  Same values either way. */
 } phc_fd_params;
 
 phc_status phc_fd_create(const phc_fd_params *p, phc_fd **out);
@@ -206,6 +210,9 @@ typedef struct {
                                          ranges x 2^(m-1) column bands */
     int index64;           /* 1: 64-bit offsets in the separable and 2D kernels even
                               when every array is < 4 GiB (testing) */
+    int k_remainder;       /* separable kernel: a last, partial 64-node tile along x
+                              updated node by node when it holds at most 32 nodes (0,
+                              default), always (1), or marched as a tile (-1) */
 } phc_sem_params;
 
 phc_status phc_sem_create(const phc_sem_params *p, phc_sem **out);
@@ -351,6 +358,7 @@ typedef struct {
     int tile_j, block_waves;  /* 0 = default (16 x 16) */
     int chunk_i;              /* 0 = default (64) */
     int blocks_per_cu;        /* 0 = default (1) */
+    int k_remainder;          /* a shot's last, partial k tile: as in phc_fd_params */
 } phc_batch_params;
 
 phc_status phc_batch_create(const phc_batch_params *p, phc_batch **out);

@@ -173,6 +173,9 @@ void orc_sem_step(orc_sem *s, float *p_i1, float *p_i2, int src_element,
                   float rhs);
 /* Diagnostics of the last step (assembled FP32 M and y). */
 void orc_sem_get_my(const orc_sem *s, float *M, float *y);
+/* threads > 1: orc_sem_step runs the element loop colour by colour on that
+ * many OpenMP threads (no race; sums in colour order: for timing). */
+void orc_sem_set_threads(orc_sem *s, int threads);
 /* y = K p with the element loop of computeOneStep (no source, no update). */
 void orc_sem_apply_k(orc_sem *s, const float *p, float *y, float *M);
 

@@ -114,6 +114,8 @@ def _bind_sem(L):
     L.orc_sem_step.argtypes = [ctypes.c_void_p, f32p, f32p, c_int, c_float]
     L.orc_sem_get_my.restype = None
     L.orc_sem_get_my.argtypes = [ctypes.c_void_p, f32p, f32p]
+    L.orc_sem_set_threads.restype = None
+    L.orc_sem_set_threads.argtypes = [ctypes.c_void_p, c_int]
     L.orc_sem_apply_k.restype = None
     L.orc_sem_apply_k.argtypes = [ctypes.c_void_p, f32p, f32p, f32p]
     L.orc_sem64_create.restype = ctypes.c_void_p
@@ -339,6 +341,14 @@ class SemProblem:
         self.src_node = int(self.nodes[self.src_element, 0])
         self.i1, self.i2 = 0, 1
         self.it = 0
+
+    def set_threads(self, n):
+        """n > 1: the element loop colour by colour on n OpenMP threads (no
+        race; FP32 sums in colour order, so the last bits can differ from
+        the sequential loop: for timing)."""
+        if self.fp64:
+            raise ValueError("the FP64 oracle runs sequentially")
+        lib().orc_sem_set_threads(self.h, int(n))
 
     def __del__(self):
         try:

@@ -170,6 +170,7 @@ struct orc_sem {
     double w[16], D[256];
     float dt;
     float *M, *y;     /* [nN] */
+    int threads;      /* > 1: colour-parallel element loop (orc_sem_set_threads) */
 };
 
 /* SEMQkGLIntegrals.hpp:206-359 computeB V2 (3D) for one element */
@@ -381,6 +382,43 @@ static void assemble(orc_sem *s, const float *p, float *y, float *M)
     }
 }
 
+/* The element loop on several threads without the reference's race: the 8
+ * colours of (x, z, y) element index parities run one after the other, and
+ * the elements of one colour share no node, so their plain scatter adds
+ * never meet (the 3D form of SEM_MESHCOLOR, SEMmesh.cpp:380-429).  Each
+ * node's contributions arrive in colour order rather than element order, so
+ * the FP32 sums can differ from the sequential loop's in the last bits:
+ * a timing path (the CPU baseline), not the parity reference. */
+static void assemble_colours(orc_sem *s, const float *p, float *y, float *M)
+{
+    const int np = s->np, ex = s->m.ex, ey = s->m.ey, ez = s->m.ez;
+#pragma omp parallel for schedule(static) num_threads(s->threads)
+    for (int n = 0; n < s->nN; n++) { M[n] = 0; y[n] = 0; }
+    for (int c = 0; c < 8; c++) {
+        const int px = c & 1, pz = (c >> 1) & 1, py = (c >> 2) & 1;
+        const int nx = (ex - px + 1) / 2, nz = (ez - pz + 1) / 2, ny = (ey - py + 1) / 2;
+        const int ne = nx * nz * ny;
+#pragma omp parallel for schedule(static) num_threads(s->threads)
+        for (int t = 0; t < ne; t++) {
+            const int l = 2 * (t % nx) + px, m = 2 * ((t / nx) % nz) + pz, n = 2 * (t / (nx * nz)) + py;
+            const int e = l + m * ex + n * ex * ez;
+            float pl[216], Y[216];
+            const int *nl = s->nodes + (size_t)e * np;
+            for (int i = 0; i < np; i++) pl[i] = p[nl[i]];
+            grad_phi_grad_phi(s, s->B + (size_t)6 * e * np, pl, Y);
+            const float cc = s->model[e];
+            const float *ml = s->mloc + (size_t)e * np;
+            for (int i = 0; i < np; i++) {
+                const int g = nl[i];
+                M[g] += ml[i] / (cc * cc);
+                y[g] += Y[i];
+            }
+        }
+    }
+}
+
+void orc_sem_set_threads(orc_sem *s, int threads) { s->threads = threads; }
+
 void orc_sem_apply_k(orc_sem *s, const float *p, float *y, float *M) { assemble(s, p, y, M); }
 
 void orc_sem_step(orc_sem *s, float *p_i1, float *p_i2, int src_element, float rhs)
@@ -391,8 +429,12 @@ void orc_sem_step(orc_sem *s, float *p_i1, float *p_i2, int src_element, float r
         const float c = s->model[src_element];
         p_i2[node] += (((s->dt * s->dt) * c) * c) * rhs;
     }
-    assemble(s, p_i2, s->y, s->M);
+    if (s->threads > 1)
+        assemble_colours(s, p_i2, s->y, s->M);
+    else
+        assemble(s, p_i2, s->y, s->M);
     /* SEMsolver.cpp:85-89 */
+#pragma omp parallel for schedule(static) num_threads(s->threads > 1 ? s->threads : 1)
     for (int n = 0; n < s->nI; n++) {
         const int I = s->interior[n];
         p_i1[I] = (2 * p_i2[I] - p_i1[I]) - ((s->dt * s->dt) * s->y[I]) / s->M[I];

@@ -31,12 +31,16 @@ struct ShotDesc {
     const float *fy, *fz;
     const float *rhs;
     int64_t src_off;
+    int krem;         // first k of the shot's remainder slab (0: none)
 };
 
-// One unit of work: tile column (kt, jt) of i-chunk ic of one shot.
+// One unit of work: tile column (kt, jt) of i-chunk ic of one shot; kt < 0:
+// part jt of the shot's k-remainder slab, BATCH_SLAB_PASSES x the block's
+// threads points in slab order.
 struct WorkItem {
     int shot, kt, jt, ic;
 };
+constexpr int BATCH_SLAB_PASSES = 8;
 
 // The per-grid fields fd_tile reads, for one work item (scalar registers).
 struct FdGrid {
@@ -49,6 +53,7 @@ struct FdGrid {
     float rhs_next;
     float *trace;
     int ilo, ihi, jlo, jhi, klo, khi;  // every point of the shot
+    int krem;
 };
 
 // Work queues of one step: queue q holds items[start[q] .. start[q]+len[q]),
@@ -100,9 +105,16 @@ fd_batch_kernel(FdStepArgs c, const ShotDesc *__restrict__ shots, const WorkItem
             g.inj = inj;
             g.rhs_next = inj ? sd.rhs[it + 1] : 0.f;
             g.trace = traces + (int64_t)wi.shot * trace_stride + step;
-        g.ilo = 0; g.ihi = sd.L.nx;
-        g.jlo = 0; g.jhi = sd.L.ny;
-        g.klo = 0; g.khi = sd.L.nz;
+            g.ilo = 0; g.ihi = sd.L.nx;
+            g.jlo = 0; g.jhi = sd.L.ny;
+            g.klo = 0; g.khi = sd.L.nz;
+            g.krem = sd.krem;
+            if (wi.kt < 0) {
+                constexpr int T = FD_TK * TY;
+                for (int m = 0; m < BATCH_SLAB_PASSES; ++m)
+                    fd_rem_points<R, ARITH, IT>(c, g, ((int64_t)wi.jt * BATCH_SLAB_PASSES + m) * T, T);
+                continue;
+            }
             fd_tile<R, TY, NJ, ARITH, IT>(c, g, wi.kt, wi.jt, wi.ic, tile);
         }
     }
@@ -178,6 +190,7 @@ struct phc_batch {
     int pending = 0;  // the previous level still owes its sponge product
     hipStream_t stream = nullptr;
     int ty = 16, nj = 1, chunk_i = 64, blocks_per_cu = 1, cus = 256;
+    int krem_mode = 0;  // phc_batch_params.k_remainder
     int nitems = 0;
     BatchQueues queues;
     int nrec = 0;  // receivers per shot
@@ -196,16 +209,23 @@ static void batch_drop_graph(phc_batch *h)
 }
 
 // Work items of every shot for the current shape in spatial order (shot,
-// i-chunk, j-tile, k-tile), cut into BATCH_QUEUES contiguous runs of about
-// equal cost (planes incl. the 2R lead-in x rows).
+// i-chunk, j-tile, k-tile, then the shot's remainder slab parts), cut into
+// BATCH_QUEUES contiguous runs of about equal cost (planes incl. the 2R
+// lead-in x rows; a slab point costs a few tile points: it reads its stencil
+// from memory).  A shot whose last k tile would hold 1..32 points (mode 0;
+// any partial tile with mode 1, none with -1) gets the slab instead.
 static phc_status batch_build_items(phc_batch *h)
 {
     std::vector<WorkItem> items;
     std::vector<double> cost;
     const int TJ = h->ty * h->nj;
+    const int T = FD_TK * h->ty;
     for (int s = 0; s < h->nshots; ++s) {
         const FdLayout &L = h->L[s];
-        const int tk = (L.nz + FD_TK - 1) / FD_TK, tj = (L.ny + TJ - 1) / TJ;
+        const int full = L.nz / FD_TK, rem = L.nz - full * FD_TK;
+        const bool slab = h->krem_mode >= 0 && full > 0 && rem > 0 && (h->krem_mode == 1 || rem <= FD_TK / 2);
+        h->hdesc[s].krem = slab ? FD_TK * full : 0;
+        const int tk = slab ? full : (L.nz + FD_TK - 1) / FD_TK, tj = (L.ny + TJ - 1) / TJ;
         const int nc = (L.nx + h->chunk_i - 1) / h->chunk_i;
         for (int ic = 0; ic < nc; ++ic)
             for (int jt = 0; jt < tj; ++jt)
@@ -215,7 +235,16 @@ static phc_status batch_build_items(phc_batch *h)
                     const int rows = std::min(TJ, L.ny - jt * TJ);
                     cost.push_back((double)(planes + 2 * h->R) * rows);
                 }
+        if (slab) {
+            const int64_t pts = (int64_t)rem * L.ny * L.nx, per = (int64_t)BATCH_SLAB_PASSES * T;
+            for (int64_t q = 0; q * per < pts; ++q) {
+                items.push_back({s, -1, (int)q, 0});
+                cost.push_back(3.0 * (double)std::min(per, pts - q * per) / FD_TK);
+            }
+        }
     }
+    // the descriptors carry each shot's slab start
+    PHC_HIP(hipMemcpy(h->shots.ptr, h->hdesc.data(), sizeof(ShotDesc) * h->nshots, hipMemcpyHostToDevice));
     const double total = std::accumulate(cost.begin(), cost.end(), 0.0);
     int q = 0;
     double acc = 0.0;
@@ -324,6 +353,8 @@ phc_status phc_batch_create(const phc_batch_params *p, phc_batch **out)
     if (p->nsamples < 1) return fail(PHC_ERR_INVALID, "nsamples must be >= 1 (got %d)", p->nsamples);
     if (p->chunk_i < 0 || p->blocks_per_cu < 0 || p->blocks_per_cu > 8)
         return fail(PHC_ERR_INVALID, "bad chunk_i or blocks_per_cu");
+    if (p->k_remainder < -1 || p->k_remainder > 1)
+        return fail(PHC_ERR_INVALID, "k_remainder must be -1, 0 or 1 (got %d)", p->k_remainder);
     const int ty = p->block_waves ? p->block_waves : 16;
     const int tj = p->tile_j ? p->tile_j : ty;
     if (ty <= 0 || tj % ty != 0 || !batch_pick(p->half_width, ty, tj / ty, p->arith))
@@ -354,6 +385,7 @@ phc_status phc_batch_create(const phc_batch_params *p, phc_batch **out)
     h->nj = tj / ty;
     h->chunk_i = p->chunk_i ? p->chunk_i : 64;
     h->blocks_per_cu = p->blocks_per_cu ? p->blocks_per_cu : 1;
+    h->krem_mode = p->k_remainder;
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)

@@ -40,7 +40,11 @@ fd_step_kernel(FdStepArgs a)
 {
     using SH = FdTileShape<R, TY, NJ>;
     __shared__ float tile[2][SH::LH][SH::LW];
-    const int b = xcd_block(blockIdx.x, gridDim.x);
+    if (a.nb_main && (int)blockIdx.x >= a.nb_main) {
+        fd_rem_points<R, ARITH, IT>(a, a, (int64_t)(blockIdx.x - a.nb_main) * (FD_TK * TY), FD_TK * TY);
+        return;
+    }
+    const int b = xcd_block(blockIdx.x, a.nb_main ? a.nb_main : gridDim.x);
     const int kt = b % a.tiles_k;
     const int rest = b / a.tiles_k;
     fd_tile<R, TY, NJ, ARITH, IT>(a, a, kt, rest % a.tiles_j, rest / a.tiles_j, tile);
@@ -55,7 +59,11 @@ fd_step_pml_kernel(FdStepArgs a)
 {
     using SH = FdTileShape<R, TY, NJ>;
     __shared__ float tile[2][SH::LH][SH::LW];
-    const int b = xcd_block(blockIdx.x, gridDim.x);
+    if (a.nb_main && (int)blockIdx.x >= a.nb_main) {
+        fd_rem_points<R, ARITH, IT, true>(a, a, (int64_t)(blockIdx.x - a.nb_main) * (FD_TK * TY), FD_TK * TY);
+        return;
+    }
+    const int b = xcd_block(blockIdx.x, a.nb_main ? a.nb_main : gridDim.x);
     const int kt = b % a.tiles_k;
     const int rest = b / a.tiles_k;
     fd_tile<R, TY, NJ, ARITH, IT, true>(a, a, kt, rest % a.tiles_j, rest / a.tiles_j, tile);
@@ -192,6 +200,7 @@ struct phc_fd {
     int pending[2] = {0, 0};
     hipStream_t stream = nullptr;
     int ty = 4, nj = 2, chunk_i = 0, index64 = 0;
+    int krem_mode = 0;  // phc_fd_params.k_remainder
     int nrec = 0;
     std::vector<int64_t> rec_offs;
     int last_n = 0, last_nrec = 0;
@@ -241,6 +250,8 @@ static phc_status fd_validate(const phc_fd_params *p)
                         R, tj, ty);
     }
     if (p->chunk_i < 0) return fail(PHC_ERR_INVALID, "negative chunk_i");
+    if (p->k_remainder < -1 || p->k_remainder > 1)
+        return fail(PHC_ERR_INVALID, "k_remainder must be -1, 0 or 1 (got %d)", p->k_remainder);
     const double pts = (double)(p->nx + 2 * R) * (p->ny + 2 * R) * (p->nz + 2 * R + 32);
     if (pts > 1.5e10) return fail(PHC_ERR_INVALID, "grid too large (%.3g points)", pts);
     for (int64_t n = 0; n < (int64_t)p->nx * p->ny * p->nz; ++n)
@@ -323,6 +334,24 @@ static int fd_num_blocks(const FdStepArgs &a)
     return a.tiles_k * a.tiles_j * chunks;
 }
 
+// k-remainder slab (mode 0: when the box's last k tile would hold 1..32
+// points; 1: whenever it is partial; -1: never): the tiles stop at the last
+// full 64-lane tile and extra blocks update the slab point by point.
+// Returns the number of blocks to launch.
+static int fd_plan_blocks(FdStepArgs &a, int mode, int block_threads)
+{
+    a.nb_main = 0;
+    a.krem = 0;
+    const int full = a.khi / FD_TK, rem = a.khi - full * FD_TK;
+    const bool use = mode >= 0 && full > 0 && rem > 0 && FD_TK * full >= a.klo && (mode == 1 || rem <= FD_TK / 2);
+    if (!use) return fd_num_blocks(a);
+    a.tiles_k = full;
+    a.krem = FD_TK * full;
+    a.nb_main = fd_num_blocks(a);
+    const int64_t pts = (int64_t)rem * (a.jhi - a.jlo) * (a.ihi - a.ilo);
+    return a.nb_main + (int)((pts + block_threads - 1) / block_threads);
+}
+
 static phc_status fd_materialize(phc_fd *h, int level)
 {
     if (!h->pending[level]) return PHC_OK;
@@ -342,7 +371,7 @@ static phc_status fd_launch_step(phc_fd *h, FdStepArgs a, bool damp)
                                    !h->index64 && h->L.total * 4 < ((int64_t)1 << 32), h->pml_on != 0);
     if (!fn) return fail(PHC_ERR_INVALID, "no kernel for R=%d block_waves=%d rows/thread=%d", h->R, h->ty, h->nj);
     a.damp_prev = damp ? 1 : 0;
-    const int nb = fd_num_blocks(a);
+    const int nb = fd_plan_blocks(a, h->krem_mode, FD_TK * h->ty);
     hipLaunchKernelGGL(fn, dim3(nb), dim3(FD_TK, h->ty), 0, h->stream, a);
     PHC_HIP(hipGetLastError());
     return PHC_OK;
@@ -424,6 +453,7 @@ phc_status phc_fd_create(const phc_fd_params *p, phc_fd **out)
     h->nj = (p->tile_j ? p->tile_j : (p->block_waves ? 2 * h->ty : dtj)) / h->ty;
     h->chunk_i = p->chunk_i ? p->chunk_i : (h->R == 4 ? 512 : 256);
     h->index64 = p->index64 ? 1 : 0;
+    h->krem_mode = p->k_remainder;
     h->L = make_fd_layout(h->nx, h->ny, h->nz, h->R);
     phc_status st = PHC_OK;
     auto bail = [&](phc_status s) { phc_fd_destroy(h); return s; };
@@ -875,7 +905,8 @@ phc_status phc_fd_synchronize(phc_fd *h)
 // Times every block shape / chunk length the library has a kernel for on the
 // handle's grid (on scratch copies of the two levels, so the simulation state
 // is untouched) and keeps the fastest.  Results are cached in a text file,
-// one line per (grid, radius, arithmetic): "fd nx ny nz R arith waves tile_j chunk_i ms".
+// one line per (grid, radius, arithmetic, step variant, k-remainder mode):
+// "fd nx ny nz R arith pml kmode waves tile_j chunk_i ms".
 namespace phc {
 
 static bool fd_tune_lookup(const char *path, const phc_fd *h, int *ty, int *tj, int *ci)
@@ -883,17 +914,24 @@ static bool fd_tune_lookup(const char *path, const phc_fd *h, int *ty, int *tj, 
     if (!path) return false;
     FILE *f = std::fopen(path, "r");
     if (!f) return false;
-    char tag[8];
-    int nx, ny, nz, R, ar, a, b, c;
-    double ms;
     bool hit = false;
-    while (std::fscanf(f, "%7s %d %d %d %d %d %d %d %d %lf", tag, &nx, &ny, &nz, &R, &ar, &a, &b, &c, &ms) == 10)
-        if (!std::strcmp(tag, "fd") && nx == h->nx && ny == h->ny && nz == h->nz && R == h->R && ar == h->arith) {
+    char line[256];
+    while (std::fgets(line, sizeof(line), f)) {  // lines with other tags are skipped whole
+        char tag[8];
+        int nx, ny, nz, R, ar, pml, km, a, b, c;
+        double ms;
+        if (std::sscanf(line, "%7s %d %d %d %d %d %d %d %d %d %d %lf", tag, &nx, &ny, &nz, &R, &ar, &pml, &km, &a, &b,
+                        &c, &ms) != 12)
+            continue;
+        if (!std::strcmp(tag, "fd") && nx == h->nx && ny == h->ny && nz == h->nz && R == h->R && ar == h->arith &&
+            pml == h->pml_on && km == h->krem_mode && a > 0 && b % a == 0 && c > 0 &&
+            fd_pick_kernel(h->R, a, b / a, h->arith, true, false)) {
             *ty = a; *tj = b; *ci = c;
             hit = true;
         }
+    }
     std::fclose(f);
-    return hit && fd_pick_kernel(h->R, *ty, *tj / *ty, h->arith, true, false) && *ci > 0;
+    return hit;
 }
 
 } // namespace phc
@@ -961,8 +999,8 @@ extern "C" phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char
         if (cache_path) {
             FILE *f = std::fopen(cache_path, "a");
             if (f) {
-                std::fprintf(f, "fd %d %d %d %d %d %d %d %d %.5f\n", h->nx, h->ny, h->nz, h->R,
-                             h->arith, bty, bty * bnj, bci, best / steps_per_trial);
+                std::fprintf(f, "fd %d %d %d %d %d %d %d %d %d %d %.5f\n", h->nx, h->ny, h->nz, h->R,
+                             h->arith, h->pml_on, h->krem_mode, bty, bty * bnj, bci, best / steps_per_trial);
                 std::fclose(f);
             }
         }

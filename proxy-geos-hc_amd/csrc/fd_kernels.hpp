@@ -70,6 +70,11 @@ struct FdStepArgs {
     float *pml_phi;
     float pml_hd2[3];
     int px3, px4, py3, py4, pz3, pz4;
+    // k-remainder slab: when the box's k extent ends in a partial 64-lane
+    // tile of at most 32 lanes, the tiles cover k < krem only and the blocks
+    // from nb_main on update the slab [krem, khi) one point per thread
+    int nb_main;        // marching blocks (0: every block marches)
+    int krem;
 };
 
 } // namespace phc

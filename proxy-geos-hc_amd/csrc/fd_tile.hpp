@@ -344,4 +344,87 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
     }
 }
 
+// The k-remainder slab [g.krem, khi) x [jlo, jhi) x [ilo, ihi) of one grid,
+// one point per thread: thread t of the call updates point first + t of the
+// slab (k fastest); the tiles cover k < g.krem.  The same update as fd_tile,
+// with the same operations in the same order, so a point gets the same bits
+// from either path; the stencil values come straight from memory (the slab
+// is at most 32 points wide: these loads are L2 hits of the rows the
+// neighbouring tiles stream).
+template <int R, int ARITH, typename IT, bool PML = false, class G>
+__device__ __forceinline__ void fd_rem_points(const FdStepArgs &a, const G &g, int64_t first, int nthreads)
+{
+    constexpr IT S = sizeof(IT) == 4 ? 4 : 1;
+    const FdLayout &L = g.L;
+    const int W = g.khi - g.krem, NJ = g.jhi - g.jlo;
+    const int64_t t = first + (int64_t)threadIdx.y * blockDim.x + threadIdx.x;
+    if (W <= 0 || t >= (int64_t)W * NJ * (g.ihi - g.ilo)) return;
+    (void)nthreads;
+    const int k = g.krem + (int)(t % W);
+    const int64_t rest = t / W;
+    const int j = g.jlo + (int)(rest % NJ);
+    const int i = g.ilo + (int)(rest / NJ);
+    const IT si = (IT)L.pitch_j * S, sj = (IT)L.pitch_k * S;
+    const IT o = (IT)fd_off(L, i, j, k) * S;
+    const float pc = ldo(g.pn, o);
+    const float pmj = ldo(g.pnm1, o), vvj = ldo(g.vp, o);
+    float lapx = 0.f, lapy = 0.f, lapz = 0.f, lap = 0.f;
+    if constexpr (ARITH == PHC_FD_ARITH_REF) {
+#pragma clang fp contract(off)
+#pragma unroll
+        for (int l = 1; l <= R; ++l) lapx += a.cx[l] * (ldo(g.pn, o + l * si) + ldo(g.pn, o - l * si));
+#pragma unroll
+        for (int l = 1; l <= R; ++l) lapy += a.cy[l] * (ldo(g.pn, o + l * sj) + ldo(g.pn, o - l * sj));
+#pragma unroll
+        for (int l = 1; l <= R; ++l) lapz += a.cz[l] * (ldo(g.pn, o + l * S) + ldo(g.pn, o - l * S));
+    } else {
+        lap = a.c0f * pc;
+#pragma unroll
+        for (int l = 1; l <= R; ++l) {
+            lap = fmaf(a.cx[l], ldo(g.pn, o + l * si) + ldo(g.pn, o - l * si), lap);
+            lap = fmaf(a.cy[l], ldo(g.pn, o + l * sj) + ldo(g.pn, o - l * sj), lap);
+            lap = fmaf(a.cz[l], ldo(g.pn, o + l * S) + ldo(g.pn, o - l * S), lap);
+        }
+    }
+    float val;
+    if constexpr (PML) {
+        const bool slab = i < a.px3 || i >= a.px4 || j < a.py3 || j >= a.py4 || k < a.pz3 || k >= a.pz4;
+        if (slab) {
+            float e, tg, phin;
+            fd_pml_terms<ARITH>(a.pml_ex[i], a.pml_ex[i + 1], a.pml_ex[i + 2], a.pml_ey[j], a.pml_ey[j + 1],
+                                a.pml_ey[j + 2], a.pml_ez[k], a.pml_ez[k + 1], a.pml_ez[k + 2], ldo(g.pn, o - si),
+                                ldo(g.pn, o + si), ldo(g.pn, o - sj), ldo(g.pn, o + sj), ldo(g.pn, o - S),
+                                ldo(g.pn, o + S), a.pml_hd2, e, tg);
+            fd_pml_update<ARITH>(pc, pmj, vvj, ldo(a.pml_phi, o), a.coef0, lapx, lapy, lapz, lap, e, tg, val, phin);
+            sto_nt(a.pml_phi, o, phin);
+            if ((int64_t)(o / S) == g.src_off) {
+                if (g.trace) *g.trace = val;
+                if (g.inj) val = add_nc(val, mul_nc(vvj, g.rhs_next));
+            }
+            sto_nt(g.pnm1, o, val);
+            return;
+        }
+    }
+    float s = 1.f;
+    if (a.sw > 0) {
+        const bool inI = (i >= a.sw) && (i < L.nx - a.sw), inJ = (j >= a.sw) && (j < L.ny - a.sw),
+                   inK = (k >= a.sw) && (k < L.nz - a.sw);
+        if (inJ && inK && !inI) s = fd_face_damping(a, i, L.nx);
+        if (inK && inI && !inJ) s = g.fy[j];
+        if (inJ && inI && !inK) s = g.fz[k];
+    }
+    if constexpr (ARITH == PHC_FD_ARITH_REF) {
+        const float pmv = mul_nc(pmj, a.damp_prev ? s : 1.f);
+        val = fd_update_ref(pc, pmv, vvj, a.coef0, lapx, lapy, lapz, s);
+    } else {
+        const float pmv = pmj * (a.damp_prev ? s : 1.f);
+        val = s * fmaf(vvj, lap, 2.f * pc - pmv);
+    }
+    if ((int64_t)(o / S) == g.src_off) {
+        if (g.trace) *g.trace = val;
+        if (g.inj) val = add_nc(val, mul_nc(vvj, g.rhs_next));
+    }
+    sto_nt(g.pnm1, o, val);
+}
+
 } // namespace phc

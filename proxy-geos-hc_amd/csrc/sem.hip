@@ -60,7 +60,46 @@ struct SemSepArgs {
     // in XCD block order): hardware block b marches tile column sched[b].x
     // over planes [sched[b].y, sched[b].z); an empty range does nothing
     const int4 *sched;
+    // x-remainder slab: when the last 64-node x tile would hold at most 32
+    // nodes, the tiles stop at x = krem and the blocks from nb_main on update
+    // the interior nodes with x in [krem, nz - 1) one per thread
+    int nb_main, krem;
 };
+
+// One node of the x-remainder slab per thread: the separable update with the
+// same operations in the same order as the march (so a node gets the same
+// bits from either path), its stencil values read from memory.
+template <int P, typename IT>
+__device__ __forceinline__ void sem_rem_points(const SemSepArgs &a, int64_t t)
+{
+    constexpr int R = P, W = 2 * P + 1;
+    constexpr IT S = sizeof(IT) == 4 ? 4 : 1;
+    const FdLayout &L = a.L;
+    const int nk = (L.nz - 1) - a.krem, nj = L.ny - 2;
+    if (nk <= 0 || t >= (int64_t)nk * nj * (L.nx - 2)) return;
+    const int k = a.krem + (int)(t % nk);
+    const int64_t rest = t / nk;
+    const int j = 1 + (int)(rest % nj);
+    const int i = 1 + (int)(rest / nj);
+    const IT si = (IT)L.pitch_j * S, sj = (IT)L.pitch_k * S;
+    const IT o = (IT)fd_off(L, i, j, k) * S;
+    const float pc = ldo(a.pn, o);
+    const int my = i % P, mz = j % P, mx = k % P;
+    float lap = 0.f;
+#pragma unroll
+    for (int d = 0; d < W; ++d) {
+        if (d == R) continue;
+        lap = fmaf(a.cy[my][d], ldo(a.pn, o + (d - R) * si) - pc, lap);
+        lap = fmaf(a.cz[mz][d], ldo(a.pn, o + (d - R) * sj) - pc, lap);
+        lap = fmaf(a.cx[mx][d], ldo(a.pn, o + (d - R) * S) - pc, lap);
+    }
+    float val = fmaf(-ldo(a.qf, o), lap, 2.f * pc - ldo(a.pnm1, o));
+    if ((int64_t)(o / S) == a.src_off) {
+        if (a.trace) *a.trace = val;
+        if (a.inj) val = add_nc(val, mul_nc(a.inj_coef, a.rhs_next));
+    }
+    sto_nt(a.pnm1, o, val);
+}
 
 // Separable SEM step.  Same march as fd_step_kernel (register rings unrolled
 // by 2P+2, halo FIFO, double-buffered LDS plane, LDS-only barrier); the
@@ -84,6 +123,11 @@ sem_sep_kernel(SemSepArgs a)
     constexpr IT S = sizeof(IT) == 4 ? 4 : 1;
     __shared__ float tile[2][LH][LW];
 
+    if (a.nb_main && (int)blockIdx.x >= a.nb_main) {
+        sem_rem_points<P, IT>(a, (int64_t)(blockIdx.x - a.nb_main) * (FD_TK * TY) +
+                                     (int64_t)threadIdx.y * FD_TK + threadIdx.x);
+        return;
+    }
     const FdLayout &L = a.L;
     const int tx = threadIdx.x, ty = threadIdx.y;
     int w, i0, i1;
@@ -94,7 +138,7 @@ sem_sep_kernel(SemSepArgs a)
         i1 = e.z;
         if (i0 >= i1) return;  // padding of a shorter XCD list (whole block)
     } else {
-        const int b = xcd_block(blockIdx.x, gridDim.x);
+        const int b = xcd_block(blockIdx.x, a.nb_main ? a.nb_main : gridDim.x);
         const int ic = b / (a.tiles_k * a.tiles_j);
         w = b - ic * (a.tiles_k * a.tiles_j);
         i0 = ic * a.chunk_i;
@@ -440,6 +484,7 @@ struct phc_sem {
         cz[SEM_MAX_P][2 * SEM_MAX_P + 1];
     int ty = 16, nj = 1, chunk_i = 0;
     int index64 = 0;  // 64-bit offsets even for arrays < 4 GiB
+    int krem_mode = 0;  // phc_sem_params.k_remainder
     // element path
     DevBuf y, dt2m, geo, rhs;
     float Dm[(SEM_MAX_P + 1) * (SEM_MAX_P + 1)];
@@ -541,12 +586,22 @@ static void sem_build_sched(int nx, int ncol, int cus, int m, std::vector<int4> 
         for (size_t l = 0; l < lists[x].size(); ++l) tab[8 * l + x] = lists[x][l];
 }
 
+// x tiles of the separable kernel (over x in [0, nz - 1), interior from 1),
+// and the first x of the remainder slab (0: none).
+static int sem_tiles_k(const phc_sem *h, int *krem)
+{
+    const int span = std::max(h->L.nz - 1, 1), full = span / FD_TK, rem = span - full * FD_TK;
+    const bool use = h->krem_mode >= 0 && full > 0 && rem > 0 && (h->krem_mode == 1 || rem <= FD_TK / 2);
+    if (krem) *krem = use ? FD_TK * full : 0;
+    return use ? full : (span + FD_TK - 1) / FD_TK;
+}
+
 static phc_status sem_prepare_sched(phc_sem *h)
 {
     const int TJ = h->ty * h->nj;
-    const int key = TJ * 8 - h->chunk_i;
+    const int key = (TJ * 8 - h->chunk_i) * 4 + (h->krem_mode + 1);
     if (h->kernel != PHC_SEM_SEPARABLE || h->chunk_i >= 0 || h->sched_tj == key) return PHC_OK;
-    const int ncol = ((std::max(h->L.nz - 1, 1) + FD_TK - 1) / FD_TK) * ((std::max(h->L.ny - 2, 1) + TJ - 1) / TJ);
+    const int ncol = sem_tiles_k(h, nullptr) * ((std::max(h->L.ny - 2, 1) + TJ - 1) / TJ);
     std::vector<int4> tab;
     sem_build_sched(h->L.nx, ncol, h->cus, -h->chunk_i, tab);
     PHC_HIP(hipStreamSynchronize(h->stream));  // queued launches may still read the old table
@@ -580,7 +635,7 @@ static phc_status sem_launch(phc_sem *h, int i1, int i2, int inj, float rhs_next
         a.trace = trace;
         // tiles over the interior: k = 1..nz-2 (from k = 0), rows 1..ny-2
         const int TJ = h->ty * h->nj;
-        a.tiles_k = (std::max(h->L.nz - 1, 1) + FD_TK - 1) / FD_TK;
+        a.tiles_k = sem_tiles_k(h, &a.krem);
         a.tiles_j = (std::max(h->L.ny - 2, 1) + TJ - 1) / TJ;
         int nb;
         if (h->chunk_i < 0) {
@@ -590,10 +645,16 @@ static phc_status sem_launch(phc_sem *h, int i1, int i2, int inj, float rhs_next
             a.chunk_i = 0;
             nb = h->sched_nb;
         } else {
+            if (h->chunk_i == 0) return fail(PHC_ERR_INVALID, "chunk_i is 0");
             a.chunk_i = h->chunk_i;
             // order 4: chunks start at multiples of 4 (the kernel's coefficient rows assume it)
             if (h->p == 4 && a.chunk_i < h->L.nx && a.chunk_i % 4) a.chunk_i += 4 - a.chunk_i % 4;
             nb = a.tiles_k * a.tiles_j * ((h->L.nx + a.chunk_i - 1) / a.chunk_i);
+        }
+        if (a.krem > 0) {
+            const int64_t pts = (int64_t)(h->L.nz - 1 - a.krem) * (h->L.ny - 2) * (h->L.nx - 2);
+            a.nb_main = nb;
+            nb += (int)((pts + FD_TK * h->ty - 1) / (FD_TK * h->ty));
         }
         SemSepFn fn = sem_pick_sep(h->p, h->ty, h->nj, !h->index64 && h->L.total * 4 < ((int64_t)1 << 32));
         if (!fn) return fail(PHC_ERR_INVALID, "no separable kernel for p=%d", h->p);
@@ -838,6 +899,9 @@ phc_status phc_sem_create(const phc_sem_params *p, phc_sem **out)
         h->nj = p->tile_j ? p->tile_j / h->ty : 1;
         h->chunk_i = p->chunk_i ? p->chunk_i : h->ny;
         h->index64 = p->index64 ? 1 : 0;
+        h->krem_mode = p->k_remainder;
+        if (h->krem_mode < -1 || h->krem_mode > 1)
+            return bail(fail(PHC_ERR_INVALID, "k_remainder must be -1, 0 or 1 (got %d)", h->krem_mode));
         if (h->chunk_i < -4)
             return bail(fail(PHC_ERR_INVALID, "chunk_i %d: planes per chunk, or -1..-4 for the per-XCD schedule",
                              h->chunk_i));
@@ -1186,15 +1250,23 @@ extern "C" phc_status phc_sem_autotune(phc_sem *h, int steps_per_trial, const ch
     if (cache_path) {
         FILE *f = std::fopen(cache_path, "r");
         if (f) {
-            char tag[8];
-            int ex, ey, ez, pp, a, b, c;
-            double ms;
-            while (std::fscanf(f, "%7s %d %d %d %d %d %d %d %lf", tag, &ex, &ey, &ez, &pp, &a, &b, &c, &ms) == 9)
+            // one entry per line, "sem ex ey ez p kmode waves tile_j chunk_i ms";
+            // lines with another tag (an FD entry in the same file) are skipped whole
+            char line[256];
+            while (std::fgets(line, sizeof(line), f)) {
+                char tag[8];
+                int ex, ey, ez, pp, km, a, b, c;
+                double ms;
+                if (std::sscanf(line, "%7s %d %d %d %d %d %d %d %d %lf", tag, &ex, &ey, &ez, &pp, &km, &a, &b, &c,
+                                &ms) != 10)
+                    continue;
+                const bool chunk_ok = c > 0 || (c >= -4 && c <= -1);
                 if (!std::strcmp(tag, "sem") && ex == h->ex && ey == h->ey && ez == h->ez && pp == h->p &&
-                    a > 0 && b % a == 0 && (c > 0 || c >= -4) && sem_pick_sep(h->p, a, b / a, true)) {
+                    km == h->krem_mode && a > 0 && b % a == 0 && chunk_ok && sem_pick_sep(h->p, a, b / a, true)) {
                     h->ty = a; h->nj = b / a; h->chunk_i = c;
                     hit = true;
                 }
+            }
             std::fclose(f);
         }
     }
@@ -1254,7 +1326,8 @@ extern "C" phc_status phc_sem_autotune(phc_sem *h, int steps_per_trial, const ch
         if (cache_path) {
             FILE *f = std::fopen(cache_path, "a");
             if (f) {
-                std::fprintf(f, "sem %d %d %d %d %d %d %d %.5f\n", h->ex, h->ey, h->ez, h->p, bty, bty * bnj,
+                std::fprintf(f, "sem %d %d %d %d %d %d %d %d %.5f\n", h->ex, h->ey, h->ez, h->p, h->krem_mode, bty,
+                             bty * bnj,
                              bci, best / steps_per_trial);
                 std::fclose(f);
             }

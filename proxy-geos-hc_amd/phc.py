@@ -46,7 +46,7 @@ class FdParams(ctypes.Structure):
                 ("sponge_width", c_int), ("sponge_alpha", c_float),
                 ("init_value", c_float), ("arith", c_int),
                 ("tile_j", c_int), ("block_waves", c_int), ("chunk_i", c_int),
-                ("index64", c_int)]
+                ("index64", c_int), ("k_remainder", c_int)]
 
 
 # name -> (restype, argtypes) of every exported C symbol declared in include/*.h
@@ -200,7 +200,7 @@ class FdSolver:
     vp^2 dt^2, sponge and source, with the FDTDKernel step surface."""
 
     def __init__(self, setup, arith=ARITH_REF, init_value=1e-6, tile_j=0, chunk_i=0,
-                 index64=False, block_waves=0):
+                 index64=False, block_waves=0, k_remainder=0):
         s = setup
         self.setup = s
         self.nx, self.ny, self.nz, self.L = s.nx, s.ny, s.nz, s.L
@@ -223,6 +223,7 @@ class FdSolver:
         p.block_waves = block_waves
         p.chunk_i = chunk_i
         p.index64 = 1 if index64 else 0
+        p.k_remainder = k_remainder
         h = c_void_p()
         check(lib().phc_fd_create(ctypes.byref(p), ctypes.byref(h)))
         self.h = h
@@ -425,7 +426,7 @@ class BatchParams(ctypes.Structure):
                 ("half_width", c_int), ("coef", c_void_p), ("coef0", c_double),
                 ("sponge_width", c_int), ("sponge_alpha", c_float), ("init_value", c_float),
                 ("arith", c_int), ("tile_j", c_int), ("block_waves", c_int),
-                ("chunk_i", c_int), ("blocks_per_cu", c_int)]
+                ("chunk_i", c_int), ("blocks_per_cu", c_int), ("k_remainder", c_int)]
 
 
 SIGNATURES.update({
@@ -463,7 +464,7 @@ class FdBatch:
     sample count) advanced together by the batched persistent kernel."""
 
     def __init__(self, setups, arith=ARITH_FP32, init_value=1e-6, tile_j=0, block_waves=0, chunk_i=0,
-                 blocks_per_cu=0):
+                 blocks_per_cu=0, k_remainder=0):
         s0 = setups[0]
         for s in setups:
             if (s.L != s0.L or s.nsamples != s0.nsamples or s.sponge_width != s0.sponge_width
@@ -491,6 +492,7 @@ class FdBatch:
         p.init_value = init_value
         p.arith = arith
         p.tile_j, p.block_waves, p.chunk_i, p.blocks_per_cu = tile_j, block_waves, chunk_i, blocks_per_cu
+        p.k_remainder = k_remainder
         h = c_void_p()
         check(lib().phc_batch_create(ctypes.byref(p), ctypes.byref(h)))
         self.h = h
@@ -691,7 +693,8 @@ class SemParams(ctypes.Structure):
                 ("order", c_int), ("model", c_void_p), ("dt", c_float),
                 ("rhs", c_void_p), ("nsamples", c_int), ("src_element", c_int),
                 ("kernel", c_int), ("weights_ref", c_int),
-                ("tile_j", c_int), ("block_waves", c_int), ("chunk_i", c_int), ("index64", c_int)]
+                ("tile_j", c_int), ("block_waves", c_int), ("chunk_i", c_int), ("index64", c_int),
+                ("k_remainder", c_int)]
 
 
 SIGNATURES.update({
@@ -981,7 +984,7 @@ class SemSolver:
     def __init__(self, ex=10, lx=2000.0, order=2, dt=0.001, f0=10.0, model=None, nsamples=100,
                  src_point=(1001.0, 1001.0, 1001.0), ey=None, ez=None, ly=None, lz=None,
                  kernel=SEM_SEPARABLE, weights_ref=True, source_order=1, rhs=None,
-                 tile_j=0, block_waves=0, chunk_i=0, index64=False):
+                 tile_j=0, block_waves=0, chunk_i=0, index64=False, k_remainder=0):
         ey = ex if ey is None else ey
         ez = ex if ez is None else ez
         ly = lx if ly is None else ly
@@ -1006,6 +1009,7 @@ class SemSolver:
         p.weights_ref = 1 if weights_ref else 0
         p.tile_j, p.block_waves, p.chunk_i = tile_j, block_waves, chunk_i
         p.index64 = 1 if index64 else 0
+        p.k_remainder = k_remainder
         h = c_void_p()
         check(lib().phc_sem_create(ctypes.byref(p), ctypes.byref(h)))
         self.h = h

@@ -42,11 +42,14 @@ def oracle_of(oracle, s):
     return P
 
 
-def test_ref_mode_bit_exact_with_oracle(gpu, oracle):
-    banner("ref mode vs oracle, %d shots" % len(DIMS))
+@pytest.mark.parametrize("krem", [0, -1, 1])
+def test_ref_mode_bit_exact_with_oracle(gpu, oracle, krem):
+    # krem: shots whose last k tile is partial (nz 70, 130; nz 35 has no full tile)
+    # update it point by point (0: when it holds <= 32 points) or march it
+    banner("ref mode vs oracle, %d shots, k remainder mode %d" % (len(DIMS), krem))
     setups = make_shots(gpu)
     rec = receivers(setups)
-    b = gpu.FdBatch(setups, arith=gpu.ARITH_REF, chunk_i=16)
+    b = gpu.FdBatch(setups, arith=gpu.ARITH_REF, chunk_i=16, k_remainder=krem)
     b.set_receivers(rec)
     n = 45
     src, rv = b.run(n)

@@ -109,7 +109,12 @@ def test_sem_step_with_caller_rhs(gpu, kernel):
         v = np.array([b.rhs[it]], np.float32)
         gpu.check(gpu.lib().phc_sem_compute_one_step_rhs(b.h, b.i1, b.i2, 1, el.ctypes.data, v.ctypes.data))
         b.i1, b.i2 = b.i2, b.i1
-    assert np.array_equal(a.current(), b.current()) and np.array_equal(a.previous(), b.previous())
+    if kernel == "element":
+        # FP32 atomics add in no fixed order: the two runs agree to rounding
+        for x, y in ((a.current(), b.current()), (a.previous(), b.previous())):
+            assert np.linalg.norm(x - y) <= 1e-6 * np.linalg.norm(x)
+    else:
+        assert np.array_equal(a.current(), b.current()) and np.array_equal(a.previous(), b.previous())
     bad = np.array([343], np.int32)
     with pytest.raises(gpu.PhcError):
         gpu.check(gpu.lib().phc_sem_compute_one_step_rhs(b.h, 0, 1, 1, bad.ctypes.data, v.ctypes.data))

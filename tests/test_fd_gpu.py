@@ -94,6 +94,36 @@ def test_unequal_half_widths_bit_exact(gpu, oracle, lh):
     assert rel_l2(f32.current(), P.current()) < 1e-5
 
 
+@pytest.mark.parametrize("nz,mode", [(70, 0), (96, 0), (100, 1), (131, 0), (64, 1), (45, -1)])
+def test_k_remainder_slab_bit_exact(gpu, oracle, nz, mode):
+    """The last partial 64-point tile along k updated point by point (extra
+    blocks of the same launch) gives the bits of the tile march: reference
+    rounding = the oracle; FP32 equal between the two paths; with the PML."""
+    banner("k remainder nz=%d mode=%d" % (nz, mode))
+    s, P = make_pair(gpu, oracle, 33, 29, nz, sponge=7, nsamples=40, xs=16, ys=14, zs=nz - 3)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF, k_remainder=mode)
+    rec = np.array([[16, 14, nz - 3], [5, 6, nz - 1], [30, 2, 64 if nz > 64 else 1]], np.int32)
+    sol.set_receivers(rec)
+    src, rv = sol.run(0, 30)
+    tr, orv = P.run(30, receivers=rec)
+    assert np.array_equal(src, tr) and np.array_equal(rv, orv)
+    assert np.array_equal(sol.current(), P.current())
+    assert np.array_equal(sol.previous(), P.previous())
+    f = [gpu.FdSolver(s, arith=gpu.ARITH_FP32, k_remainder=m) for m in (-1, mode)]
+    for x in f:
+        x.run(0, 30)
+    assert np.array_equal(f[0].current(), f[1].current())
+    # PML variant
+    s2 = gpu.FdSetup(nx=30, ny=27, nz=nz, L=4, f0=30.0, sponge_width=0, model="two_layer", nsamples=40)
+    P2 = oracle.FdProblem(nx=30, ny=27, nz=nz, L=4, f0=30.0, model=s2.vp.copy(), nsamples=40, sponge_width=0,
+                          pml=True, ndamp=6)
+    p2 = gpu.FdSolver(s2, arith=gpu.ARITH_REF, k_remainder=mode)
+    p2.set_pml(ndamp=6, f0=30.0)
+    src2, _ = p2.run(0, 30)
+    tr2, _ = P2.run(30)
+    assert np.array_equal(src2, tr2) and np.array_equal(p2.current(), P2.current())
+
+
 def test_add_rhs_value_equals_add_rhs(gpu):
     """addRHS with the caller's sample (phc_fd_add_rhs_value) gives the bits of
     addRHS from the handle's samples."""
@@ -147,11 +177,14 @@ def test_autotune_keeps_state_and_parity(gpu, oracle, tmp_path):
     sol.run(0, 10)
     before = sol.current().copy()
     cache = str(tmp_path / "tune.txt")
+    # a SEM entry (another tag, another field count) ahead of the FD ones
+    open(cache, "w").write("sem 10 10 10 4 0 16 16 -1 0.20000\n")
     cfg = sol.autotune(steps_per_trial=3, cache_path=cache)
     assert np.array_equal(sol.current(), before)
     assert cfg[0] % cfg[1] == 0 and cfg[2] >= 1
     assert cfg == sol.autotune(steps_per_trial=3, cache_path=cache)  # cache hit
-    assert open(cache).read().startswith("fd 48 40 100 4 0 ")
+    lines = open(cache).read().splitlines()
+    assert len(lines) == 2 and lines[1].startswith("fd 48 40 100 4 0 0 0 ")
     src, _ = sol.run(10, 20)
     tr, _ = P.run(30)
     assert np.array_equal(src, tr[10:])

@@ -125,12 +125,31 @@ def test_every_block_shape_gives_the_same_bits(gpu, tmp_path, ex, ey, ez, order)
                                  (16, 16, -1), (4, 8, -1), (16, 16, -2), (4, 8, -3), (16, 16, -4)]:
         banner("shape waves=%d tile_j=%d chunk=%d" % (waves, tile_j, chunk))
         cache = tmp_path / ("s%d_%d_%d.txt" % (waves, tile_j, chunk))
-        cache.write_text("sem %d %d %d %d %d %d %d 1.0\n" % (ex, ey, ez, order, waves, tile_j, chunk))
+        cache.write_text("sem %d %d %d %d 0 %d %d %d 1.0\n" % (ex, ey, ez, order, waves, tile_j, chunk))
         s = gpu.SemSolver(**kw)
         assert s.autotune(steps_per_trial=1, cache_path=str(cache)) == (tile_j, waves, chunk)
         s.run(30)
         assert np.array_equal(s.current(), ref.current())
         assert np.array_equal(s.previous(), ref.previous())
+
+
+@pytest.mark.parametrize("ex,ey,ez,order", TILINGS)
+def test_x_remainder_slab_gives_the_same_bits(gpu, ex, ey, ez, order):
+    # the last partial x tile marched as a tile (-1) or updated node by node
+    # (1; 0 when it holds at most 32 nodes), chunked and per-XCD
+    kw = dict(ex=ex, ey=ey, ez=ez, lx=20.0 * ex, ly=20.0 * ey, lz=20.0 * ez, order=order, dt=0.0004,
+              f0=25.0, nsamples=40, src_point=(20.0 * (ex - 1) + 1, 21.0, 21.0))
+    ref = gpu.SemSolver(k_remainder=-1, **kw)
+    src_ref, _ = ref.run(30)
+    for mode, chunk in [(1, 0), (0, 0), (1, -1), (0, -3), (1, 12)]:
+        banner("x remainder mode %d chunk %d" % (mode, chunk))
+        s = gpu.SemSolver(k_remainder=mode, chunk_i=chunk, **kw)
+        src, _ = s.run(30)
+        assert np.array_equal(src, src_ref)
+        assert np.array_equal(s.current(), ref.current())
+        assert np.array_equal(s.previous(), ref.previous())
+        s.compute_one_step(30, s.i1, s.i2)
+        s.close()
 
 
 def test_xcd_schedule_rounds_and_pieces_bit_exact(gpu):
