@@ -446,6 +446,16 @@ phc_status phc_lb_eval(phc_lb *h, float *loss, float *grad);
 /* Adam (lr, 0.9, 0.999, 1e-8) on minibatches of batch_per_face random samples
  * of every face; loss_history[s] (may be NULL) = minibatch loss at step s. */
 phc_status phc_lb_train(phc_lb *h, int steps, int batch_per_face, float lr, uint64_t seed, float *loss_history);
+/* No counterpart in the reference.  phc_lb_train for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same steps and the same values, with
+ * the per-call buffers (minibatch indices, per-block partial gradients and
+ * losses, loss history) in one static device array per GPU instead of
+ * buffers allocated per call.  Limits: batch_per_face <= 8192 and
+ * steps <= 16384 per call (PHC_ERR_INVALID beyond).  Calls that overlap on
+ * one GPU share that array and race. */
+phc_status phc_lb_train_noalloc(phc_lb *h, int steps, int batch_per_face, float lr, uint64_t seed,
+                                float *loss_history);
 /* Gaussian noise of standard deviation std added to the field inputs of every
  * training sample (the clean target kept), from the next phc_lb_train on:
  * the network then learns not to amplify errors in its own inputs, which a

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -813,6 +814,7 @@ hipError_t lb_enqueue_apply(const LbInfer &lb, const FdLayout &L, float *pnew, h
 using namespace phc;
 
 struct phc_lb {
+    int device = 0;  // the device current at create (the stream's device)
     DevBuf theta, grad, m, v, loss, wbf, idx;
     DevBuf data[LB_FACES];
     DevBuf count;  // unsigned long long per face
@@ -821,6 +823,53 @@ struct phc_lb {
     float noise = 0.f;  // training input noise (phc_lb_set_train_noise)
     hipStream_t stream = nullptr;
 };
+
+// ---------------------------------------------------- training without allocation
+// phc_lb_train_noalloc keeps phc_lb_train's per-call buffers (minibatch
+// indices, per-block partial gradients and losses, loss history) in one static
+// device array, `scratch`, of fixed size: LB_NA_MAX_BATCH samples per face and
+// LB_NA_MAX_STEPS steps per call, 5.6 MiB.  Each GPU holds its own copy until
+// the process ends; its address on a device is resolved once, at the first
+// phc_lb_create with that device current, and kept in lb_na_ptr.  Calls on one
+// GPU share the array: calls that overlap on one GPU race.
+constexpr int LB_NA_MAX_BATCH = 8192;
+constexpr int LB_NA_MAX_STEPS = 16384;
+constexpr int LB_NA_MAX_BLOCKS = LB_FACES * ((LB_NA_MAX_BATCH + LB_TB - 1) / LB_TB);
+constexpr size_t lb_na_round(size_t n) { return (n + 63) / 64 * 64; }  // 256-byte regions
+constexpr size_t LB_NA_PART = 0;
+constexpr size_t LB_NA_PLOSS = LB_NA_PART + lb_na_round((size_t)LB_NA_MAX_BLOCKS * LB_NP);
+constexpr size_t LB_NA_LOSS = LB_NA_PLOSS + lb_na_round(LB_NA_MAX_BLOCKS);
+constexpr size_t LB_NA_IDX = LB_NA_LOSS + lb_na_round(LB_NA_MAX_STEPS);
+constexpr size_t LB_NA_FLOATS = LB_NA_IDX + lb_na_round((size_t)LB_FACES * LB_NA_MAX_BATCH);
+static_assert(sizeof(int) == sizeof(float), "indices share the float array");
+
+static __device__ __align__(256) float scratch[LB_NA_FLOATS];
+
+constexpr int LB_NA_MAX_DEVICES = 64;
+static std::mutex lb_na_mutex;
+static float *lb_na_ptr[LB_NA_MAX_DEVICES];
+
+// The device address of `scratch` on device dev, which is current.
+static phc_status lb_na_setup(int dev)
+{
+    if (dev < 0 || dev >= LB_NA_MAX_DEVICES) return fail(PHC_ERR_INVALID, "device %d outside 0..%d", dev,
+                                                         LB_NA_MAX_DEVICES - 1);
+    std::lock_guard<std::mutex> g(lb_na_mutex);
+    if (lb_na_ptr[dev]) return PHC_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(PHC_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+    lb_na_ptr[dev] = static_cast<float *>(p);
+    return PHC_OK;
+}
+
+static float *lb_na_base(int dev)
+{
+    if (dev < 0 || dev >= LB_NA_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> g(lb_na_mutex);
+    return lb_na_ptr[dev];
+}
 
 static phc_status lb_upload_bf16(phc_lb *h)
 {
@@ -863,6 +912,9 @@ phc_status phc_lb_create(uint64_t seed, phc_lb **out)
     *out = nullptr;
     phc_lb *h = new phc_lb();
     auto bail = [&](phc_status s) { phc_lb_destroy(h); return s; };
+    if (hipGetDevice(&h->device) != hipSuccess) return bail(fail(PHC_ERR_HIP, "hipGetDevice failed"));
+    phc_status sst = lb_na_setup(h->device);
+    if (sst != PHC_OK) return bail(sst);
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(PHC_ERR_HIP, "hipStreamCreate failed"));
     const size_t n = LB_FACES * LB_NP;
@@ -1135,50 +1187,40 @@ phc_status phc_lb_set_train_noise(phc_lb *h, float std)
     return PHC_OK;
 }
 
-// Adam on minibatches of batch_per_face random samples of each face;
-// loss_history[s] (may be NULL) = minibatch loss before step s.  The gradient
-// of each step is summed per block and the blocks in a fixed order, so a run
-// repeats bit for bit.
-phc_status phc_lb_train(phc_lb *h, int steps, int batch_per_face, float lr, uint64_t seed, float *loss_history)
+// The training loop of phc_lb_train on given per-call buffers: idx (LB_FACES x
+// batch_per_face ints), lh (steps floats), part (per-block partial gradients,
+// LB_NP floats per block), pl (per-block losses).  Every buffer is written
+// before it is read, so none needs clearing.
+static phc_status lb_train_core(phc_lb *h, const int64_t *c, int steps, int batch_per_face, float lr, uint64_t seed,
+                                int *idx, float *lh, float *part, float *pl, float *loss_history)
 {
-    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
-    if (steps < 0 || batch_per_face < 1 || batch_per_face > (1 << 24) || !(lr > 0.f))
-        return fail(PHC_ERR_INVALID, "bad training parameters");
-    int64_t c[LB_FACES];
-    PHC_TRY(lb_counts(h, c));
     int nf = 0;
     for (int f = 0; f < LB_FACES; ++f) nf += c[f] > 0;
-    if (nf == 0) return fail(PHC_ERR_STATE, "no samples");
-    PHC_TRY(h->idx.alloc(sizeof(int) * (size_t)batch_per_face * LB_FACES));
     LbBatch B;
     B.block_start[0] = 0;
     for (int f = 0; f < LB_FACES; ++f) {
         B.samples[f] = h->data[f].as<float>();
-        B.idx[f] = h->idx.as<int>() + (size_t)f * batch_per_face;
+        B.idx[f] = idx + (size_t)f * batch_per_face;
         B.count[f] = c[f] > 0 ? batch_per_face : 0;
         B.block_start[f + 1] = B.block_start[f] + (B.count[f] + LB_TB - 1) / LB_TB;
     }
     B.inv_total = 1.f / ((float)batch_per_face * nf);
     const int np = LB_FACES * LB_NP;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    DevBuf lh, part, pl;
-    PHC_TRY(lh.alloc(sizeof(float) * std::max(steps, 1)));
-    PHC_TRY(part.alloc(sizeof(float) * (size_t)B.block_start[LB_FACES] * LB_NP));
-    PHC_TRY(pl.alloc(sizeof(float) * (size_t)B.block_start[LB_FACES]));
-    B.partial = part.as<float>();
-    B.ploss = pl.as<float>();
+    B.partial = part;
+    B.ploss = pl;
     for (int s = 0; s < steps; ++s) {
         for (int f = 0; f < LB_FACES; ++f)
             if (c[f] > 0)
                 hipLaunchKernelGGL(lb_random_idx_kernel, dim3((batch_per_face + 255) / 256), dim3(256), 0, h->stream,
-                                   h->idx.as<int>() + (size_t)f * batch_per_face, batch_per_face, (int)c[f],
+                                   idx + (size_t)f * batch_per_face, batch_per_face, (int)c[f],
                                    seed * 1000003ull + (uint64_t)s * LB_FACES + f);
         B.noise = h->noise;
         B.nseed = seed * 7919ull + (uint64_t)s;
         hipLaunchKernelGGL(lb_train_kernel, dim3(B.block_start[LB_FACES]), dim3(LB_TB), 0, h->stream,
-                           h->theta.as<float>(), h->grad.as<float>(), lh.as<float>() + s, B);
+                           h->theta.as<float>(), h->grad.as<float>(), lh + s, B);
         hipLaunchKernelGGL(lb_reduce_kernel, dim3((np + 255) / 256), dim3(256), 0, h->stream, B,
-                           h->grad.as<float>(), lh.as<float>() + s);
+                           h->grad.as<float>(), lh + s);
         ++h->adam_t;
         const float c1 = 1.f / (1.f - std::pow(b1, (float)h->adam_t));
         const float c2 = 1.f / (1.f - std::pow(b2, (float)h->adam_t));
@@ -1189,8 +1231,55 @@ phc_status phc_lb_train(phc_lb *h, int steps, int batch_per_face, float lr, uint
     PHC_TRY(lb_upload_bf16(h));
     PHC_HIP(hipStreamSynchronize(h->stream));
     if (loss_history && steps > 0)
-        PHC_HIP(hipMemcpy(loss_history, lh.ptr, sizeof(float) * steps, hipMemcpyDeviceToHost));
+        PHC_HIP(hipMemcpy(loss_history, lh, sizeof(float) * steps, hipMemcpyDeviceToHost));
     return PHC_OK;
+}
+
+static phc_status lb_train_check(phc_lb *h, int steps, int batch_per_face, float lr, int64_t *c)
+{
+    if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
+    if (steps < 0 || batch_per_face < 1 || batch_per_face > (1 << 24) || !(lr > 0.f))
+        return fail(PHC_ERR_INVALID, "bad training parameters");
+    PHC_TRY(lb_counts(h, c));
+    int nf = 0;
+    for (int f = 0; f < LB_FACES; ++f) nf += c[f] > 0;
+    if (nf == 0) return fail(PHC_ERR_STATE, "no samples");
+    return PHC_OK;
+}
+
+// Adam on minibatches of batch_per_face random samples of each face;
+// loss_history[s] (may be NULL) = minibatch loss before step s.  The gradient
+// of each step is summed per block and the blocks in a fixed order, so a run
+// repeats bit for bit.
+phc_status phc_lb_train(phc_lb *h, int steps, int batch_per_face, float lr, uint64_t seed, float *loss_history)
+{
+    int64_t c[LB_FACES];
+    PHC_TRY(lb_train_check(h, steps, batch_per_face, lr, c));
+    int nblocks = 0;
+    for (int f = 0; f < LB_FACES; ++f) nblocks += c[f] > 0 ? (batch_per_face + LB_TB - 1) / LB_TB : 0;
+    PHC_TRY(h->idx.alloc(sizeof(int) * (size_t)batch_per_face * LB_FACES));
+    DevBuf lh, part, pl;
+    PHC_TRY(lh.alloc(sizeof(float) * std::max(steps, 1)));
+    PHC_TRY(part.alloc(sizeof(float) * (size_t)nblocks * LB_NP));
+    PHC_TRY(pl.alloc(sizeof(float) * (size_t)nblocks));
+    return lb_train_core(h, c, steps, batch_per_face, lr, seed, h->idx.as<int>(), lh.as<float>(), part.as<float>(),
+                         pl.as<float>(), loss_history);
+}
+
+phc_status phc_lb_train_noalloc(phc_lb *h, int steps, int batch_per_face, float lr, uint64_t seed,
+                                float *loss_history)
+{
+    int64_t c[LB_FACES];
+    PHC_TRY(lb_train_check(h, steps, batch_per_face, lr, c));
+    if (batch_per_face > LB_NA_MAX_BATCH)
+        return fail(PHC_ERR_INVALID, "batch of %d samples per face exceeds the no-alloc limit %d", batch_per_face,
+                    LB_NA_MAX_BATCH);
+    if (steps > LB_NA_MAX_STEPS)
+        return fail(PHC_ERR_INVALID, "%d steps exceed the no-alloc limit %d per call", steps, LB_NA_MAX_STEPS);
+    float *base = lb_na_base(h->device);
+    if (!base) return fail(PHC_ERR_STATE, "no scratch address for device %d", h->device);
+    return lb_train_core(h, c, steps, batch_per_face, lr, seed, reinterpret_cast<int *>(base + LB_NA_IDX),
+                         base + LB_NA_LOSS, base + LB_NA_PART, base + LB_NA_PLOSS, loss_history);
 }
 
 phc_status phc_lb_infer(phc_lb *h, int face, const float *samples, int64_t n, float *y)

@@ -582,6 +582,7 @@ SIGNATURES.update({
                                         c_float, ctypes.c_uint64]),
     "phc_lb_eval": (c_int, [c_void_p, ctypes.POINTER(c_float), c_void_p]),
     "phc_lb_train": (c_int, [c_void_p, c_int, c_int, c_float, ctypes.c_uint64, c_void_p]),
+    "phc_lb_train_noalloc": (c_int, [c_void_p, c_int, c_int, c_float, ctypes.c_uint64, c_void_p]),
     "phc_lb_set_train_noise": (c_int, [c_void_p, c_float]),
     "phc_lb_infer": (c_int, [c_void_p, c_int, f32p, ctypes.c_int64, f32p]),
     "phc_fd_set_learned_boundary": (c_int, [c_void_p, c_void_p, c_float, c_float]),
