@@ -268,12 +268,14 @@ def bench_pml(phc, n=512, steps=200):
 
 def bench_sem_elem(phc, steps=100, steps_p4=20):
     """The reference's CPU-runnable SEM case (50^3 elements of order 2, 101^3
-    nodes, c = 1500, f0 = 5 Hz, dt = 1e-3) through the three SEM step kernels:
+    nodes, c = 1500, f0 = 5 Hz, dt = 1e-3) through the SEM step kernels:
     separable, element with atomic scatter, element with the 8-colour
-    scatter; then the same three on 100^3 elements of order 4 (401^3 nodes,
-    dt = 2e-4, f0 = 8 Hz)."""
+    scatter, element write-once march (leapfrog fused); then the same on
+    100^3 elements of order 4 (401^3 nodes, dt = 2e-4, f0 = 8 Hz), with the
+    roofline fraction at 16 B per node update."""
     out = {"workload": "SEM 50^3 elements of order 2 (101^3 nodes), %d steps, FP32" % steps}
-    for name, kern in (("separable", 0), ("element_atomic", 1), ("element_colour", 2)):
+    kernels = (("separable", 0), ("element_atomic", 1), ("element_colour", 2), ("element_march", 3))
+    for name, kern in kernels:
         s = phc.SemSolver(ex=50, lx=2000.0, order=2, dt=1e-3, f0=5.0, nsamples=steps + 1, kernel=kern)
         if kern == 0:
             s.autotune(steps_per_trial=5)
@@ -284,7 +286,7 @@ def bench_sem_elem(phc, steps=100, steps_p4=20):
         out[name] = {"ms_per_step": round(ms, 5), "value": round(101 ** 3 / (ms * 1e-3) / 1e6, 1),
                      "unit": "Mnode-updates/s"}
     p4 = {"workload": "SEM 100^3 elements of order 4 (401^3 nodes), %d steps, FP32" % steps_p4}
-    for name, kern in (("separable", 0), ("element_atomic", 1), ("element_colour", 2)):
+    for name, kern in kernels:
         s = phc.SemSolver(ex=100, lx=2000.0, order=4, dt=2e-4, f0=8.0, nsamples=steps_p4 + 6, kernel=kern)
         if kern == 0:
             s.autotune(steps_per_trial=5)
@@ -294,7 +296,8 @@ def bench_sem_elem(phc, steps=100, steps_p4=20):
         ms = s.last_kernel_ms()
         s.close()
         p4[name] = {"ms_per_step": round(ms, 5), "value": round(401 ** 3 / (ms * 1e-3) / 1e6, 1),
-                    "unit": "Mnode-updates/s"}
+                    "unit": "Mnode-updates/s",
+                    "roofline_frac": round(401 ** 3 * BYTES_PER_UPDATE / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
     out["order4"] = p4
     return out
 
@@ -315,7 +318,8 @@ def bench_learned_step(phc, n=256, steps=300):
     dt = learned.fd_dt()
     inter = np.array([15, 34, 78, 121, 215])
     out = {"workload": "FD %d^3 region, learned 4-point layer vs 40-point sponge, %d steps" % (n, steps)}
-    for name, pad, sw, use_lb in (("learned", 4, 0, True), ("sponge40", 40, 40, False)):
+    pml_w = int(phc.lib().phc_fd_pml_width(1500.0, 10.0, learned.DX))
+    for name, pad, sw, use_lb in (("learned", 4, 0, True), ("sponge40", 40, 40, False), ("pml", pml_w, 0, None)):
         N = n + 2 * pad
         vp = learned.layered_vp2dt2((N, N, N), pad, inter, np.linspace(1500.0, 4500.0, 6), dt)
         s = phc.FdSetup(N, N, N, L=4, dx=learned.DX, xs=n // 2 + pad, ys=n // 2 + pad, zs=n // 2 + pad,
@@ -323,6 +327,8 @@ def bench_learned_step(phc, n=256, steps=300):
         sol = phc.FdSolver(s, arith=phc.ARITH_FP32, init_value=0.0)
         if use_lb:
             sol.set_learned_boundary(lb, halo=0.0)
+        elif use_lb is None:
+            sol.set_pml(ndamp=pad, f0=10.0)
         shape = sol.autotune(steps_per_trial=5)
         sol.set_receivers(rec + pad)
         sol.run(0, steps)  # capture + first replay

@@ -193,6 +193,11 @@ typedef struct phc_sem phc_sem;
  * (SEMmesh.cpp:380-429, commonMacros.hpp:58-63), whose 4 colours cover
  * only the first element layer in 3D. */
 #define PHC_SEM_ELEMENT_COLOR 2
+/* element kernel that writes every node once: a block owns the nodes of a
+ * tile of elements and marches the element layers, adds K_e u_e of the
+ * tile's elements (4 colours, no shared node within one) into LDS and applies
+ * the leapfrog update (SEMsolver.cpp:46-89 in one pass, no y array) */
+#define PHC_SEM_ELEMENT_MARCH 3
 
 typedef struct {
     int ex, ey, ez;        /* elements per axis (SEMproxy.cpp:12-14) */

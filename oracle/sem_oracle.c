@@ -394,13 +394,23 @@ static void assemble_colours(orc_sem *s, const float *p, float *y, float *M)
     const int np = s->np, ex = s->m.ex, ey = s->m.ey, ez = s->m.ez;
 #pragma omp parallel for schedule(static) num_threads(s->threads)
     for (int n = 0; n < s->nN; n++) { M[n] = 0; y[n] = 0; }
-    for (int c = 0; c < 8; c++) {
+#ifdef ORC_PLANT_SCATTER_RACE
+    /* a planted mistake for the sanitizer check: all elements in one parallel
+     * loop with plain adds, as the reference's OpenMP scatter does
+     * (SEMmacros.hpp:50-56); ThreadSanitizer must report it */
+    const int ncolours = 1;
+#else
+    const int ncolours = 8;
+#endif
+    for (int c = 0; c < ncolours; c++) {
         const int px = c & 1, pz = (c >> 1) & 1, py = (c >> 2) & 1;
         const int nx = (ex - px + 1) / 2, nz = (ez - pz + 1) / 2, ny = (ey - py + 1) / 2;
-        const int ne = nx * nz * ny;
+        const int ne = ncolours == 8 ? nx * nz * ny : s->nE;
 #pragma omp parallel for schedule(static) num_threads(s->threads)
         for (int t = 0; t < ne; t++) {
-            const int l = 2 * (t % nx) + px, m = 2 * ((t / nx) % nz) + pz, n = 2 * (t / (nx * nz)) + py;
+            const int l = ncolours == 8 ? 2 * (t % nx) + px : t % ex;
+            const int m = ncolours == 8 ? 2 * ((t / nx) % nz) + pz : (t / ex) % ez;
+            const int n = ncolours == 8 ? 2 * (t / (nx * nz)) + py : t / (ex * ez);
             const int e = l + m * ex + n * ex * ez;
             float pl[216], Y[216];
             const int *nl = s->nodes + (size_t)e * np;

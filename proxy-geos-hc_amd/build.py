@@ -97,6 +97,33 @@ def build_checks():
     return outs
 
 
+def sanitize_host(jobs=8):
+    """libphc.so with AddressSanitizer on its host code (the device code is
+    built as usual; -Xarch_host keeps the sanitizer off the GPU side) and
+    tests/cpp/host_asan_check linked against it, in _build/asan/.  Returns the
+    check program's path; it runs without a GPU."""
+    out = os.path.join(BUILD, "asan")
+    os.makedirs(out, exist_ok=True)
+    san = ["-O1", "-g", "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer"]
+    flags = [f for f in HIP_FLAGS if f != "-O3"] + san
+    srcs = [s for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objs = [os.path.join(out, s.replace(".hip", ".o")) for s in srcs]
+
+    def compile_one(pair):
+        src, obj = pair
+        _run([HIPCC] + flags + ["-c", os.path.join(CSRC, src), "-o", obj])
+
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(compile_one, zip(srcs, objs)))
+    lib = os.path.join(out, "libphc.so")
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Xarch_host", "-fsanitize=address", "-o", lib] + objs)
+    exe = os.path.join(out, "host_asan_check")
+    _run([HIPCC, "-std=c++17"] + san + ["-I" + INCLUDE, "-I" + HOST,
+                                       os.path.join(ROOT, "tests", "cpp", "host_asan_check.cpp"), "-o", exe,
+                                       "-L" + out, "-lphc", "-Wl,-rpath," + out])
+    return exe
+
+
 def build_all(jobs=8):
     with cf.ThreadPoolExecutor(max_workers=2) as ex:
         f1 = ex.submit(build_libphc, jobs)
@@ -110,4 +137,4 @@ def build_all(jobs=8):
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
     {"all": build_all, "phc": build_libphc, "oracle": build_oracle, "host": build_host,
-     "checks": build_checks}[what]()
+     "checks": build_checks, "sanitize_host": sanitize_host}[what]()

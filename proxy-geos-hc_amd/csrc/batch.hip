@@ -190,7 +190,8 @@ struct phc_batch {
     int pending = 0;  // the previous level still owes its sponge product
     hipStream_t stream = nullptr;
     int ty = 16, nj = 1, chunk_i = 64, blocks_per_cu = 1, cus = 256;
-    int krem_mode = 0;  // phc_batch_params.k_remainder
+    int krem_mode = 0;  // phc_batch_params.k_remainder (as requested)
+    int krem_eff = 0;   // the mode the work list uses (the autotuner may pick -1 for 0)
     int nitems = 0;
     BatchQueues queues;
     int nrec = 0;  // receivers per shot
@@ -223,7 +224,7 @@ static phc_status batch_build_items(phc_batch *h)
     for (int s = 0; s < h->nshots; ++s) {
         const FdLayout &L = h->L[s];
         const int full = L.nz / FD_TK, rem = L.nz - full * FD_TK;
-        const bool slab = h->krem_mode >= 0 && full > 0 && rem > 0 && (h->krem_mode == 1 || rem <= FD_TK / 2);
+        const bool slab = h->krem_eff >= 0 && full > 0 && rem > 0 && (h->krem_eff == 1 || rem <= FD_TK / 2);
         h->hdesc[s].krem = slab ? FD_TK * full : 0;
         const int tk = slab ? full : (L.nz + FD_TK - 1) / FD_TK, tj = (L.ny + TJ - 1) / TJ;
         const int nc = (L.nx + h->chunk_i - 1) / h->chunk_i;
@@ -385,7 +386,7 @@ phc_status phc_batch_create(const phc_batch_params *p, phc_batch **out)
     h->nj = tj / ty;
     h->chunk_i = p->chunk_i ? p->chunk_i : 64;
     h->blocks_per_cu = p->blocks_per_cu ? p->blocks_per_cu : 1;
-    h->krem_mode = p->k_remainder;
+    h->krem_mode = h->krem_eff = p->k_remainder;
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
@@ -634,8 +635,11 @@ phc_status phc_batch_autotune(phc_batch *h, int steps_per_trial, int *tile_j, in
     static const int chunks[] = {32, 64, 128, 256, 512};
     static const int bpcs[] = {1, 2, 4};
     double best = 1e30;
-    int bty = h->ty, bnj = h->nj, bci = h->chunk_i, bbp = h->blocks_per_cu;
+    int bty = h->ty, bnj = h->nj, bci = h->chunk_i, bbp = h->blocks_per_cu, bku = h->krem_eff;
     phc_status st = PHC_OK;
+    // requested mode 0: also the partial k tiles marched as tiles (-1)
+    const int modes[2] = {h->krem_mode, -1};
+    for (int mi = 0; mi < (h->krem_mode == 0 ? 2 : 1); ++mi)
     for (const auto &sh : shapes)
         for (int ci : chunks)
             for (int bp : bpcs) {
@@ -645,6 +649,7 @@ phc_status phc_batch_autotune(phc_batch *h, int steps_per_trial, int *tile_j, in
                 h->nj = sh[1];
                 h->chunk_i = ci;
                 h->blocks_per_cu = bp;
+                h->krem_eff = modes[mi];
                 if ((st = batch_build_items(h))) break;
                 float ms = 0.f;
                 for (int rep = 0; rep < 2 && st == PHC_OK; ++rep) {  // first pass warms up
@@ -661,7 +666,7 @@ phc_status phc_batch_autotune(phc_batch *h, int steps_per_trial, int *tile_j, in
                 }
                 if (st == PHC_OK && ms < best) {
                     best = ms;
-                    bty = h->ty; bnj = h->nj; bci = h->chunk_i; bbp = h->blocks_per_cu;
+                    bty = h->ty; bnj = h->nj; bci = h->chunk_i; bbp = h->blocks_per_cu; bku = h->krem_eff;
                 }
             }
     (void)hipEventDestroy(e0);
@@ -672,7 +677,7 @@ phc_status phc_batch_autotune(phc_batch *h, int steps_per_trial, int *tile_j, in
     save.release();
     tr.release();
     ctr.release();
-    h->ty = bty; h->nj = bnj; h->chunk_i = bci; h->blocks_per_cu = bbp;
+    h->ty = bty; h->nj = bnj; h->chunk_i = bci; h->blocks_per_cu = bbp; h->krem_eff = bku;
     PHC_TRY(st);
     PHC_TRY(batch_build_items(h));
     if (tile_j) *tile_j = bty * bnj;

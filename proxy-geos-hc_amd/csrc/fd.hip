@@ -200,7 +200,8 @@ struct phc_fd {
     int pending[2] = {0, 0};
     hipStream_t stream = nullptr;
     int ty = 4, nj = 2, chunk_i = 0, index64 = 0;
-    int krem_mode = 0;  // phc_fd_params.k_remainder
+    int krem_mode = 0;  // phc_fd_params.k_remainder (as requested)
+    int krem_eff = 0;   // the mode the launches use (the autotuner may pick -1 for 0)
     int nrec = 0;
     std::vector<int64_t> rec_offs;
     int last_n = 0, last_nrec = 0;
@@ -371,7 +372,7 @@ static phc_status fd_launch_step(phc_fd *h, FdStepArgs a, bool damp)
                                    !h->index64 && h->L.total * 4 < ((int64_t)1 << 32), h->pml_on != 0);
     if (!fn) return fail(PHC_ERR_INVALID, "no kernel for R=%d block_waves=%d rows/thread=%d", h->R, h->ty, h->nj);
     a.damp_prev = damp ? 1 : 0;
-    const int nb = fd_plan_blocks(a, h->krem_mode, FD_TK * h->ty);
+    const int nb = fd_plan_blocks(a, h->krem_eff, FD_TK * h->ty);
     hipLaunchKernelGGL(fn, dim3(nb), dim3(FD_TK, h->ty), 0, h->stream, a);
     PHC_HIP(hipGetLastError());
     return PHC_OK;
@@ -453,7 +454,7 @@ phc_status phc_fd_create(const phc_fd_params *p, phc_fd **out)
     h->nj = (p->tile_j ? p->tile_j : (p->block_waves ? 2 * h->ty : dtj)) / h->ty;
     h->chunk_i = p->chunk_i ? p->chunk_i : (h->R == 4 ? 512 : 256);
     h->index64 = p->index64 ? 1 : 0;
-    h->krem_mode = p->k_remainder;
+    h->krem_mode = h->krem_eff = p->k_remainder;
     h->L = make_fd_layout(h->nx, h->ny, h->nz, h->R);
     phc_status st = PHC_OK;
     auto bail = [&](phc_status s) { phc_fd_destroy(h); return s; };
@@ -904,12 +905,14 @@ phc_status phc_fd_synchronize(phc_fd *h)
 // ================================================================ autotuner
 // Times every block shape / chunk length the library has a kernel for on the
 // handle's grid (on scratch copies of the two levels, so the simulation state
-// is untouched) and keeps the fastest.  Results are cached in a text file,
-// one line per (grid, radius, arithmetic, step variant, k-remainder mode):
-// "fd nx ny nz R arith pml kmode waves tile_j chunk_i ms".
+// is untouched) and keeps the fastest; with k_remainder 0 it also times the
+// last partial k tile as a tile (-1) and keeps the faster of the two.  Results
+// are cached in a text file, one line per (grid, radius, arithmetic, step
+// variant, requested k mode):
+// "fd nx ny nz R arith pml kmode waves tile_j chunk_i kmode_used ms".
 namespace phc {
 
-static bool fd_tune_lookup(const char *path, const phc_fd *h, int *ty, int *tj, int *ci)
+static bool fd_tune_lookup(const char *path, const phc_fd *h, int *ty, int *tj, int *ci, int *km_used)
 {
     if (!path) return false;
     FILE *f = std::fopen(path, "r");
@@ -918,15 +921,15 @@ static bool fd_tune_lookup(const char *path, const phc_fd *h, int *ty, int *tj, 
     char line[256];
     while (std::fgets(line, sizeof(line), f)) {  // lines with other tags are skipped whole
         char tag[8];
-        int nx, ny, nz, R, ar, pml, km, a, b, c;
+        int nx, ny, nz, R, ar, pml, km, a, b, c, ku;
         double ms;
-        if (std::sscanf(line, "%7s %d %d %d %d %d %d %d %d %d %d %lf", tag, &nx, &ny, &nz, &R, &ar, &pml, &km, &a, &b,
-                        &c, &ms) != 12)
+        if (std::sscanf(line, "%7s %d %d %d %d %d %d %d %d %d %d %d %lf", tag, &nx, &ny, &nz, &R, &ar, &pml, &km, &a,
+                        &b, &c, &ku, &ms) != 13)
             continue;
         if (!std::strcmp(tag, "fd") && nx == h->nx && ny == h->ny && nz == h->nz && R == h->R && ar == h->arith &&
-            pml == h->pml_on && km == h->krem_mode && a > 0 && b % a == 0 && c > 0 &&
-            fd_pick_kernel(h->R, a, b / a, h->arith, true, false)) {
-            *ty = a; *tj = b; *ci = c;
+            pml == h->pml_on && km == h->krem_mode && a > 0 && b % a == 0 && c > 0 && ku >= -1 && ku <= 1 &&
+            (ku == km || km == 0) && fd_pick_kernel(h->R, a, b / a, h->arith, true, false)) {
+            *ty = a; *tj = b; *ci = c; *km_used = ku;
             hit = true;
         }
     }
@@ -941,9 +944,9 @@ extern "C" phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char
 {
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
     if (steps_per_trial < 1) return fail(PHC_ERR_INVALID, "steps_per_trial must be >= 1");
-    int ty = 0, tj = 0, ci = 0;
-    if (fd_tune_lookup(cache_path, h, &ty, &tj, &ci)) {
-        h->ty = ty; h->nj = tj / ty; h->chunk_i = ci;
+    int ty = 0, tj = 0, ci = 0, ku = 0;
+    if (fd_tune_lookup(cache_path, h, &ty, &tj, &ci, &ku)) {
+        h->ty = ty; h->nj = tj / ty; h->chunk_i = ci; h->krem_eff = ku;
     } else {
         const size_t bytes = sizeof(float) * (size_t)h->L.total;
         DevBuf s0, s1;
@@ -962,13 +965,18 @@ extern "C" phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char
         const int shapes[][2] = {{4, 8}, {4, 16}, {8, 8}, {8, 16}, {16, 16}, {16, 32}};
         const int chunks[] = {64, 128, 256, 512, 1024};
         double best = 1e30;
-        int bty = h->ty, bnj = h->nj, bci = h->chunk_i;
+        int bty = h->ty, bnj = h->nj, bci = h->chunk_i, bku = h->krem_eff;
         phc_status st = PHC_OK;
+        const int rem = h->nz % FD_TK;
+        const bool both = h->krem_mode == 0 && rem > 0 && rem <= FD_TK / 2 && h->nz > FD_TK;
+        const int modes[2] = {h->krem_mode, -1};
+        for (int mi = 0; mi < (both ? 2 : 1) && st == PHC_OK; ++mi)
         for (auto &sh : shapes) {
+            const int ku = modes[mi];
             if (!fd_pick_kernel(h->R, sh[0], sh[1] / sh[0], h->arith, true, false)) continue;
             for (int c : chunks) {
                 if (c > h->nx && c != chunks[0]) continue;
-                h->ty = sh[0]; h->nj = sh[1] / sh[0]; h->chunk_i = std::min(c, h->nx);
+                h->ty = sh[0]; h->nj = sh[1] / sh[0]; h->chunk_i = std::min(c, h->nx); h->krem_eff = ku;
                 float ms = 0.f;
                 for (int rep = 0; rep < 2 && st == PHC_OK; ++rep) {  // first pass warms up
                     if (hipEventRecord(e0, h->stream) != hipSuccess) st = fail(PHC_ERR_HIP, "event record failed");
@@ -987,20 +995,20 @@ extern "C" phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char
                         st = fail(PHC_ERR_HIP, "trial timing failed");
                 }
                 if (st != PHC_OK) break;
-                if (ms < best) { best = ms; bty = h->ty; bnj = h->nj; bci = h->chunk_i; }
+                if (ms < best) { best = ms; bty = h->ty; bnj = h->nj; bci = h->chunk_i; bku = h->krem_eff; }
             }
             if (st != PHC_OK) break;
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        h->ty = bty; h->nj = bnj; h->chunk_i = bci;
+        h->ty = bty; h->nj = bnj; h->chunk_i = bci; h->krem_eff = bku;
         fd_drop_graph(h);
         PHC_TRY(st);
         if (cache_path) {
             FILE *f = std::fopen(cache_path, "a");
             if (f) {
-                std::fprintf(f, "fd %d %d %d %d %d %d %d %d %d %d %.5f\n", h->nx, h->ny, h->nz, h->R,
-                             h->arith, h->pml_on, h->krem_mode, bty, bty * bnj, bci, best / steps_per_trial);
+                std::fprintf(f, "fd %d %d %d %d %d %d %d %d %d %d %d %.5f\n", h->nx, h->ny, h->nz, h->R,
+                             h->arith, h->pml_on, h->krem_mode, bty, bty * bnj, bci, bku, best / steps_per_trial);
                 std::fclose(f);
             }
         }

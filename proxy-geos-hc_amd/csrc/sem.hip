@@ -366,6 +366,181 @@ sem_elem_kernel(SemElemArgs a)
     }
 }
 
+// ------------------------------------------------ element path, write-once
+//
+// The element operator assembled without a scatter to memory and with the
+// leapfrog update fused (SEMsolver.cpp:46-89 in one pass).  A block owns the
+// nodes of a tile of TE x TE elements in x and z (node x in [ex0 p, (ex0+TE) p),
+// likewise z) over a range of element layers along y, and marches the layers
+// one by one:
+//  * the node patch of the layer's (TE+1)^2 elements (the tile plus one
+//    element below it in x and z, whose faces carry owned nodes) is staged
+//    in LDS;
+//  * the elements are taken in the 4 colours of their (x, z) index parities:
+//    elements of one colour share no node, so each element's K_e u_e (the
+//    sum-factorised gradient, geometric factors, divergence, as in
+//    sem_elem_kernel) is added to an LDS accumulator of the owned nodes
+//    with plain adds, in a fixed order;
+//  * the node planes of the layer below its top face are then complete: the
+//    block applies the leapfrog update to its owned interior nodes and writes
+//    each of them once; the top plane is carried to the next layer.
+// A range of layers [ey0, ey1) owns node planes [ey0 p, ey1 p) and first
+// computes layer ey0 - 1 to complete its lowest plane.  Each node is written
+// by exactly one block and only level i2 is read across blocks, so the new
+// level can overwrite level i1 in place.
+struct SemMarchArgs {
+    const float *pn;     // level i2, node order
+    float *pnm1;         // level i1: read, overwritten by the new level
+    const float *dt2m;   // dt^2 / M per node
+    const float *geo;    // per point w w w B (6 Voigt entries); element stride geo_stride
+    int64_t geo_stride;
+    float D[(SEM_MAX_P + 1) * (SEM_MAX_P + 1)];
+    int ex, ey, ez, nx, ny, nz;
+    int tiles_x, tiles_z, ychunk;
+    int64_t src_node;
+    int inj;
+    float inj_coef, rhs_next;
+    float *trace;
+};
+
+template <int P, int TE, int NT>
+__global__ void __launch_bounds__(NT) sem_march_kernel(SemMarchArgs a)
+{
+    constexpr int q = P + 1, n3 = q * q * q;
+    constexpr int PW = (TE + 1) * P + 1;  // patch nodes per axis (x, z)
+    constexpr int OW = TE * P;            // owned nodes per axis
+    constexpr int NG = (TE + 2) / 2;      // elements per axis of one colour (at most)
+    constexpr int NPF = (P * PW * PW + NT - 1) / NT;  // prefetched patch values per thread
+    __shared__ float u[q][PW][PW];        // layer's node patch [plane][z][x]
+    __shared__ float fl[NG * NG][3][n3];  // fluxes of one colour's elements
+    __shared__ float acc[q][OW][OW];      // K u of the owned nodes [plane][z][x]
+    __shared__ float Gs[6 * n3];          // geometric factors (one element: the box mesh)
+    __shared__ float Ds[q * q];
+    const int tid = threadIdx.x;
+    const int b = blockIdx.x;
+    const int txi = b % a.tiles_x, rest = b / a.tiles_x;
+    const int tzi = rest % a.tiles_z, yr = rest / a.tiles_z;
+    const int ex0 = txi * TE, ez0 = tzi * TE;
+    const int ey0 = yr * a.ychunk, ey1 = min(ey0 + a.ychunk, a.ey);
+    const int X0 = ex0 * P - P, Z0 = ez0 * P - P;  // patch origin (node coordinates)
+    const int64_t plane = (int64_t)a.nx * a.nz;
+    const bool gshared = a.geo_stride == 0;
+    for (int t = tid; t < q * q; t += NT) Ds[t] = a.D[t];
+    if (gshared)
+        for (int t = tid; t < 6 * n3; t += NT) Gs[t] = a.geo[t];
+    for (int t = tid; t < q * OW * OW; t += NT) (&acc[0][0][0])[t] = 0.f;
+    // patch value t of planes 1..P of layer ey (plane ey P + 1 + t / PW^2)
+    auto patch_load = [&](int ey, int t) {
+        const int c = 1 + t / (PW * PW), r = t % (PW * PW), z = r / PW, x = r - z * PW;
+        const int gx = X0 + x, gz = Z0 + z;
+        float v = 0.f;
+        if (t < P * PW * PW && gx >= 0 && gx < a.nx && gz >= 0 && gz < a.nz)
+            v = a.pn[(int64_t)(ey * P + c) * plane + (int64_t)gz * a.nx + gx];
+        return v;
+    };
+    float pf[NPF];
+    const int eyf = max(ey0 - 1, 0);
+    // the first layer's bottom plane, and its other planes into registers
+    for (int t = tid; t < PW * PW; t += NT) {
+        const int z = t / PW, x = t - z * PW, gx = X0 + x, gz = Z0 + z;
+        u[0][z][x] = (gx >= 0 && gx < a.nx && gz >= 0 && gz < a.nz)
+                         ? a.pn[(int64_t)(eyf * P) * plane + (int64_t)gz * a.nx + gx] : 0.f;
+    }
+#pragma unroll
+    for (int f = 0; f < NPF; ++f) pf[f] = patch_load(eyf, tid + f * NT);
+    for (int ey = eyf; ey < ey1; ++ey) {
+        // planes 1..P of this layer from the registers (plane 0 is the
+        // previous layer's top plane, already in place)
+#pragma unroll
+        for (int f = 0; f < NPF; ++f) {
+            const int t = tid + f * NT;
+            if (t < P * PW * PW) (&u[1][0][0])[t] = pf[f];
+        }
+        __syncthreads();
+        // the next layer's planes load while this layer is computed
+        if (ey + 1 < ey1) {
+#pragma unroll
+            for (int f = 0; f < NPF; ++f) pf[f] = patch_load(ey + 1, tid + f * NT);
+        }
+#pragma unroll 1
+        for (int col = 0; col < 4; ++col) {
+            const int px = col & 1, pz = col >> 1;
+            const int ngx = (TE + 1 - px + 1) / 2, ngz = (TE + 1 - pz + 1) / 2;
+            const int npts = ngx * ngz * n3;
+            // gradient (differences from the point's own value) and flux
+            for (int t = tid; t < npts; t += NT) {
+                const int le = t / n3, pt = t - le * n3;
+                const int lx = px + 2 * (le % ngx), lz = pz + 2 * (le / ngx);
+                const int exg = ex0 - 1 + lx, ezg = ez0 - 1 + lz;
+                if (exg < 0 || exg >= a.ex || ezg < 0 || ezg >= a.ez) continue;
+                const int pa = pt % q, pb = (pt / q) % q, pc = pt / (q * q);
+                const int xb = lx * P, zb = lz * P;
+                const float uc = u[pc][zb + pb][xb + pa];
+                float s1 = 0.f, s2 = 0.f, s3 = 0.f;
+#pragma unroll
+                for (int m = 0; m < q; ++m) {
+                    s1 = fmaf(Ds[m * q + pa], u[pc][zb + pb][xb + m] - uc, s1);
+                    s2 = fmaf(Ds[m * q + pb], u[pc][zb + m][xb + pa] - uc, s2);
+                    s3 = fmaf(Ds[m * q + pc], u[m][zb + pb][xb + pa] - uc, s3);
+                }
+                const float *G = gshared ? Gs + 6 * pt
+                                         : a.geo + (exg + (int64_t)ezg * a.ex + (int64_t)ey * a.ex * a.ez) *
+                                                       a.geo_stride + 6 * pt;
+                fl[le][0][pt] = G[0] * s1 + G[3] * s2 + G[4] * s3;
+                fl[le][1][pt] = G[3] * s1 + G[1] * s2 + G[5] * s3;
+                fl[le][2][pt] = G[4] * s1 + G[5] * s2 + G[2] * s3;
+            }
+            __syncthreads();
+            // divergence, added to the owned nodes (no two elements of one
+            // colour share a node)
+            for (int t = tid; t < npts; t += NT) {
+                const int le = t / n3, pt = t - le * n3;
+                const int lx = px + 2 * (le % ngx), lz = pz + 2 * (le / ngx);
+                const int exg = ex0 - 1 + lx, ezg = ez0 - 1 + lz;
+                if (exg < 0 || exg >= a.ex || ezg < 0 || ezg >= a.ez) continue;
+                const int pa = pt % q, pb = (pt / q) % q, pc = pt / (q * q);
+                const int ox = lx * P + pa - P, oz = lz * P + pb - P;
+                if (ox < 0 || ox >= OW || oz < 0 || oz >= OW) continue;
+                float v = 0.f;
+#pragma unroll
+                for (int m = 0; m < q; ++m) {
+                    v = fmaf(Ds[pa * q + m], fl[le][0][m + pb * q + pc * q * q], v);
+                    v = fmaf(Ds[pb * q + m], fl[le][1][pa + m * q + pc * q * q], v);
+                    v = fmaf(Ds[pc * q + m], fl[le][2][pa + pb * q + m * q * q], v);
+                }
+                acc[pc][oz][ox] += v;
+            }
+            __syncthreads();
+        }
+        // planes ey p .. ey p + p - 1 are complete: update the owned interior
+        // nodes (a range's first layer below ey0 only completes plane ey0 p)
+        if (ey >= ey0) {
+            for (int t = tid; t < P * OW * OW; t += NT) {
+                const int c = t / (OW * OW), r = t - c * (OW * OW), oz = r / OW, ox = r - oz * OW;
+                const int gx = ex0 * P + ox, gz = ez0 * P + oz, gy = ey * P + c;
+                if (gx < 1 || gx >= a.nx - 1 || gz < 1 || gz >= a.nz - 1 || gy < 1 || gy >= a.ny - 1) continue;
+                const int64_t n = (int64_t)gy * plane + (int64_t)gz * a.nx + gx;
+                float val = fmaf(-a.dt2m[n], acc[c][oz][ox], 2.f * u[c][oz + P][ox + P] - a.pnm1[n]);
+                if (n == a.src_node) {
+                    if (a.trace) *a.trace = val;
+                    if (a.inj) val = add_nc(val, mul_nc(a.inj_coef, a.rhs_next));
+                }
+                a.pnm1[n] = val;
+            }
+        }
+        __syncthreads();
+        // carry the top planes (the next layer's bottom) and clear the rest
+        for (int t = tid; t < OW * OW; t += NT) {
+            const int oz = t / OW, ox = t - oz * OW;
+            acc[0][oz][ox] = acc[P][oz][ox];
+#pragma unroll
+            for (int c = 1; c <= P; ++c) acc[c][oz][ox] = 0.f;
+        }
+        for (int t = tid; t < PW * PW; t += NT) (&u[0][0][0])[t] = (&u[P][0][0])[t];
+        __syncthreads();
+    }
+}
+
 // Leapfrog update of interior nodes (SEMsolver.cpp:85-89) with dt^2/M
 // precomputed, fused with clearing y for the next step and with the source
 // injection of the next time sample.
@@ -440,6 +615,21 @@ static SemSepFn sem_pick_sep(int p, int ty, int nj, bool idx32)
     case 3: return sep_shape<3>(ty, nj, idx32);
     case 4: return sep_shape<4>(ty, nj, idx32);
     case 5: return sep_shape<5>(ty, nj, idx32);
+    default: return nullptr;
+    }
+}
+
+using SemMarchFn = void (*)(SemMarchArgs);
+constexpr int SEM_MARCH_TE = 4;    // elements per tile side
+constexpr int SEM_MARCH_NT = 256;  // threads per block
+static SemMarchFn sem_pick_march(int p)
+{
+    switch (p) {
+    case 1: return sem_march_kernel<1, SEM_MARCH_TE, SEM_MARCH_NT>;
+    case 2: return sem_march_kernel<2, SEM_MARCH_TE, SEM_MARCH_NT>;
+    case 3: return sem_march_kernel<3, SEM_MARCH_TE, SEM_MARCH_NT>;
+    case 4: return sem_march_kernel<4, SEM_MARCH_TE, SEM_MARCH_NT>;
+    case 5: return sem_march_kernel<5, SEM_MARCH_TE, SEM_MARCH_NT>;
     default: return nullptr;
     }
 }
@@ -662,6 +852,35 @@ static phc_status sem_launch(phc_sem *h, int i1, int i2, int inj, float rhs_next
         hipLaunchKernelGGL(fn, dim3(nb), dim3(FD_TK, h->ty), 0, h->stream, a);
         PHC_HIP(hipGetLastError());
         if (e1) PHC_HIP(hipEventRecord(e1, h->stream));
+    } else if (h->kernel == PHC_SEM_ELEMENT_MARCH) {
+        SemMarchArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.pn = h->lev[i2].as<float>();
+        a.pnm1 = h->lev[i1].as<float>();
+        a.dt2m = h->dt2m.as<float>();
+        a.geo = h->geo.as<float>();
+        a.geo_stride = 0;
+        std::memcpy(a.D, h->Dm, sizeof(a.D));
+        a.ex = h->ex; a.ey = h->ey; a.ez = h->ez;
+        a.nx = h->nx; a.ny = h->ny; a.nz = h->nz;
+        a.tiles_x = (h->ex + SEM_MARCH_TE - 1) / SEM_MARCH_TE;
+        a.tiles_z = (h->ez + SEM_MARCH_TE - 1) / SEM_MARCH_TE;
+        // layer ranges along y so that there are about two blocks per CU
+        const int cols = a.tiles_x * a.tiles_z;
+        const int ranges = std::max(1, std::min(h->ey, (2 * h->cus + cols - 1) / cols));
+        a.ychunk = (h->ey + ranges - 1) / ranges;
+        const int nb = cols * ((h->ey + a.ychunk - 1) / a.ychunk);
+        a.src_node = h->src_node;
+        a.inj = inj;
+        a.inj_coef = h->inj_coef;
+        a.rhs_next = rhs_next;
+        a.trace = trace;
+        SemMarchFn fn = sem_pick_march(h->p);
+        if (!fn) return fail(PHC_ERR_INVALID, "no element march kernel for p=%d", h->p);
+        if (e0) PHC_HIP(hipEventRecord(e0, h->stream));
+        hipLaunchKernelGGL(fn, dim3(nb), dim3(SEM_MARCH_NT), 0, h->stream, a);
+        PHC_HIP(hipGetLastError());
+        if (e1) PHC_HIP(hipEventRecord(e1, h->stream));
     } else {
         SemElemArgs a;
         std::memset(&a, 0, sizeof(a));
@@ -795,7 +1014,8 @@ phc_status phc_sem_create(const phc_sem_params *p, phc_sem **out)
     if (!(p->lx > 0) || !(p->ly > 0) || !(p->lz > 0)) return fail(PHC_ERR_INVALID, "bad domain size");
     if (!p->model) return fail(PHC_ERR_INVALID, "model is NULL");
     if (p->nsamples < 0 || (p->nsamples > 0 && !p->rhs)) return fail(PHC_ERR_INVALID, "bad rhs");
-    if (p->kernel != PHC_SEM_SEPARABLE && p->kernel != PHC_SEM_ELEMENT && p->kernel != PHC_SEM_ELEMENT_COLOR)
+    if (p->kernel != PHC_SEM_SEPARABLE && p->kernel != PHC_SEM_ELEMENT && p->kernel != PHC_SEM_ELEMENT_COLOR &&
+        p->kernel != PHC_SEM_ELEMENT_MARCH)
         return fail(PHC_ERR_INVALID, "unknown SEM kernel %d", p->kernel);
     const int64_t nE = (int64_t)p->ex * p->ey * p->ez;
     if (p->src_element < 0 || p->src_element >= nE)

@@ -31,6 +31,9 @@ VMIN, VMAX = 1500.0, 4500.0
 CFL = 0.4
 SPONGE = 40
 LAYER = phc.LB_WIDTH
+# the reference's PML width for this model and source (FDTDinit.hpp:167-172:
+# 4 * vmin / (2.5 f0) / dx = 24 points at 1500 m/s, 10 Hz, 10 m)
+PML_F0 = 10.0
 
 
 def fd_dt(cfl=CFL, vmax=VMAX, L=4, dx=DX):
@@ -185,7 +188,9 @@ def compare(lb, n=256, steps=2000, f0=10.0, seed=20240512, src=None, with_none=T
     def run32(pad, sw, model):
         s = setup(pad, sw)
         sol = phc.FdSolver(s, arith=phc.ARITH_FP32, init_value=0.0)
-        if model is not None:
+        if model == "pml":
+            sol.set_pml(ndamp=pad, f0=PML_F0)
+        elif model is not None:
             sol.set_learned_boundary(model, halo=0.0, layer_damping=layer_damping)
         sol.autotune(steps_per_trial=5)
         sol.set_receivers(rec + pad)
@@ -220,7 +225,8 @@ def compare(lb, n=256, steps=2000, f0=10.0, seed=20240512, src=None, with_none=T
         tr_true = truth
     out["truth_traces"] = tr_true
     out["layer_damping"] = layer_damping
-    runs = [("learned", LAYER, 0, lb), ("sponge40", SPONGE, SPONGE, None)]
+    pml_w = int(phc.lib().phc_fd_pml_width(VMIN, PML_F0, DX))
+    runs = [("learned", LAYER, 0, lb), ("sponge40", SPONGE, SPONGE, None), ("pml", pml_w, 0, "pml")]
     if with_none:
         runs.append(("none", LAYER, 0, None))
     # receiver groups: the centre of each face's 3x3 grid, and the interior line
@@ -242,6 +248,7 @@ def compare(lb, n=256, steps=2000, f0=10.0, seed=20240512, src=None, with_none=T
                 name, n + 2 * pad, wall, out[name]["trace_rel_l2"], out[name]["trace_rel_l2_face_centres"],
                 out[name]["trace_rel_l2_interior"]))
     out["speedup_learned_vs_sponge40"] = round(out["sponge40"]["wall_s"] / out["learned"]["wall_s"], 3)
+    out["speedup_learned_vs_pml"] = round(out["pml"]["wall_s"] / out["learned"]["wall_s"], 3)
     return out
 
 

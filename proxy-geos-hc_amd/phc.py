@@ -686,6 +686,7 @@ def lb_corr_sizes(nx, ny, nz):
 SEM_SEPARABLE = 0
 SEM_ELEMENT = 1
 SEM_ELEMENT_COLOR = 2
+SEM_ELEMENT_MARCH = 3
 
 
 class SemParams(ctypes.Structure):

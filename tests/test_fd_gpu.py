@@ -184,7 +184,7 @@ def test_autotune_keeps_state_and_parity(gpu, oracle, tmp_path):
     assert cfg[0] % cfg[1] == 0 and cfg[2] >= 1
     assert cfg == sol.autotune(steps_per_trial=3, cache_path=cache)  # cache hit
     lines = open(cache).read().splitlines()
-    assert len(lines) == 2 and lines[1].startswith("fd 48 40 100 4 0 0 0 ")
+    assert len(lines) == 2 and lines[1].startswith("fd 48 40 100 4 0 0 0 ")  # requested k mode 0
     src, _ = sol.run(10, 20)
     tr, _ = P.run(30)
     assert np.array_equal(src, tr[10:])

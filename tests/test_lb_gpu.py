@@ -176,6 +176,51 @@ def test_learned_steps_match_cpu_emulation(gpu, oracle):
     assert np.array_equal(sol2.previous(), sol.previous())
 
 
+def test_learned_steps_full_size_264(gpu, oracle):
+    """The learned grid of the bench (256^3 region + 4-point layer = 264^3):
+    from a smooth field that fills the layers, 3 learned steps (reference
+    rounding) against the CPU emulation -- stencil bit-exact, corrections
+    to the MFMA inference tolerance."""
+    banner("learned steps at 264^3 vs CPU emulation")
+    n = 264
+    rng = np.random.default_rng(15)
+    s = thin_solver(gpu, n, 8, src=(130, 131, 129))
+    theta = (rng.normal(0, 0.3, (lo.FACES, lo.NP)) * 0.5).astype(np.float32)
+    lb = gpu.LearnedBoundary(seed=2)
+    lb.set_params(theta)
+    P = oracle.FdProblem(nx=n, L=4, xs=s.xs, ys=s.ys, zs=s.zs, f0=s.f0, cfl=0.4, model=s.vp.copy(),
+                         nsamples=s.nsamples, sponge_width=0)
+    x = np.arange(n + 8, dtype=np.float64)
+    f1 = (np.sin(0.043 * x)[:, None, None] * np.cos(0.051 * x)[None, :, None] *
+          np.sin(0.037 * x + 0.3)[None, None, :]).astype(np.float32)
+    f1[:4] = f1[-4:] = 0.0        # the halo holds the constant h = 0
+    f1[:, :4] = f1[:, -4:] = 0.0
+    f1[:, :, :4] = f1[:, :, -4:] = 0.0
+    f0 = (0.98 * f1).astype(np.float32)
+    P.p1[:] = f1.reshape(-1)      # current (i2)
+    P.p0[:] = f0.reshape(-1)      # previous (i1)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF, init_value=0.0)
+    sol.set_field(sol.i2, f1)
+    sol.set_field(sol.i1, f0)
+    sol.set_learned_boundary(lb, halo=0.0)
+    vp3 = s.vp.reshape(n, n, n)
+    c1 = float(s.cx[1])
+    for it in range(3):
+        corr = lo.corrections(theta, P.current().astype(np.float64), P.previous().astype(np.float64), vp3,
+                              (n, n, n), 0.0, c1)
+        assert max(np.abs(c).max() for c in corr) > 0
+        P.run(1)
+        cur = P.current()
+        upd = cur.astype(np.float64)
+        lo.apply_corrections(upd, corr, (n, n, n))
+        cur[...] = upd.astype(np.float32)
+        sol.step(it)
+    g, r = sol.current(), P.current()
+    rel = np.linalg.norm((g - r).astype(np.float64)) / np.linalg.norm(r.astype(np.float64))
+    print("[lb-gpu] 264^3 learned steps rel L2 %.3e" % rel, flush=True)
+    assert rel < 1e-4, rel
+
+
 def test_training_reduces_the_loss(gpu):
     banner("training")
     s, sol = wide_run(gpu, 56, 10, 90, (30, 27, 26), seed=3)

@@ -12,7 +12,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = [0, 1, 2]  # SEM_SEPARABLE, SEM_ELEMENT, SEM_ELEMENT_COLOR
+KERNELS = [0, 1, 2, 3]  # SEM_SEPARABLE, SEM_ELEMENT, SEM_ELEMENT_COLOR, SEM_ELEMENT_MARCH
 
 
 def banner(msg):
@@ -249,6 +249,33 @@ def test_per_step_surface_equals_fused_run(gpu, oracle, kernel):
         # element kernel: FP32 atomic scatter, summation order varies by run
         assert rel_l2(a.current(), b.current()) < 1e-6
         assert rel_l2(a.previous(), b.previous()) < 1e-6
+
+
+@pytest.mark.parametrize("ex,ey,ez,order", [(9, 7, 6, 2), (13, 3, 5, 4), (6, 9, 10, 3), (5, 5, 5, 5)])
+def test_element_march_tiles_and_ranges(gpu, oracle, ex, ey, ez, order):
+    """The write-once element kernel on meshes whose x and z extents are not
+    multiples of its 4-element tiles and whose y layers are cut into ranges:
+    within 1e-5 of the oracle, and two runs give the same bits (no atomics)."""
+    banner("element march %d x %d x %d order %d" % (ex, ey, ez, order))
+    h = 20.0
+    rng = np.random.default_rng(ex * 100 + ez)
+    model = (1600 + 1800 * rng.random(ex * ey * ez)).astype(np.float32)
+    dt = {2: 0.0006, 3: 0.0004, 4: 0.0003, 5: 0.0002}[order]
+    src = (h * (ex - 1) + 1, h * (ey // 2) + 1, h * 1 + 1)
+    s, O = pair(gpu, oracle, gpu.SEM_ELEMENT_MARCH, ex=ex, ey=ey, ez=ez, lx=h * ex, ly=h * ey, lz=h * ez,
+                order=order, dt=dt, f0=25.0, nsamples=80, model=model, src=src)
+    src_t, _ = s.run(70)
+    tr, _ = O.run(70)
+    assert rel_l2(s.current(), O.current()) < 1e-5
+    assert rel_l2(s.previous(), O.previous()) < 1e-5
+    assert rel_l2(src_t, tr) < 1e-5
+    again = gpu.SemSolver(ex=ex, ey=ey, ez=ez, lx=h * ex, ly=h * ey, lz=h * ez, order=order, dt=dt, f0=25.0,
+                          nsamples=80, model=model, src_point=src, kernel=gpu.SEM_ELEMENT_MARCH)
+    for _ in range(70):
+        again.step()
+    assert np.array_equal(again.current(), s.current())
+    f = s.current().reshape(s.ny, s.nz, s.nx)
+    assert not f[:, :, 0].any() and not f[:, :, -1].any() and not f[:, 0, :].any() and not f[0].any()
 
 
 def test_separable_and_element_kernels_agree(gpu):
