@@ -77,7 +77,11 @@ typedef struct {
     /* kernel tiling; 0 selects the built-in default for the radius */
     int tile_j;        /* output rows per block along j (block_waves x 1, 2 or 4) */
     int block_waves;   /* wavefronts per block: 4, 8 or 16 */
-    int chunk_i;       /* planes per block along the marching axis i */
+    int chunk_i;       /* planes per block along the marching axis i; -m
+                        * (m = 1..4): per-XCD schedule, the axis cut into
+                        * 8 >> (m-1) plane ranges, each XCD's tiles marching
+                        * one range (the table phc_sem_schedule_table
+                        * describes, with one-plane groups) */
     int index64;       /* 1: force 64-bit element offsets (default: 32-bit when
                         * every array is below 4 GiB) */
     int k_remainder;   /* a last, partial 64-point tile along k: 0 (default)

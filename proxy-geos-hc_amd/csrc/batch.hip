@@ -115,7 +115,8 @@ fd_batch_kernel(FdStepArgs c, const ShotDesc *__restrict__ shots, const WorkItem
                     fd_rem_points<R, ARITH, IT>(c, g, ((int64_t)wi.jt * BATCH_SLAB_PASSES + m) * T, T);
                 continue;
             }
-            fd_tile<R, TY, NJ, ARITH, IT>(c, g, wi.kt, wi.jt, wi.ic, tile);
+            const int i0 = wi.ic * c.chunk_i;
+            fd_tile<R, TY, NJ, ARITH, IT>(c, g, wi.kt, wi.jt, i0, min(i0 + c.chunk_i, g.ihi), tile);
         }
     }
 }

@@ -29,10 +29,32 @@
 #include "lb.hpp"
 #include "march.hpp"
 #include "phc_common.hpp"
+#include "xcd_sched.hpp"
 
 namespace phc {
 
 // ----------------------------------------------------------------- kernels
+
+// Tile column and plane range of a marching block: from the per-XCD table,
+// or tiles in XCD block order with chunks of chunk_i planes.
+__device__ __forceinline__ void fd_block_range(const FdStepArgs &a, int &kt, int &jt, int &i0, int &i1)
+{
+    int w;
+    if (a.sched) {
+        const int4 e = a.sched[blockIdx.x];
+        w = e.x;
+        i0 = e.y;
+        i1 = e.z;
+    } else {
+        const int b = xcd_block(blockIdx.x, a.nb_main ? a.nb_main : gridDim.x);
+        const int nt = a.tiles_k * a.tiles_j, ic = b / nt;
+        w = b - ic * nt;
+        i0 = a.ilo + ic * a.chunk_i;
+        i1 = min(i0 + a.chunk_i, a.ihi);
+    }
+    kt = w % a.tiles_k;
+    jt = w / a.tiles_k;
+}
 
 template <int R, int TY, int NJ, int ARITH, typename IT>
 __global__ void __launch_bounds__(FD_TK *TY)
@@ -44,10 +66,10 @@ fd_step_kernel(FdStepArgs a)
         fd_rem_points<R, ARITH, IT>(a, a, (int64_t)(blockIdx.x - a.nb_main) * (FD_TK * TY), FD_TK * TY);
         return;
     }
-    const int b = xcd_block(blockIdx.x, a.nb_main ? a.nb_main : gridDim.x);
-    const int kt = b % a.tiles_k;
-    const int rest = b / a.tiles_k;
-    fd_tile<R, TY, NJ, ARITH, IT>(a, a, kt, rest % a.tiles_j, rest / a.tiles_j, tile);
+    int kt, jt, i0, i1;
+    fd_block_range(a, kt, jt, i0, i1);
+    if (i0 >= i1) return;  // padding of a shorter XCD list (whole block)
+    fd_tile<R, TY, NJ, ARITH, IT>(a, a, kt, jt, i0, i1, tile);
 }
 
 // computeOneStepPML (FDTDkernels.hpp:87-104) in one launch: the march over
@@ -63,10 +85,10 @@ fd_step_pml_kernel(FdStepArgs a)
         fd_rem_points<R, ARITH, IT, true>(a, a, (int64_t)(blockIdx.x - a.nb_main) * (FD_TK * TY), FD_TK * TY);
         return;
     }
-    const int b = xcd_block(blockIdx.x, a.nb_main ? a.nb_main : gridDim.x);
-    const int kt = b % a.tiles_k;
-    const int rest = b / a.tiles_k;
-    fd_tile<R, TY, NJ, ARITH, IT, true>(a, a, kt, rest % a.tiles_j, rest / a.tiles_j, tile);
+    int kt, jt, i0, i1;
+    fd_block_range(a, kt, jt, i0, i1);
+    if (i0 >= i1) return;  // padding of a shorter XCD list (whole block)
+    fd_tile<R, TY, NJ, ARITH, IT, true>(a, a, kt, jt, i0, i1, tile);
 }
 
 // addRHS (FDTDkernels.hpp:121-128): one point.
@@ -202,6 +224,11 @@ struct phc_fd {
     int ty = 4, nj = 2, chunk_i = 0, index64 = 0;
     int krem_mode = 0;  // phc_fd_params.k_remainder (as requested)
     int krem_eff = 0;   // the mode the launches use (the autotuner may pick -1 for 0)
+    // chunk_i < 0: the per-XCD schedule table, rebuilt when its key changes
+    int cus = 256;
+    DevBuf sched;
+    int sched_nb = 0;
+    long long sched_key = -1;
     int nrec = 0;
     std::vector<int64_t> rec_offs;
     int last_n = 0, last_nrec = 0;
@@ -250,7 +277,8 @@ static phc_status fd_validate(const phc_fd_params *p)
             return fail(PHC_ERR_INVALID, "no kernel for half width %d, tile_j %d, %d waves per block",
                         R, tj, ty);
     }
-    if (p->chunk_i < 0) return fail(PHC_ERR_INVALID, "negative chunk_i");
+    if (p->chunk_i < -4)
+        return fail(PHC_ERR_INVALID, "chunk_i %d: planes per chunk, or -1..-4 for the per-XCD schedule", p->chunk_i);
     if (p->k_remainder < -1 || p->k_remainder > 1)
         return fail(PHC_ERR_INVALID, "k_remainder must be -1, 0 or 1 (got %d)", p->k_remainder);
     const double pts = (double)(p->nx + 2 * R) * (p->ny + 2 * R) * (p->nz + 2 * R + 32);
@@ -331,7 +359,8 @@ static void fd_fill_args(const phc_fd *h, FdStepArgs &a, int ca, int cb)
 
 static int fd_num_blocks(const FdStepArgs &a)
 {
-    const int chunks = std::max(1, (a.ihi - a.ilo + a.chunk_i - 1) / a.chunk_i);
+    // (with a per-XCD schedule the table's length replaces this count)
+    const int chunks = a.chunk_i > 0 ? std::max(1, (a.ihi - a.ilo + a.chunk_i - 1) / a.chunk_i) : 1;
     return a.tiles_k * a.tiles_j * chunks;
 }
 
@@ -366,23 +395,72 @@ static phc_status fd_materialize(phc_fd *h, int level)
     return PHC_OK;
 }
 
-static phc_status fd_launch_step(phc_fd *h, FdStepArgs a, bool damp)
-{
-    FdKernelFn fn = fd_pick_kernel(h->R, h->ty, h->nj, h->arith,
-                                   !h->index64 && h->L.total * 4 < ((int64_t)1 << 32), h->pml_on != 0);
-    if (!fn) return fail(PHC_ERR_INVALID, "no kernel for R=%d block_waves=%d rows/thread=%d", h->R, h->ty, h->nj);
-    a.damp_prev = damp ? 1 : 0;
-    const int nb = fd_plan_blocks(a, h->krem_eff, FD_TK * h->ty);
-    hipLaunchKernelGGL(fn, dim3(nb), dim3(FD_TK, h->ty), 0, h->stream, a);
-    PHC_HIP(hipGetLastError());
-    return PHC_OK;
-}
-
 static void fd_drop_graph(phc_fd *h)
 {
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     h->gexec = nullptr;
     h->g_it0 = h->g_n = h->g_cur = h->g_damp = h->g_nrec = -1;
+}
+
+static FdKernelFn fd_handle_kernel(const phc_fd *h)
+{
+    return fd_pick_kernel(h->R, h->ty, h->nj, h->arith, !h->index64 && h->L.total * 4 < ((int64_t)1 << 32),
+                          h->pml_on != 0);
+}
+
+// The per-XCD schedule (chunk_i = -m) for the tiles `a` plans: ranges of
+// [ilo, ihi) x the tile columns, dealt over as many slots per XCD as the
+// kernel keeps resident per CU.  Built on the host and uploaded when its key
+// (tiles, planes, split, block shape) changes; never during a capture
+// (phc_fd_run prepares it before capturing).
+static phc_status fd_prepare_sched(phc_fd *h, const FdStepArgs &a)
+{
+    const int ncol = a.tiles_k * a.tiles_j;
+    const long long key = ((((long long)ncol * 4096 + (a.ihi - a.ilo)) * 8 + (-h->chunk_i)) * 64 + h->ty) * 64 +
+                          h->nj + ((long long)a.ilo << 50);
+    if (h->sched_key == key && h->sched.ptr) return PHC_OK;
+    FdKernelFn fn = fd_handle_kernel(h);
+    if (!fn) return fail(PHC_ERR_INVALID, "no kernel for R=%d block_waves=%d rows/thread=%d", h->R, h->ty, h->nj);
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(fn), FD_TK * h->ty, 0) !=
+            hipSuccess || per_cu < 1)
+        per_cu = 1;
+    std::vector<int4> tab;
+    build_xcd_schedule(a.ihi - a.ilo, a.ilo, ncol, std::max(h->cus / 8, 1) * per_cu, -h->chunk_i, 1, tab);
+    PHC_HIP(hipStreamSynchronize(h->stream));  // queued launches may still read the old table
+    fd_drop_graph(h);
+    PHC_TRY(h->sched.alloc(sizeof(int4) * tab.size()));
+    PHC_HIP(hipMemcpy(h->sched.ptr, tab.data(), sizeof(int4) * tab.size(), hipMemcpyHostToDevice));
+    h->sched_nb = (int)tab.size();
+    h->sched_key = key;
+    return PHC_OK;
+}
+
+// Blocks to launch for `a` (tiles, k-remainder slab, schedule table set in a).
+static phc_status fd_plan_launch(phc_fd *h, FdStepArgs &a, int *nb)
+{
+    *nb = fd_plan_blocks(a, h->krem_eff, FD_TK * h->ty);
+    a.sched = nullptr;
+    if (h->chunk_i < 0) {
+        const int slab = a.nb_main ? *nb - a.nb_main : 0;
+        PHC_TRY(fd_prepare_sched(h, a));
+        a.sched = h->sched.as<int4>();
+        a.nb_main = slab ? h->sched_nb : 0;
+        *nb = h->sched_nb + slab;
+    }
+    return PHC_OK;
+}
+
+static phc_status fd_launch_step(phc_fd *h, FdStepArgs a, bool damp)
+{
+    FdKernelFn fn = fd_handle_kernel(h);
+    if (!fn) return fail(PHC_ERR_INVALID, "no kernel for R=%d block_waves=%d rows/thread=%d", h->R, h->ty, h->nj);
+    a.damp_prev = damp ? 1 : 0;
+    int nb = 0;
+    PHC_TRY(fd_plan_launch(h, a, &nb));
+    hipLaunchKernelGGL(fn, dim3(nb), dim3(FD_TK, h->ty), 0, h->stream, a);
+    PHC_HIP(hipGetLastError());
+    return PHC_OK;
 }
 
 // One full step from levels cb (current) and ca (previous, overwritten by the
@@ -460,6 +538,12 @@ phc_status phc_fd_create(const phc_fd_params *p, phc_fd **out)
     auto bail = [&](phc_status s) { phc_fd_destroy(h); return s; };
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(PHC_ERR_HIP, "hipStreamCreate failed"));
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            h->cus = cus;
+    }
     const size_t bytes = sizeof(float) * (size_t)h->L.total;
     if ((st = h->lev[0].alloc(bytes)) || (st = h->lev[1].alloc(bytes)) || (st = h->vp.alloc(bytes)))
         return bail(st);
@@ -666,6 +750,12 @@ phc_status phc_fd_run(phc_fd *h, int it0, int nsteps, int *cur)
         if (!(h->gexec && h->g_it0 == it0 && h->g_n == nsteps && h->g_cur == c0 &&
               h->g_damp == damp_key && h->g_nrec == h->nrec)) {
             fd_drop_graph(h);
+            {   // the schedule table is uploaded now, not inside the capture
+                FdStepArgs a;
+                fd_fill_args(h, a, 1 - c0, c0);
+                int nb = 0;
+                PHC_TRY(fd_plan_launch(h, a, &nb));
+            }
             hipGraph_t graph = nullptr;
             PHC_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
             phc_status st = fd_enqueue_run(h, it0, nsteps, c0, damp0, nullptr);
@@ -927,7 +1017,8 @@ static bool fd_tune_lookup(const char *path, const phc_fd *h, int *ty, int *tj, 
                         &b, &c, &ku, &ms) != 13)
             continue;
         if (!std::strcmp(tag, "fd") && nx == h->nx && ny == h->ny && nz == h->nz && R == h->R && ar == h->arith &&
-            pml == h->pml_on && km == h->krem_mode && a > 0 && b % a == 0 && c > 0 && ku >= -1 && ku <= 1 &&
+            pml == h->pml_on && km == h->krem_mode && a > 0 && b % a == 0 && c >= -4 && c != 0 && ku >= -1 &&
+            ku <= 1 &&
             (ku == km || km == 0) && fd_pick_kernel(h->R, a, b / a, h->arith, true, false)) {
             *ty = a; *tj = b; *ci = c; *km_used = ku;
             hit = true;
@@ -963,7 +1054,7 @@ extern "C" phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char
         PHC_HIP(hipEventCreate(&e0));
         PHC_HIP(hipEventCreate(&e1));
         const int shapes[][2] = {{4, 8}, {4, 16}, {8, 8}, {8, 16}, {16, 16}, {16, 32}};
-        const int chunks[] = {64, 128, 256, 512, 1024};
+        const int chunks[] = {64, 128, 256, 512, 1024, -1, -2, -3, -4};
         double best = 1e30;
         int bty = h->ty, bnj = h->nj, bci = h->chunk_i, bku = h->krem_eff;
         phc_status st = PHC_OK;
@@ -976,7 +1067,7 @@ extern "C" phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char
             if (!fd_pick_kernel(h->R, sh[0], sh[1] / sh[0], h->arith, true, false)) continue;
             for (int c : chunks) {
                 if (c > h->nx && c != chunks[0]) continue;
-                h->ty = sh[0]; h->nj = sh[1] / sh[0]; h->chunk_i = std::min(c, h->nx); h->krem_eff = ku;
+                h->ty = sh[0]; h->nj = sh[1] / sh[0]; h->chunk_i = c < 0 ? c : std::min(c, h->nx); h->krem_eff = ku;
                 float ms = 0.f;
                 for (int rep = 0; rep < 2 && st == PHC_OK; ++rep) {  // first pass warms up
                     if (hipEventRecord(e0, h->stream) != hipSuccess) st = fail(PHC_ERR_HIP, "event record failed");

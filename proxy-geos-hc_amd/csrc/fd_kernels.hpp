@@ -75,6 +75,11 @@ struct FdStepArgs {
     // from nb_main on update the slab [krem, khi) one point per thread
     int nb_main;        // marching blocks (0: every block marches)
     int krem;
+    // per-XCD schedule (null: one block per (tile, chunk of chunk_i planes)
+    // in XCD block order): hardware block b < nb_main marches tile column
+    // sched[b].x (k tile + tiles_k * j tile) over planes [sched[b].y,
+    // sched[b].z); an empty range does nothing
+    const int4 *sched;
 };
 
 } // namespace phc

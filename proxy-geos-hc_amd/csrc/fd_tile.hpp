@@ -101,12 +101,12 @@ struct FdTileShape {
     static constexpr int LH = TJ + 2 * R;
 };
 
-// March of one tile column (k-tile kt, j-tile jt, i-chunk ic) of one grid.
+// March of one tile column (k-tile kt, j-tile jt) of one grid over planes [i0, i1).
 // PML: points outside the PML's inner box take the pml3D update (the
 // stencil values come from the same register ring and LDS tile), so one
 // launch covers the slabs and the inner box.
 template <int R, int TY, int NJ, int ARITH, typename IT, bool PML = false, class G>
-__device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt, int jt, int ic,
+__device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt, int jt, int i0, int i1,
                                         float (*tile)[FdTileShape<R, TY, NJ>::LH][FdTileShape<R, TY, NJ>::LW])
 {
     constexpr int TJ = TY * NJ;
@@ -119,8 +119,6 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
     const FdLayout &L = g.L;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int k0 = kt * FD_TK, j0 = jt * TJ;
-    const int i0 = g.ilo + ic * a.chunk_i;
-    const int i1 = min(i0 + a.chunk_i, g.ihi);
     const int k = k0 + tx;
     const IT pitch = (IT)L.pitch_j * S;
 

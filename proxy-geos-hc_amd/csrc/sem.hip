@@ -35,6 +35,7 @@
 
 #include "fd_kernels.hpp"
 #include "march.hpp"
+#include "xcd_sched.hpp"
 #include "phc_common.hpp"
 #include "sem_setup.hpp"
 
@@ -375,15 +376,23 @@ sem_elem_kernel(SemElemArgs a)
 // one by one:
 //  * the node patch of the layer's (TE+1)^2 elements (the tile plus one
 //    element below it in x and z, whose faces carry owned nodes) is staged
-//    in LDS;
-//  * the elements are taken in the 4 colours of their (x, z) index parities:
-//    elements of one colour share no node, so each element's K_e u_e (the
-//    sum-factorised gradient, geometric factors, divergence, as in
-//    sem_elem_kernel) is added to an LDS accumulator of the owned nodes
-//    with plain adds, in a fixed order;
+//    in LDS, its p+1 node planes in a ring (a layer's top plane is the next
+//    layer's bottom one and stays in place);
+//  * each lane takes one node plane (local y index c) of one element: it
+//    keeps the plane's (p+1)^2 values in registers, forms the x and z
+//    derivatives there (the 1D derivative matrix entries are kernel-argument
+//    constants), reads the y derivative's p+1 values from the patch, and
+//    applies the geometric factors (SEMQkGLIntegrals.hpp:363-520: gradient,
+//    flux, divergence); the y fluxes go through LDS for the y part of the
+//    divergence, the x and z ones stay in registers;
+//  * each element's K_e u_e is left in LDS and every owned node sums the
+//    (at most 4) contributions of the layer's elements that hold it, in a
+//    fixed order, plus the carried contribution of the layer below to its
+//    bottom plane: no atomics, the same bits on every run;
 //  * the node planes of the layer below its top face are then complete: the
-//    block applies the leapfrog update to its owned interior nodes and writes
-//    each of them once; the top plane is carried to the next layer.
+//    owner applies the leapfrog update to its interior nodes and writes each
+//    of them once; the top plane's sum is carried to the next layer in
+//    registers (a thread keeps the same node columns all along).
 // A range of layers [ey0, ey1) owns node planes [ey0 p, ey1 p) and first
 // computes layer ey0 - 1 to complete its lowest plane.  Each node is written
 // by exactly one block and only level i2 is read across blocks, so the new
@@ -403,21 +412,31 @@ struct SemMarchArgs {
     float *trace;
 };
 
-template <int P, int TE, int NT>
+// Elements per tile side of the march kernel: (TE+1)^2 (p+1) element planes
+// fit one 256-thread block.
+constexpr int sem_march_te(int p)
+{
+    return p == 1 ? 10 : p == 2 ? 8 : p == 3 ? 7 : p == 4 ? 6 : 5;
+}
+constexpr int SEM_MARCH_NT = 256;
+
+template <int P, int NT>
 __global__ void __launch_bounds__(NT) sem_march_kernel(SemMarchArgs a)
 {
-    constexpr int q = P + 1, n3 = q * q * q;
-    constexpr int PW = (TE + 1) * P + 1;  // patch nodes per axis (x, z)
-    constexpr int OW = TE * P;            // owned nodes per axis
-    constexpr int NG = (TE + 2) / 2;      // elements per axis of one colour (at most)
-    constexpr int NPF = (P * PW * PW + NT - 1) / NT;  // prefetched patch values per thread
-    __shared__ float u[q][PW][PW];        // layer's node patch [plane][z][x]
-    __shared__ float fl[NG * NG][3][n3];  // fluxes of one colour's elements
-    __shared__ float acc[q][OW][OW];      // K u of the owned nodes [plane][z][x]
-    __shared__ float Gs[6 * n3];          // geometric factors (one element: the box mesh)
-    __shared__ float Ds[q * q];
+    constexpr int TE = sem_march_te(P);
+    constexpr int q = P + 1, n2 = q * q, n3 = q * n2;
+    constexpr int NE = (TE + 1) * (TE + 1);            // elements of a layer's patch
+    constexpr int PW = (TE + 1) * P + 1;               // patch nodes per axis (x, z)
+    constexpr int OW = TE * P;                         // owned nodes per axis
+    constexpr int NPF = (P * PW * PW + NT - 1) / NT;   // prefetched patch values per thread
+    constexpr int NCOL = (OW * OW + NT - 1) / NT;      // owned node columns per thread
+    static_assert(NE * q <= NT, "one lane per element plane");
+    __shared__ float u[q][PW][PW];    // node patch, ring of planes [y][z][x]
+    __shared__ float fv[NE][q][n2];   // y fluxes, then K_e u_e, per element [y][z][x]
+    __shared__ float Gs[n3 * 6];      // geometric factors (one element: the box mesh)
+    __shared__ float Ds[n2];
     const int tid = threadIdx.x;
-    const int b = blockIdx.x;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
     const int txi = b % a.tiles_x, rest = b / a.tiles_x;
     const int tzi = rest % a.tiles_z, yr = rest / a.tiles_z;
     const int ex0 = txi * TE, ez0 = tzi * TE;
@@ -425,11 +444,16 @@ __global__ void __launch_bounds__(NT) sem_march_kernel(SemMarchArgs a)
     const int X0 = ex0 * P - P, Z0 = ez0 * P - P;  // patch origin (node coordinates)
     const int64_t plane = (int64_t)a.nx * a.nz;
     const bool gshared = a.geo_stride == 0;
-    for (int t = tid; t < q * q; t += NT) Ds[t] = a.D[t];
+    for (int t = tid; t < n2; t += NT) Ds[t] = a.D[t];
     if (gshared)
         for (int t = tid; t < 6 * n3; t += NT) Gs[t] = a.geo[t];
-    for (int t = tid; t < q * OW * OW; t += NT) (&acc[0][0][0])[t] = 0.f;
-    // patch value t of planes 1..P of layer ey (plane ey P + 1 + t / PW^2)
+    // the lane's element plane
+    const int le = tid / q, pc = tid - le * q;
+    const int lx = le % (TE + 1), lz = le / (TE + 1);
+    const int exg = ex0 - 1 + lx, ezg = ez0 - 1 + lz;
+    const bool act = le < NE && exg >= 0 && exg < a.ex && ezg >= 0 && ezg < a.ez;
+    const int xb = lx * P, zb = lz * P;
+    // patch value t of the planes above the bottom one of layer ey
     auto patch_load = [&](int ey, int t) {
         const int c = 1 + t / (PW * PW), r = t % (PW * PW), z = r / PW, x = r - z * PW;
         const int gx = X0 + x, gz = Z0 + z;
@@ -438,9 +462,12 @@ __global__ void __launch_bounds__(NT) sem_march_kernel(SemMarchArgs a)
             v = a.pn[(int64_t)(ey * P + c) * plane + (int64_t)gz * a.nx + gx];
         return v;
     };
+    float carry[NCOL];
+#pragma unroll
+    for (int c = 0; c < NCOL; ++c) carry[c] = 0.f;
     float pf[NPF];
     const int eyf = max(ey0 - 1, 0);
-    // the first layer's bottom plane, and its other planes into registers
+    int base = 0;  // ring slot of the layer's bottom plane
     for (int t = tid; t < PW * PW; t += NT) {
         const int z = t / PW, x = t - z * PW, gx = X0 + x, gz = Z0 + z;
         u[0][z][x] = (gx >= 0 && gx < a.nx && gz >= 0 && gz < a.nz)
@@ -448,13 +475,25 @@ __global__ void __launch_bounds__(NT) sem_march_kernel(SemMarchArgs a)
     }
 #pragma unroll
     for (int f = 0; f < NPF; ++f) pf[f] = patch_load(eyf, tid + f * NT);
+    __syncthreads();  // Ds, Gs
+    float dcol[q], drow[q];  // D[m][c], D[c][m] of the lane's plane c
+#pragma unroll
+    for (int m = 0; m < q; ++m) {
+        dcol[m] = Ds[m * q + pc];
+        drow[m] = Ds[pc * q + m];
+    }
     for (int ey = eyf; ey < ey1; ++ey) {
-        // planes 1..P of this layer from the registers (plane 0 is the
-        // previous layer's top plane, already in place)
+        int slot[q];  // ring slot of the layer's plane c
+#pragma unroll
+        for (int c = 0; c < q; ++c) slot[c] = (base + c) % q;
+        // planes 1..P of this layer from the registers
 #pragma unroll
         for (int f = 0; f < NPF; ++f) {
             const int t = tid + f * NT;
-            if (t < P * PW * PW) (&u[1][0][0])[t] = pf[f];
+            if (t < P * PW * PW) {
+                const int c = 1 + t / (PW * PW);
+                (&u[slot[c]][0][0])[t % (PW * PW)] = pf[f];
+            }
         }
         __syncthreads();
         // the next layer's planes load while this layer is computed
@@ -462,82 +501,101 @@ __global__ void __launch_bounds__(NT) sem_march_kernel(SemMarchArgs a)
 #pragma unroll
             for (int f = 0; f < NPF; ++f) pf[f] = patch_load(ey + 1, tid + f * NT);
         }
-#pragma unroll 1
-        for (int col = 0; col < 4; ++col) {
-            const int px = col & 1, pz = col >> 1;
-            const int ngx = (TE + 1 - px + 1) / 2, ngz = (TE + 1 - pz + 1) / 2;
-            const int npts = ngx * ngz * n3;
-            // gradient (differences from the point's own value) and flux
-            for (int t = tid; t < npts; t += NT) {
-                const int le = t / n3, pt = t - le * n3;
-                const int lx = px + 2 * (le % ngx), lz = pz + 2 * (le / ngx);
-                const int exg = ex0 - 1 + lx, ezg = ez0 - 1 + lz;
-                if (exg < 0 || exg >= a.ex || ezg < 0 || ezg >= a.ez) continue;
-                const int pa = pt % q, pb = (pt / q) % q, pc = pt / (q * q);
-                const int xb = lx * P, zb = lz * P;
-                const float uc = u[pc][zb + pb][xb + pa];
-                float s1 = 0.f, s2 = 0.f, s3 = 0.f;
+        // gradient and flux of the lane's plane
+        float U[q][q], F0[q][q], F1[q][q];
+        if (act) {
+            const float(*up)[PW] = u[slot[pc]];
 #pragma unroll
-                for (int m = 0; m < q; ++m) {
-                    s1 = fmaf(Ds[m * q + pa], u[pc][zb + pb][xb + m] - uc, s1);
-                    s2 = fmaf(Ds[m * q + pb], u[pc][zb + m][xb + pa] - uc, s2);
-                    s3 = fmaf(Ds[m * q + pc], u[m][zb + pb][xb + pa] - uc, s3);
-                }
-                const float *G = gshared ? Gs + 6 * pt
-                                         : a.geo + (exg + (int64_t)ezg * a.ex + (int64_t)ey * a.ex * a.ez) *
-                                                       a.geo_stride + 6 * pt;
-                fl[le][0][pt] = G[0] * s1 + G[3] * s2 + G[4] * s3;
-                fl[le][1][pt] = G[3] * s1 + G[1] * s2 + G[5] * s3;
-                fl[le][2][pt] = G[4] * s1 + G[5] * s2 + G[2] * s3;
-            }
-            __syncthreads();
-            // divergence, added to the owned nodes (no two elements of one
-            // colour share a node)
-            for (int t = tid; t < npts; t += NT) {
-                const int le = t / n3, pt = t - le * n3;
-                const int lx = px + 2 * (le % ngx), lz = pz + 2 * (le / ngx);
-                const int exg = ex0 - 1 + lx, ezg = ez0 - 1 + lz;
-                if (exg < 0 || exg >= a.ex || ezg < 0 || ezg >= a.ez) continue;
-                const int pa = pt % q, pb = (pt / q) % q, pc = pt / (q * q);
-                const int ox = lx * P + pa - P, oz = lz * P + pb - P;
-                if (ox < 0 || ox >= OW || oz < 0 || oz >= OW) continue;
-                float v = 0.f;
+            for (int pb = 0; pb < q; ++pb)
 #pragma unroll
-                for (int m = 0; m < q; ++m) {
-                    v = fmaf(Ds[pa * q + m], fl[le][0][m + pb * q + pc * q * q], v);
-                    v = fmaf(Ds[pb * q + m], fl[le][1][pa + m * q + pc * q * q], v);
-                    v = fmaf(Ds[pc * q + m], fl[le][2][pa + pb * q + m * q * q], v);
+                for (int pa = 0; pa < q; ++pa) U[pb][pa] = up[zb + pb][xb + pa];
+            const float *Gg = gshared ? Gs : a.geo + (exg + (int64_t)ezg * a.ex + (int64_t)ey * a.ex * a.ez) *
+                                                         a.geo_stride;
+#pragma unroll
+            for (int pb = 0; pb < q; ++pb)
+#pragma unroll
+                for (int pa = 0; pa < q; ++pa) {
+                    const float uc = U[pb][pa];
+                    float s1 = 0.f, s2 = 0.f, s3 = 0.f;
+#pragma unroll
+                    for (int m = 0; m < q; ++m) {
+                        s1 = fmaf(a.D[m * q + pa], U[pb][m] - uc, s1);
+                        s2 = fmaf(a.D[m * q + pb], U[m][pa] - uc, s2);
+                        s3 = fmaf(dcol[m], u[slot[m]][zb + pb][xb + pa] - uc, s3);
+                    }
+                    const float *G = Gg + 6 * (pa + q * pb + n2 * pc);
+                    const float g0 = G[0], g1 = G[1], g2 = G[2], g3 = G[3], g4 = G[4], g5 = G[5];
+                    F0[pb][pa] = g0 * s1 + g3 * s2 + g4 * s3;
+                    F1[pb][pa] = g3 * s1 + g1 * s2 + g5 * s3;
+                    fv[le][pc][pb * q + pa] = g4 * s1 + g5 * s2 + g2 * s3;
                 }
-                acc[pc][oz][ox] += v;
-            }
-            __syncthreads();
         }
-        // planes ey p .. ey p + p - 1 are complete: update the owned interior
-        // nodes (a range's first layer below ey0 only completes plane ey0 p)
-        if (ey >= ey0) {
-            for (int t = tid; t < P * OW * OW; t += NT) {
-                const int c = t / (OW * OW), r = t - c * (OW * OW), oz = r / OW, ox = r - oz * OW;
-                const int gx = ex0 * P + ox, gz = ez0 * P + oz, gy = ey * P + c;
-                if (gx < 1 || gx >= a.nx - 1 || gz < 1 || gz >= a.nz - 1 || gy < 1 || gy >= a.ny - 1) continue;
-                const int64_t n = (int64_t)gy * plane + (int64_t)gz * a.nx + gx;
-                float val = fmaf(-a.dt2m[n], acc[c][oz][ox], 2.f * u[c][oz + P][ox + P] - a.pnm1[n]);
-                if (n == a.src_node) {
-                    if (a.trace) *a.trace = val;
-                    if (a.inj) val = add_nc(val, mul_nc(a.inj_coef, a.rhs_next));
+        __syncthreads();
+        // divergence: x and z parts from registers, y part from the fluxes
+        // of the element's other planes
+        if (act) {
+#pragma unroll
+            for (int pb = 0; pb < q; ++pb)
+#pragma unroll
+                for (int pa = 0; pa < q; ++pa) {
+                    float v = 0.f;
+#pragma unroll
+                    for (int m = 0; m < q; ++m) {
+                        v = fmaf(a.D[pa * q + m], F0[pb][m], v);
+                        v = fmaf(a.D[pb * q + m], F1[m][pa], v);
+                        v = fmaf(drow[m], fv[le][m][pb * q + pa], v);
+                    }
+                    U[pb][pa] = v;
                 }
-                a.pnm1[n] = val;
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int pb = 0; pb < q; ++pb)
+#pragma unroll
+                for (int pa = 0; pa < q; ++pa) fv[le][pc][pb * q + pa] = U[pb][pa];
+        }
+        __syncthreads();
+        // assembly of the owned nodes (interior ones: every element holding
+        // one is inside the mesh), leapfrog, one store per node
+#pragma unroll
+        for (int c = 0; c < NCOL; ++c) {
+            const int col = tid + c * NT;
+            if (col >= OW * OW) continue;
+            const int oz = col / OW, ox = col - oz * OW;
+            const int gx = ex0 * P + ox, gz = ez0 * P + oz;
+            if (gx < 1 || gx >= a.nx - 1 || gz < 1 || gz >= a.nz - 1) continue;
+            const int X = ox + P, Z = oz + P;
+            const int lxa = X / P, pxa = X - lxa * P, lza = Z / P, pza = Z - lza * P;
+            // contributing (element, local index) pairs along x and z: the
+            // element below first when the node is on a face
+            const int nxe = pxa == 0 ? 2 : 1, nze = pza == 0 ? 2 : 1;
+            const int ex_[2] = {pxa == 0 ? lxa - 1 : lxa, lxa}, ax_[2] = {pxa == 0 ? P : pxa, 0};
+            const int ez_[2] = {pza == 0 ? lza - 1 : lza, lza}, az_[2] = {pza == 0 ? P : pza, 0};
+#pragma unroll
+            for (int cc = 0; cc <= P; ++cc) {
+                float g = cc == 0 ? carry[c] : 0.f;
+                for (int iz = 0; iz < nze; ++iz)
+                    for (int ix = 0; ix < nxe; ++ix)
+                        g += fv[ez_[iz] * (TE + 1) + ex_[ix]][cc][az_[iz] * q + ax_[ix]];
+                if (cc == P) {
+                    carry[c] = g;
+                } else if (ey >= ey0) {
+                    const int gy = ey * P + cc;
+                    if (gy >= 1 && gy < a.ny - 1) {
+                        const int64_t n = (int64_t)gy * plane + (int64_t)gz * a.nx + gx;
+                        float val = fmaf(-a.dt2m[n], g, 2.f * u[slot[cc]][Z][X] - a.pnm1[n]);
+                        if (n == a.src_node) {
+                            if (a.trace) *a.trace = val;
+                            if (a.inj) val = add_nc(val, mul_nc(a.inj_coef, a.rhs_next));
+                        }
+                        a.pnm1[n] = val;
+                    }
+                }
             }
         }
         __syncthreads();
-        // carry the top planes (the next layer's bottom) and clear the rest
-        for (int t = tid; t < OW * OW; t += NT) {
-            const int oz = t / OW, ox = t - oz * OW;
-            acc[0][oz][ox] = acc[P][oz][ox];
-#pragma unroll
-            for (int c = 1; c <= P; ++c) acc[c][oz][ox] = 0.f;
-        }
-        for (int t = tid; t < PW * PW; t += NT) (&u[0][0][0])[t] = (&u[P][0][0])[t];
-        __syncthreads();
+        base = slot[P];  // the top plane is the next layer's bottom one
     }
 }
 
@@ -620,16 +678,14 @@ static SemSepFn sem_pick_sep(int p, int ty, int nj, bool idx32)
 }
 
 using SemMarchFn = void (*)(SemMarchArgs);
-constexpr int SEM_MARCH_TE = 4;    // elements per tile side
-constexpr int SEM_MARCH_NT = 256;  // threads per block
 static SemMarchFn sem_pick_march(int p)
 {
     switch (p) {
-    case 1: return sem_march_kernel<1, SEM_MARCH_TE, SEM_MARCH_NT>;
-    case 2: return sem_march_kernel<2, SEM_MARCH_TE, SEM_MARCH_NT>;
-    case 3: return sem_march_kernel<3, SEM_MARCH_TE, SEM_MARCH_NT>;
-    case 4: return sem_march_kernel<4, SEM_MARCH_TE, SEM_MARCH_NT>;
-    case 5: return sem_march_kernel<5, SEM_MARCH_TE, SEM_MARCH_NT>;
+    case 1: return sem_march_kernel<1, SEM_MARCH_NT>;
+    case 2: return sem_march_kernel<2, SEM_MARCH_NT>;
+    case 3: return sem_march_kernel<3, SEM_MARCH_NT>;
+    case 4: return sem_march_kernel<4, SEM_MARCH_NT>;
+    case 5: return sem_march_kernel<5, SEM_MARCH_NT>;
     default: return nullptr;
     }
 }
@@ -736,44 +792,11 @@ static phc_status sem_get(phc_sem *h, const DevBuf &src, float *host)
     return PHC_OK;
 }
 
-// Per-XCD schedule of the separable kernel (chunk_i = -m, m = 1..4).  The
-// hardware deals block b to XCD b % 8, and each XCD starts its blocks in order
-// as its CUs free up, one 16-wave block per CU at a time.  The march axis is
-// cut into R = 8 >> (m - 1) plane ranges (between 4-plane groups) and the tile
-// columns into 8 / R bands; XCD x takes range x % R of band x / R, so the
-// tiles of one L2 march the same planes (longer ranges: less ring priming
-// per march).  The columns that would leave a last round part-empty are cut
-// into shorter pieces that fill one round instead.  The lists are padded to
-// one length with empty ranges; table entry b belongs to hardware block b.
-// Coverage does not depend on where a block runs.
+// Per-XCD schedule of the separable kernel (chunk_i = -m, m = 1..4): one
+// 16-wave block per CU at a time, plane ranges between 4-plane groups.
 static void sem_build_sched(int nx, int ncol, int cus, int m, std::vector<int4> &tab)
 {
-    const int R = 8 >> (m - 1), B = 8 / R;
-    const int cpx = std::max(cus / 8, 1);  // CUs per XCD
-    std::vector<std::vector<int4>> lists(8);
-    size_t longest = 0;
-    const int G = (nx + 3) / 4;  // 4-plane groups of the march axis, dealt evenly over the ranges
-    for (int x = 0; x < 8; ++x) {
-        const int r = x % R, band = x / R;
-        const int a = 4 * (G * r / R), b = std::min(4 * (G * (r + 1) / R), nx);
-        const int c0 = ncol * band / B, c1 = ncol * (band + 1) / B;
-        auto &li = lists[x];
-        if (a < b && c0 < c1) {
-            const int rem = (c1 - c0) % cpx, full = c1 - rem;
-            for (int w = c0; w < full; ++w) li.push_back(make_int4(w, a, b, 0));
-            const int g = (b - a + 3) / 4;  // 4-plane groups of the range
-            const int pieces = rem ? std::max(1, std::min(cpx / rem, g)) : 0;
-            for (int w = full; w < c1; ++w)
-                for (int q = 0; q < pieces; ++q) {
-                    const int g0 = g * q / pieces, g1 = g * (q + 1) / pieces;
-                    li.push_back(make_int4(w, a + 4 * g0, std::min(a + 4 * g1, b), 0));
-                }
-        }
-        longest = std::max(longest, li.size());
-    }
-    tab.assign(8 * longest, make_int4(0, 0, 0, 0));
-    for (int x = 0; x < 8; ++x)
-        for (size_t l = 0; l < lists[x].size(); ++l) tab[8 * l + x] = lists[x][l];
+    build_xcd_schedule(nx, 0, ncol, std::max(cus / 8, 1), m, 4, tab);
 }
 
 // x tiles of the separable kernel (over x in [0, nz - 1), interior from 1),
@@ -863,11 +886,20 @@ static phc_status sem_launch(phc_sem *h, int i1, int i2, int inj, float rhs_next
         std::memcpy(a.D, h->Dm, sizeof(a.D));
         a.ex = h->ex; a.ey = h->ey; a.ez = h->ez;
         a.nx = h->nx; a.ny = h->ny; a.nz = h->nz;
-        a.tiles_x = (h->ex + SEM_MARCH_TE - 1) / SEM_MARCH_TE;
-        a.tiles_z = (h->ez + SEM_MARCH_TE - 1) / SEM_MARCH_TE;
-        // layer ranges along y so that there are about two blocks per CU
+        const int TE = sem_march_te(h->p);
+        a.tiles_x = (h->ex + TE - 1) / TE;
+        a.tiles_z = (h->ez + TE - 1) / TE;
+        // layer ranges along y: the count that minimises the most layers any
+        // CU marches (blocks per CU x (layers per range + the range's
+        // priming layer))
         const int cols = a.tiles_x * a.tiles_z;
-        const int ranges = std::max(1, std::min(h->ey, (2 * h->cus + cols - 1) / cols));
+        int ranges = 1;
+        long best = -1;
+        for (int r = 1; r <= std::min(h->ey, 64); ++r) {
+            const int chunk = (h->ey + r - 1) / r, nr = (h->ey + chunk - 1) / chunk;
+            const long cost = (long)((cols * nr + h->cus - 1) / h->cus) * (chunk + (nr > 1 ? 1 : 0));
+            if (best < 0 || cost < best) { best = cost; ranges = nr; }
+        }
         a.ychunk = (h->ey + ranges - 1) / ranges;
         const int nb = cols * ((h->ey + a.ychunk - 1) / a.ychunk);
         a.src_node = h->src_node;
