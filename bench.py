@@ -61,6 +61,23 @@ def progress(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+# Oracle against the reference on one host: oracle/time_oracle.py 8 (this
+# build's oracle) beside the reference's own drivers timed on the same 8
+# cores (its published FD case 150^3: 69.7 Mpoint-updates/s; its SEM case
+# "50 2000": 8.17 Mnode-updates/s).  A CPU baseline here divided by the ratio
+# estimates the reference's rate on the same cores.
+ORACLE_OVER_REFERENCE = {
+    "fd": {"oracle": 234.1, "reference": 69.7, "ratio": 3.36, "unit": "Mpoint-updates/s",
+           "case": "150^3 radius 4, 20-point sponge, 8 threads, same host"},
+    "sem": {"oracle": 7.85, "reference": 8.17, "ratio": 0.96, "unit": "Mnode-updates/s",
+            "case": "50^3 elements of order 2, 8 threads, same host"},
+}
+
+
+def host_cores():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
 def cpu_baseline(setup, nsteps=3):
     """The oracle (a restatement of the reference's FD loop: two passes per
     step, FP64 combine) on this host's cores, on a bounded sample."""
@@ -72,28 +89,33 @@ def cpu_baseline(setup, nsteps=3):
     t0 = time.perf_counter()
     P.run(nsteps)
     dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = host_cores()
     pts = setup.nx * setup.ny * setup.nz
     return {"value": pts * nsteps / dt / 1e6, "unit": "Mpoint-updates/s", "cores": cores,
             "kind": "port",
-            "sample": "%d steps of the same 512^3 layered workload (oracle/fd_oracle.c, OpenMP)" % nsteps}
+            "sample": "%d steps of the same 512^3 layered workload (oracle/fd_oracle.c, OpenMP)" % nsteps,
+            "oracle_over_reference": ORACLE_OVER_REFERENCE["fd"]}
 
 
-def cpu_baseline_sem_p4(ex=20, steps=2):
+def cpu_baseline_sem_p4(ex=24, steps=3):
     """The SEM oracle (the reference's element loop restated: computeB V2 +
-    gradPhiGradPhi per element, sequential scatter) on one host core, on a
-    bounded sample of the order-4 workload."""
+    gradPhiGradPhi per element) on every host core (OpenMP over the elements
+    of one colour at a time, so the scatter does not race), on a bounded
+    sample of the order-4 workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
+    cores = host_cores()
     P = oracle.SemProblem(ex=ex, lx=20.0 * ex, order=4, dt=2e-4, f0=8.0, nsamples=steps + 1,
                           src_point=(10.0 * ex + 2.5, 10.0 * ex + 2.5, 10.0 * ex + 2.5))
+    P.set_threads(cores)
     P.run(1)
     t0 = time.perf_counter()
     P.run(steps)
     dt = time.perf_counter() - t0
-    return {"value": round(P.nN * steps / dt / 1e6, 2), "unit": "Mnode-updates/s", "cores": 1, "kind": "port",
-            "sample": "%d steps of a %d^3-element order-4 mesh (%d^3 nodes), oracle/sem_oracle.c"
-                      % (steps, ex, 4 * ex + 1)}
+    return {"value": round(P.nN * steps / dt / 1e6, 2), "unit": "Mnode-updates/s", "cores": cores, "kind": "port",
+            "sample": "%d steps of a %d^3-element order-4 mesh (%d^3 nodes), oracle/sem_oracle.c, OpenMP"
+                      % (steps, ex, 4 * ex + 1),
+            "oracle_over_reference": ORACLE_OVER_REFERENCE["sem"]}
 
 
 def cpu_baseline_sem2d(steps=200):
@@ -106,7 +128,8 @@ def cpu_baseline_sem2d(steps=200):
     O.run(steps)
     dt = time.perf_counter() - t0
     return {"value": round(O.nN * steps / dt / 1e6, 2), "unit": "Mnode-updates/s", "cores": 1, "kind": "port",
-            "sample": "%d steps of the published 65 x 65 order-3 mesh, oracle/sem2d_oracle.c" % steps}
+            "sample": "%d steps of the published 65 x 65 order-3 mesh, oracle/sem2d_oracle.c (one thread: the "
+                      "2D oracle has no threaded loop)" % steps}
 
 
 def bench_batch(phc, nsteps=800, nshots=64, baselines=True, shape=None):
@@ -270,11 +293,13 @@ def bench_sem_elem(phc, steps=100, steps_p4=20):
     """The reference's CPU-runnable SEM case (50^3 elements of order 2, 101^3
     nodes, c = 1500, f0 = 5 Hz, dt = 1e-3) through the SEM step kernels:
     separable, element with atomic scatter, element with the 8-colour
-    scatter, element write-once march (leapfrog fused); then the same on
+    scatter, element write-once march (leapfrog fused), the 8-colour element
+    kernel with its contractions on MFMA; then the same on
     100^3 elements of order 4 (401^3 nodes, dt = 2e-4, f0 = 8 Hz), with the
     roofline fraction at 16 B per node update."""
     out = {"workload": "SEM 50^3 elements of order 2 (101^3 nodes), %d steps, FP32" % steps}
-    kernels = (("separable", 0), ("element_atomic", 1), ("element_colour", 2), ("element_march", 3))
+    kernels = (("separable", 0), ("element_atomic", 1), ("element_colour", 2), ("element_march", 3),
+               ("element_colour_mfma", 4))
     for name, kern in kernels:
         s = phc.SemSolver(ex=50, lx=2000.0, order=2, dt=1e-3, f0=5.0, nsamples=steps + 1, kernel=kern)
         if kern == 0:

@@ -198,10 +198,17 @@ typedef struct phc_sem phc_sem;
  * only the first element layer in 3D. */
 #define PHC_SEM_ELEMENT_COLOR 2
 /* element kernel that writes every node once: a block owns the nodes of a
- * tile of elements and marches the element layers, adds K_e u_e of the
- * tile's elements (4 colours, no shared node within one) into LDS and applies
- * the leapfrog update (SEMsolver.cpp:46-89 in one pass, no y array) */
+ * tile of elements and marches the element layers; K_e u_e of the layer's
+ * elements is formed in LDS (one lane per element node plane), every owned
+ * node sums the contributions of the elements holding it in a fixed order,
+ * and the leapfrog update follows (SEMsolver.cpp:46-89 in one pass, no y
+ * array) */
 #define PHC_SEM_ELEMENT_MARCH 3
+/* the colour-scatter element kernel with its contractions (gradient and
+ * divergence, SEMQkGLIntegrals.hpp:363-520) on the matrix cores: one element
+ * per wavefront, each direction's contraction as FP32 16x16x4 MFMA products
+ * (lines of the element x the padded 1D derivative matrix).  Orders 1..5. */
+#define PHC_SEM_ELEMENT_MFMA 4
 
 typedef struct {
     int ex, ey, ez;        /* elements per axis (SEMproxy.cpp:12-14) */

@@ -495,14 +495,27 @@ __global__ void lb_to_bf16_kernel(const float *theta, uint16_t *w, int n)
 typedef short lb_bf16x4 __attribute__((ext_vector_type(4)));
 typedef float lb_f32x16 __attribute__((ext_vector_type(16)));
 
+typedef __bf16 lb_bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float lb_f32x2_t __attribute__((ext_vector_type(2)));
+
+// Two floats to packed bf16, round to nearest even (one v_cvt_pk_bf16_f32;
+// the same values as f2bf for finite inputs), element 0 in the low half.
+__device__ inline uint32_t f2bf2(float a, float b)
+{
+    const lb_f32x2_t v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, lb_bf16x2_t));
+}
+
+// a = hi + lo with hi = bf16(a) and lo = bf16(a - hi): ~16 bits of a.
 __device__ inline void lb_split(const float *a, lb_bf16x4 &hi, lb_bf16x4 &lo)
 {
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const uint16_t h = f2bf(a[e]);
-        hi[e] = (short)h;
-        lo[e] = (short)f2bf(a[e] - bf2f(h));
-    }
+    const uint32_t h01 = f2bf2(a[0], a[1]), h23 = f2bf2(a[2], a[3]);
+    const uint32_t l01 = f2bf2(a[0] - __uint_as_float(h01 << 16), a[1] - __uint_as_float(h01 & 0xffff0000u));
+    const uint32_t l23 = f2bf2(a[2] - __uint_as_float(h23 << 16), a[3] - __uint_as_float(h23 & 0xffff0000u));
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 hv = {h01, h23}, lv = {l01, l23};
+    hi = __builtin_bit_cast(lb_bf16x4, hv);
+    lo = __builtin_bit_cast(lb_bf16x4, lv);
 }
 
 // One face's weights in the registers of a wavefront, in MFMA operand
@@ -594,85 +607,104 @@ __device__ inline void lb_lane_inputs(const float x[LB_NIN], float xk[12])
         }
 }
 
-constexpr int LB_GPW = 4;  // groups of 32 layer points per wavefront
+// Correction pass, staged: a block takes a tile of one face's layer, the
+// LB_W depths x LB_TU rows (u) x LB_TV columns (v), and first copies the
+// 2 LB_R + LB_W planes along the normal that the tile's normal stencils read
+// (halo included) of both levels into LDS, with loads along the grid's
+// contiguous axis.  Wavefront w then evaluates depth w: LB_TU groups of 32
+// consecutive v, each lane's 18 field inputs read from LDS.  The inputs,
+// network and stores are those of lb_correct_kernel, so the corrections are
+// the same floats.
+constexpr int LB_TU = 8, LB_TV = 32;
+constexpr int LB_NPL = 2 * LB_R + LB_W;  // normal planes of a tile (12)
+static_assert(LB_TV == 32 && LB_W == 4, "a group is 32 consecutive v; one wavefront per depth");
 
-// Groups of 32 lanes of the correction pass on one face: 32 consecutive
-// compact indices on the i- and j-faces; on the k-faces 8 consecutive j of
-// one i times the LB_W depths.
-static_assert(LB_W == 4, "k-face groups are 8 rows of LB_W = 4 depths");
-__host__ __device__ inline int64_t lb_correct_groups(const LbFaceGeom &g, int64_t cnt)
-{
-    return g.axis == 2 ? (int64_t)g.nu * ((g.nv + 7) / 8) : (cnt + 31) / 32;
-}
-
-struct LbCorrectArgs {
-    LbGridView G;
+struct LbCorrectTileArgs {
+    const float *pn, *pnm1, *vp;
+    FdLayout L;
     const uint16_t *w;
     float *corr;
-    int64_t off[LB_FACES + 1];  // compact offsets
-    int64_t blk0[LB_FACES + 1]; // first block of each face
+    int64_t off[LB_FACES + 1];
+    int blk0[LB_FACES + 1];  // first block of each face
     float h, c1;
 };
 
-// Correction pass: corr[face][d][u][v] = vp2 * c1 * s * y for every layer
-// point.  A block works on one face: its biases b1 go to LDS, each wavefront
-// keeps the face's other weights in registers and evaluates LB_GPW groups of
-// 32 points (lb_correct_groups).
-__global__ void __launch_bounds__(256) lb_correct_kernel(LbCorrectArgs A)
+__global__ void __launch_bounds__(256) lb_correct_tile_kernel(LbCorrectTileArgs A)
 {
     __shared__ float b1s[LB_W * LB_H];
+    __shared__ float S[2][LB_NPL][LB_TU][LB_TV];
+    __shared__ float Svp[LB_W][LB_TU][LB_TV];  // vp at the tile's layer points [depth][u][v]
     int f = 0;
-    while (f + 1 < LB_FACES && (int64_t)blockIdx.x >= A.blk0[f + 1]) ++f;
+    while (f + 1 < LB_FACES && (int)blockIdx.x >= A.blk0[f + 1]) ++f;
+    const int n[3] = {A.L.nx, A.L.ny, A.L.nz};
+    const LbFaceGeom g = lb_face(n, f);
+    const int a = g.axis, ua = (a == 0) ? 1 : 0, va = (a == 2) ? 1 : 2;
+    const int tb = (int)blockIdx.x - A.blk0[f];
+    const int ntv = (g.nv + LB_TV - 1) / LB_TV;
+    const int u0 = (tb / ntv) * LB_TU, v0 = (tb % ntv) * LB_TV;
+    // normal index of plane 0 of the box: -R at the low face, n - W - R at
+    // the high one (depths n-1-d, offsets -R..R)
+    const int nrm0 = g.side ? n[a] - LB_W - LB_R : -LB_R;
     const uint16_t *w = A.w + f * LB_NP;
     for (int t = threadIdx.x; t < LB_W * LB_H; t += blockDim.x) b1s[t] = bf2f(w[LB_OFF_B1 + t]);
+    // stage the box: for the i- and j-faces v (= k) runs fastest, for the
+    // k-faces the normal (= k) does
+    constexpr int NBOX = LB_NPL * LB_TU * LB_TV;
+    for (int t = threadIdx.x; t < NBOX; t += blockDim.x) {
+        int pl, uu, vv;
+        if (a == 2) {
+            pl = t % LB_NPL;
+            const int r = t / LB_NPL;
+            vv = r % LB_TV;
+            uu = r / LB_TV;
+        } else {
+            vv = t % LB_TV;
+            const int r = t / LB_TV;
+            uu = r % LB_TU;
+            pl = r / LB_TU;
+        }
+        const int u = min(u0 + uu, g.nu - 1), v = min(v0 + vv, g.nv - 1);
+        int c[3];
+        c[a] = nrm0 + pl;
+        c[ua] = u;
+        c[va] = v;
+        const int64_t o = fd_off(A.L, c[0], c[1], c[2]);
+        S[0][pl][uu][vv] = A.pn[o];
+        S[1][pl][uu][vv] = A.pnm1[o];
+    }
+    for (int t = threadIdx.x; t < LB_W * LB_TU * LB_TV; t += blockDim.x) {
+        const int vv = t % LB_TV, uu = (t / LB_TV) % LB_TU, dd = t / (LB_TV * LB_TU);
+        int c[3];
+        c[a] = g.side ? n[a] - 1 - dd : dd;
+        c[ua] = min(u0 + uu, g.nu - 1);
+        c[va] = min(v0 + vv, g.nv - 1);
+        Svp[dd][uu][vv] = A.vp[fd_off(A.L, c[0], c[1], c[2])];
+    }
     __syncthreads();
     LbWaveWeights W;
     lb_load_wave_weights(w, W);
-    const LbFaceGeom g = lb_face(A.G.n, f);
-    const int64_t cnt = A.off[f + 1] - A.off[f];
-    const int64_t ngroups = lb_correct_groups(g, cnt);
-    const int64_t g0 = ((int64_t)(blockIdx.x - A.blk0[f]) * 4 + (threadIdx.x >> 6)) * LB_GPW;
-    const int p = threadIdx.x & 31;  // this lane's point of the group (both half-waves)
-    const int nvb = (g.nv + 7) >> 3;
-    for (int gi = 0; gi < LB_GPW; ++gi) {
-        const int64_t gidx = g0 + gi;
-        if (gidx >= ngroups) break;  // uniform per wavefront
-        int d, u, v;
-        bool ok;
-        if (g.axis == 2) {
-            // k-faces: 8 consecutive j (v) x the 4 depths, so the lanes of a
-            // row read its normal stencils from one or two 64-byte segments
-            u = (int)(gidx / nvb);
-            v = (int)(gidx - (int64_t)u * nvb) * 8 + (p >> 2);
-            d = p & 3;
-            ok = v < g.nv;
-            if (!ok) v = g.nv - 1;
-        } else {
-            // i- and j-faces: 32 consecutive compact indices, v (= k) fastest;
-            // the group's first point, then this lane's offset with carries
-            // (nv >= 16, so at most two)
-            const int64_t base = gidx * 32;
-            const int64_t idx = base + p;
-            ok = idx < cnt;
-            const int ci = (int)(ok ? idx : cnt - 1);  // layers hold < 2^31 points
-            const int b0 = (int)base;
-            v = b0 % g.nv;
-            u = (b0 / g.nv) % g.nu;
-            d = b0 / (g.nv * g.nu);
-            v += ci - b0;
-            while (v >= g.nv) {
-                v -= g.nv;
-                if (++u == g.nu) { u = 0; ++d; }
-            }
-        }
+    const int d = threadIdx.x >> 6;  // this wavefront's depth
+    const int p = threadIdx.x & 31;  // this lane's v in the group (both half-waves)
+    const int v = v0 + p;
+    const bool vok = v < g.nv;
+#pragma unroll 1
+    for (int uu = 0; uu < LB_TU; ++uu) {
+        const int u = u0 + uu;
+        if (u >= g.nu) break;  // uniform per wavefront
         float x[LB_NIN], xk[12];
-        int64_t center;
-        lb_load_raw(A.G, g, d, u, v, x, &center);
+#pragma unroll
+        for (int m = 0; m < 9; ++m) {
+            // offset m - 4 along the inward normal
+            const int pl = g.side ? LB_R + LB_W - 1 - d + LB_R - m : d + m;
+            x[m] = S[0][pl][uu][p];
+            x[9 + m] = S[1][pl][uu][p];
+        }
+        x[18] = Svp[d][uu][p];
         const float vraw = x[18];
         const float s = lb_finish_inputs(d, A.h, A.c1, x);
         lb_lane_inputs(x, xk);
         const float y = lb_mlp_wave(W, b1s, xk, d);
-        if ((threadIdx.x & 32) == 0 && ok)
+        if ((threadIdx.x & 32) == 0 && vok)
             A.corr[A.off[f] + ((int64_t)d * g.nu + u) * g.nv + v] = vraw * A.c1 * s * y;
     }
 }
@@ -698,46 +730,42 @@ __global__ void __launch_bounds__(256) lb_infer_kernel(const uint16_t *w, const 
 }
 
 // Apply pass: one thread per point of the union of the layers; adds the
-// corrections of every face whose layer holds the point.  The union is walked
-// as rows (blockIdx.y, decoded once per block): 2*LB_W*ny rows along k for
-// the i-layers, 2*LB_W*(nx-2*LB_W) rows along k for the j-layers (i
-// interior); a second launch covers the k-layers (i, j interior) with one row
-// per i and 2*LB_W lanes per j, so lanes run along k through both layers.
+// corrections of every face whose layer holds the point.  One launch walks
+// the union in three parts, each with its fastest index along k: the
+// i-layers (2 LB_W planes of ny x nz points), the j-layers with i interior
+// (rows along k), and the k-layers with i and j interior (the 2 LB_W points
+// of each (i, j) column, both ends).
 struct LbApplyArgs {
     float *p;
     FdLayout L;
     const float *corr;
     int64_t off[LB_FACES + 1];
-    int rowsA, rowsB;
-    int kface;          // 1: the k-layers, i and j interior (the second launch)
+    int nA, nB, nC;     // points of the three parts
     float ldamp[LB_W];  // damping factor of the new value per layer depth (1: none)
 };
 
 __global__ void __launch_bounds__(256) lb_apply_kernel(LbApplyArgs A)
 {
     const int nx = A.L.nx, ny = A.L.ny, nz = A.L.nz;
-    const int row = blockIdx.y;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
     int i, j, k;
-    if (A.kface) {  // row = interior i; 2 LB_W lanes per j run along k through both k-layers
-        const int q = t % (2 * LB_W);
-        i = LB_W + row;
-        j = LB_W + t / (2 * LB_W);
-        k = q < LB_W ? q : nz - 2 * LB_W + q;
-        if (j >= ny - LB_W) return;
-    } else if (row < A.rowsA) {  // i in layers, all j, k along the row
-        const int q = row / ny;
-        j = row - q * ny;
+    if (t < A.nA) {
+        k = t % nz;
+        const int r = t / nz, q = r / ny;
+        j = r - q * ny;
         i = q < LB_W ? q : nx - 2 * LB_W + q;
-        k = t;
-        if (k >= nz) return;
-    } else {  // j in layers, i interior, k along the row
-        const int r = row - A.rowsA;
-        const int q = r % (2 * LB_W);
+    } else if ((t -= A.nA) < A.nB) {
+        k = t % nz;
+        const int r = t / nz, q = r % (2 * LB_W);
         i = LB_W + r / (2 * LB_W);
         j = q < LB_W ? q : ny - 2 * LB_W + q;
-        k = t;
-        if (k >= nz) return;
+    } else if ((t -= A.nB) < A.nC) {
+        const int q = t % (2 * LB_W), r = t / (2 * LB_W), nyi = ny - 2 * LB_W;
+        j = LB_W + r % nyi;
+        i = LB_W + r / nyi;
+        k = q < LB_W ? q : nz - 2 * LB_W + q;
+    } else {
+        return;
     }
     const int n[3] = {nx, ny, nz};
     const int c[3] = {i, j, k};
@@ -762,27 +790,26 @@ __global__ void __launch_bounds__(256) lb_apply_kernel(LbApplyArgs A)
 hipError_t lb_enqueue_correct(const LbInfer &lb, const FdLayout &L, const float *pn, const float *pnm1,
                               const float *vp, hipStream_t stream)
 {
-    LbCorrectArgs A;
-    A.G.pn = pn;
-    A.G.pnm1 = pnm1;
-    A.G.vp = vp;
-    A.G.L = L;
-    A.G.o[0] = A.G.o[1] = A.G.o[2] = 0;
-    A.G.n[0] = L.nx;
-    A.G.n[1] = L.ny;
-    A.G.n[2] = L.nz;
-    A.w = lb.w;
-    A.corr = lb.corr;
-    A.h = lb.halo;
-    A.c1 = lb.c1;
-    A.blk0[0] = 0;
-    for (int f = 0; f <= LB_FACES; ++f) A.off[f] = lb.corr_off[f];
-    const int64_t per_block = 4 * LB_GPW;  // groups per block
+    // grids under the learned layer have at least 4 LB_W points per side
+    // (phc_fd_set_learned_boundary), so every tile's box of normal planes
+    // lies inside the grid and its halo
+    LbCorrectTileArgs T;
+    T.pn = pn;
+    T.pnm1 = pnm1;
+    T.vp = vp;
+    T.L = L;
+    T.w = lb.w;
+    T.corr = lb.corr;
+    T.h = lb.halo;
+    T.c1 = lb.c1;
+    T.blk0[0] = 0;
+    for (int f = 0; f <= LB_FACES; ++f) T.off[f] = lb.corr_off[f];
     const int nn[3] = {L.nx, L.ny, L.nz};
-    for (int f = 0; f < LB_FACES; ++f)
-        A.blk0[f + 1] = A.blk0[f] + (lb_correct_groups(lb_face(nn, f), A.off[f + 1] - A.off[f]) + per_block - 1) /
-                                        per_block;
-    hipLaunchKernelGGL(lb_correct_kernel, dim3((unsigned)A.blk0[LB_FACES]), dim3(256), 0, stream, A);
+    for (int f = 0; f < LB_FACES; ++f) {
+        const LbFaceGeom g = lb_face(nn, f);
+        T.blk0[f + 1] = T.blk0[f] + ((g.nu + LB_TU - 1) / LB_TU) * ((g.nv + LB_TV - 1) / LB_TV);
+    }
+    hipLaunchKernelGGL(lb_correct_tile_kernel, dim3((unsigned)T.blk0[LB_FACES]), dim3(256), 0, stream, T);
     return hipGetLastError();
 }
 
@@ -794,19 +821,14 @@ hipError_t lb_enqueue_apply(const LbInfer &lb, const FdLayout &L, float *pnew, h
     A.corr = lb.corr;
     for (int f = 0; f <= LB_FACES; ++f) A.off[f] = lb.corr_off[f];
     for (int d = 0; d < LB_W; ++d) A.ldamp[d] = lb.ldamp[d];
-    A.rowsA = 2 * LB_W * L.ny;
-    A.rowsB = 2 * LB_W * (L.nx - 2 * LB_W);
-    A.kface = 0;
-    hipLaunchKernelGGL(lb_apply_kernel, dim3((L.nz + 255) / 256, A.rowsA + A.rowsB), dim3(256), 0, stream, A);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const int klen = 2 * LB_W * (L.ny - 2 * LB_W);  // lanes per interior i
-    if (L.nx > 2 * LB_W && klen > 0) {
-        A.kface = 1;
-        hipLaunchKernelGGL(lb_apply_kernel, dim3((klen + 255) / 256, L.nx - 2 * LB_W), dim3(256), 0, stream, A);
-        e = hipGetLastError();
-    }
-    return e;
+    // (every side is at least 4 LB_W points, phc_fd_set_learned_boundary)
+    A.nA = 2 * LB_W * L.ny * L.nz;
+    A.nB = (L.nx - 2 * LB_W) * 2 * LB_W * L.nz;
+    A.nC = (L.nx - 2 * LB_W) * (L.ny - 2 * LB_W) * 2 * LB_W;
+    const int64_t tot = (int64_t)A.nA + A.nB + A.nC;
+    if (tot == 0) return hipSuccess;
+    hipLaunchKernelGGL(lb_apply_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream, A);
+    return hipGetLastError();
 }
 
 } // namespace phc

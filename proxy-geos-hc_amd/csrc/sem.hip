@@ -293,6 +293,7 @@ struct SemElemArgs {
                          // one element (stride 0) or every element
     int64_t geo_stride;  // floats between elements in geo
     float D[(SEM_MAX_P + 1) * (SEM_MAX_P + 1)];  // D[a][b] = l_a'(xi_b)
+    float dgd[SEM_MAX_P + 1];  // MFMA kernel: gradient diagonal, -sum_{a != b} D[a][b]
     int ex, ey, ez, nx, ny, nz;
     int nelem;           // elements this launch covers
     // colour < 0: every element, atomic scatter.  colour c in 0..7: only the
@@ -362,6 +363,126 @@ sem_elem_kernel(SemElemArgs a)
             acc = fmaf(Ds[pb * q + m], g[le][1][pa + m * q + pc * q * q], acc);
             acc = fmaf(Ds[pc * q + m], g[le][2][pa + pb * q + m * q * q], acc);
         }
+        if (a.colour < 0) atomicAdd(&a.y[node], acc);
+        else a.y[node] += acc;
+    }
+}
+
+// The element operator with its contractions on the matrix cores (kernel
+// PHC_SEM_ELEMENT_MFMA): one element per wavefront, four per block.  Each of
+// the six contractions (three gradients, three divergence parts) is a product
+// of the element's lines with the 1D derivative matrix,
+//     out[line][j] = sum_m in[line][m] M[m][j],
+// q^2 lines (rows) x q terms (K, padded to 8) x q outputs (N, padded to 16),
+// taken as ceil(q^2/16) x 2 v_mfma_f32_16x16x4f32 products.  Operand layout
+// of that instruction (one block): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] of a 16x4 by 4x16 step and receives D[4 (l/16) + v][l%16],
+// v = 0..3.  The gradient uses D with each column's diagonal entry dgd[b] =
+// -sum_{a != b} D[a q + b] (so a constant has zero gradient, the exact
+// property the vector kernel keeps with differences); the divergence uses D
+// transposed.  Lines of direction x are (b, c) with the term index running
+// over a, of z (a, c) over b, of y (a, b) over c; geometric factors are
+// applied per point between the two products, and the three divergence parts
+// are added per point before the colour scatter.
+typedef float sem_f32x4 __attribute__((ext_vector_type(4)));
+
+template <int P>
+__device__ __forceinline__ int sem_line_point(int d, int line, int m)
+{
+    constexpr int q = P + 1;
+    const int r0 = line % q, r1 = line / q;
+    if (d == 0) return m + q * r0 + q * q * r1;   // x: line (b, c)
+    if (d == 1) return r0 + q * m + q * q * r1;   // z: line (a, c)
+    return r0 + q * r1 + q * q * m;               // y: line (a, b)
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) sem_elem_mfma_kernel(SemElemArgs a)
+{
+    constexpr int q = P + 1, n2 = q * q, n3 = q * n2;
+    constexpr int RT = (n2 + 15) / 16;  // 16-line row tiles per direction
+    static_assert(q <= 8 && n3 <= 256, "K padded to 8; points of an element within 4 lanes each");
+    __shared__ float u[4][n3];
+    __shared__ float gr[4][3][n3];  // gradients, then fluxes (per point)
+    __shared__ float dv[4][3][n3];  // divergence parts (per point)
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + w;  // element of this launch
+    if (k >= a.nelem) return;          // whole wavefront
+    int exi, ezi, eyi;
+    if (a.colour < 0) {
+        exi = k % a.ex; ezi = (k / a.ex) % a.ez; eyi = k / (a.ex * a.ez);
+    } else {
+        exi = 2 * (k % a.ncx) + (a.colour & 1);
+        ezi = 2 * ((k / a.ncx) % a.ncz) + ((a.colour >> 1) & 1);
+        eyi = 2 * (k / (a.ncx * a.ncz)) + ((a.colour >> 2) & 1);
+    }
+    const int e = exi + ezi * a.ex + eyi * a.ex * a.ez;
+    auto node_of = [&](int pt) {
+        const int pa = pt % q, pb = (pt / q) % q, pc = pt / n2;
+        return (exi * P + pa) + (ezi * P + pb) * a.nx + (eyi * P + pc) * a.nx * a.nz;
+    };
+    for (int pt = lane; pt < n3; pt += 64) u[w][pt] = a.pn[node_of(pt)];
+    // B operands of the two K steps: gradient Dg[kk][j], divergence D[j][kk]
+    const int j = lane & 15, kq = lane >> 4;
+    float bg[2], bd[2];
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+        const int kk = 4 * st + kq;
+        const bool in = kk < q && j < q;
+        bg[st] = in ? (kk == j ? a.dgd[j] : a.D[kk * q + j]) : 0.f;
+        bd[st] = in ? a.D[j * q + kk] : 0.f;
+    }
+    wave_lds_sync();
+    // gradients
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+            sem_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int st = 0; st < 2; ++st) {
+                const int line = 16 * rt + j, kk = 4 * st + kq;
+                const float av = (line < n2 && kk < q) ? u[w][sem_line_point<P>(d, line, kk)] : 0.f;
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bg[st], acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int line = 16 * rt + 4 * kq + v;
+                if (line < n2 && j < q) gr[w][d][sem_line_point<P>(d, line, j)] = acc[v];
+            }
+        }
+    wave_lds_sync();
+    // fluxes, per point: (w w w B) grad u; B Voigt order 11, 22, 33, 12, 13, 23
+    for (int pt = lane; pt < n3; pt += 64) {
+        const float s1 = gr[w][0][pt], s2 = gr[w][1][pt], s3 = gr[w][2][pt];
+        const float *G = a.geo + (int64_t)e * a.geo_stride + 6 * pt;
+        gr[w][0][pt] = G[0] * s1 + G[3] * s2 + G[4] * s3;
+        gr[w][1][pt] = G[3] * s1 + G[1] * s2 + G[5] * s3;
+        gr[w][2][pt] = G[4] * s1 + G[5] * s2 + G[2] * s3;
+    }
+    wave_lds_sync();
+    // divergence parts
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+            sem_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int st = 0; st < 2; ++st) {
+                const int line = 16 * rt + j, kk = 4 * st + kq;
+                const float av = (line < n2 && kk < q) ? gr[w][d][sem_line_point<P>(d, line, kk)] : 0.f;
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bd[st], acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int line = 16 * rt + 4 * kq + v;
+                if (line < n2 && j < q) dv[w][d][sem_line_point<P>(d, line, j)] = acc[v];
+            }
+        }
+    wave_lds_sync();
+    for (int pt = lane; pt < n3; pt += 64) {
+        const float acc = (dv[w][0][pt] + dv[w][1][pt]) + dv[w][2][pt];
+        const int node = node_of(pt);
         if (a.colour < 0) atomicAdd(&a.y[node], acc);
         else a.y[node] += acc;
     }
@@ -660,6 +781,7 @@ static SemSepFn sep_shape(int ty, int nj, bool idx32)
     if (ty == 16 && nj == 1) return sep_idx<P, 16, 1>(idx32);
     if (ty == 16 && nj == 2) return sep_idx<P, 16, 2>(idx32);
     if (ty == 8 && nj == 2) return sep_idx<P, 8, 2>(idx32);
+    if (ty == 8 && nj == 1) return sep_idx<P, 8, 1>(idx32);
     if (ty == 4 && nj == 2) return sep_idx<P, 4, 2>(idx32);
     if (ty == 4 && nj == 1) return sep_idx<P, 4, 1>(idx32);
     return nullptr;
@@ -691,8 +813,19 @@ static SemMarchFn sem_pick_march(int p)
 }
 
 using SemElemFn = void (*)(SemElemArgs);
-static SemElemFn sem_pick_elem(int p, int *epb)
+static SemElemFn sem_pick_elem(int p, bool mfma, int *epb)
 {
+    if (mfma) {
+        *epb = 4;  // one element per wavefront
+        switch (p) {
+        case 1: return sem_elem_mfma_kernel<1>;
+        case 2: return sem_elem_mfma_kernel<2>;
+        case 3: return sem_elem_mfma_kernel<3>;
+        case 4: return sem_elem_mfma_kernel<4>;
+        case 5: return sem_elem_mfma_kernel<5>;
+        default: *epb = 0; return nullptr;
+        }
+    }
     switch (p) {
     case 1: *epb = 32; return sem_elem_kernel<1, 32>;
     case 2: *epb = 9; return sem_elem_kernel<2, 9>;
@@ -792,11 +925,11 @@ static phc_status sem_get(phc_sem *h, const DevBuf &src, float *host)
     return PHC_OK;
 }
 
-// Per-XCD schedule of the separable kernel (chunk_i = -m, m = 1..4): one
-// 16-wave block per CU at a time, plane ranges between 4-plane groups.
-static void sem_build_sched(int nx, int ncol, int cus, int m, std::vector<int4> &tab)
+// Per-XCD schedule of the separable kernel (chunk_i = -m, m = 1..4): slots
+// = blocks resident per XCD, plane ranges between 4-plane groups.
+static void sem_build_sched(int nx, int ncol, int slots, int m, std::vector<int4> &tab)
 {
-    build_xcd_schedule(nx, 0, ncol, std::max(cus / 8, 1), m, 4, tab);
+    build_xcd_schedule(nx, 0, ncol, std::max(slots, 1), m, 4, tab);
 }
 
 // x tiles of the separable kernel (over x in [0, nz - 1), interior from 1),
@@ -812,11 +945,18 @@ static int sem_tiles_k(const phc_sem *h, int *krem)
 static phc_status sem_prepare_sched(phc_sem *h)
 {
     const int TJ = h->ty * h->nj;
-    const int key = (TJ * 8 - h->chunk_i) * 4 + (h->krem_mode + 1);
+    const int key = ((TJ * 8 - h->chunk_i) * 4 + (h->krem_mode + 1)) * 32 + h->ty;
     if (h->kernel != PHC_SEM_SEPARABLE || h->chunk_i >= 0 || h->sched_tj == key) return PHC_OK;
     const int ncol = sem_tiles_k(h, nullptr) * ((std::max(h->L.ny - 2, 1) + TJ - 1) / TJ);
+    // blocks of this shape resident per CU (the table deals one round of
+    // them per XCD at a time)
+    int per_cu = 1;
+    SemSepFn fn = sem_pick_sep(h->p, h->ty, h->nj, !h->index64 && h->L.total * 4 < ((int64_t)1 << 32));
+    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(fn),
+                                                            FD_TK * h->ty, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
     std::vector<int4> tab;
-    sem_build_sched(h->L.nx, ncol, h->cus, -h->chunk_i, tab);
+    sem_build_sched(h->L.nx, ncol, std::max(h->cus / 8, 1) * per_cu, -h->chunk_i, tab);
     PHC_HIP(hipStreamSynchronize(h->stream));  // queued launches may still read the old table
     sem_drop_graph(h);
     PHC_TRY(h->sched.alloc(sizeof(int4) * tab.size()));
@@ -924,7 +1064,16 @@ static phc_status sem_launch(phc_sem *h, int i1, int i2, int inj, float rhs_next
         a.ex = h->ex; a.ey = h->ey; a.ez = h->ez;
         a.nx = h->nx; a.ny = h->ny; a.nz = h->nz;
         int epb = 0;
-        SemElemFn fn = sem_pick_elem(h->p, &epb);
+        SemElemFn fn = sem_pick_elem(h->p, h->kernel == PHC_SEM_ELEMENT_MFMA, &epb);
+        {
+            const int q = h->p + 1;
+            for (int c = 0; c < q; ++c) {
+                double sum = 0.0;
+                for (int m = 0; m < q; ++m)
+                    if (m != c) sum += (double)h->Dm[m * q + c];
+                a.dgd[c] = (float)(-sum);
+            }
+        }
         if (!fn) return fail(PHC_ERR_INVALID, "no element kernel for p=%d", h->p);
         if (e0) PHC_HIP(hipEventRecord(e0, h->stream));
         if (h->kernel == PHC_SEM_ELEMENT) {
@@ -1020,7 +1169,7 @@ phc_status phc_sem_schedule_table(int march_planes, int tile_columns, int cus, i
         return fail(PHC_ERR_INVALID, "planes %d, columns %d, CUs %d, split %d", march_planes, tile_columns, cus,
                     split);
     std::vector<int4> tab;
-    sem_build_sched(march_planes, tile_columns, cus, split, tab);
+    sem_build_sched(march_planes, tile_columns, std::max(cus / 8, 1), split, tab);
     if (entries) {
         if (*n_entries < (int)tab.size())
             return fail(PHC_ERR_INVALID, "table has %d entries, room for %d", (int)tab.size(), *n_entries);
@@ -1047,7 +1196,7 @@ phc_status phc_sem_create(const phc_sem_params *p, phc_sem **out)
     if (!p->model) return fail(PHC_ERR_INVALID, "model is NULL");
     if (p->nsamples < 0 || (p->nsamples > 0 && !p->rhs)) return fail(PHC_ERR_INVALID, "bad rhs");
     if (p->kernel != PHC_SEM_SEPARABLE && p->kernel != PHC_SEM_ELEMENT && p->kernel != PHC_SEM_ELEMENT_COLOR &&
-        p->kernel != PHC_SEM_ELEMENT_MARCH)
+        p->kernel != PHC_SEM_ELEMENT_MARCH && p->kernel != PHC_SEM_ELEMENT_MFMA)
         return fail(PHC_ERR_INVALID, "unknown SEM kernel %d", p->kernel);
     const int64_t nE = (int64_t)p->ex * p->ey * p->ez;
     if (p->src_element < 0 || p->src_element >= nE)
@@ -1535,7 +1684,7 @@ extern "C" phc_status phc_sem_autotune(phc_sem *h, int steps_per_trial, const ch
         phc_status st = PHC_OK;
         if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
             st = fail(PHC_ERR_HIP, "event create failed");
-        const int shapes[][2] = {{16, 16}, {16, 32}, {8, 16}, {4, 8}, {4, 4}};
+        const int shapes[][2] = {{16, 16}, {16, 32}, {8, 16}, {8, 8}, {4, 8}, {4, 4}};
         // chunk lengths that split the march axis evenly (n chunks, multiples
         // of 4 planes): a short last chunk would be mostly ring priming
         std::vector<int> chunks;

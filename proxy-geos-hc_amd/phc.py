@@ -687,6 +687,7 @@ SEM_SEPARABLE = 0
 SEM_ELEMENT = 1
 SEM_ELEMENT_COLOR = 2
 SEM_ELEMENT_MARCH = 3
+SEM_ELEMENT_MFMA = 4
 
 
 class SemParams(ctypes.Structure):

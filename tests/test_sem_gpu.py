@@ -12,7 +12,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = [0, 1, 2, 3]  # SEM_SEPARABLE, SEM_ELEMENT, SEM_ELEMENT_COLOR, SEM_ELEMENT_MARCH
+KERNELS = [0, 1, 2, 3, 4]  # SEM_SEPARABLE, SEM_ELEMENT, SEM_ELEMENT_COLOR, SEM_ELEMENT_MARCH, SEM_ELEMENT_MFMA
 
 
 def banner(msg):
