@@ -26,6 +26,9 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + INCLUDE, "-I" + CSRC]
 
+# roctx ranges around the host API calls (rocprofv3 --marker-trace)
+ROCTX_LIBS = ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx"]
+
 HIP_SOURCES = ["phc_common.hip", "setup.hip", "fd.hip", "sem.hip", "batch.hip", "mlp.hip", "fd64.hip", "sem64.hip", "sem2d.hip"]
 
 
@@ -49,7 +52,7 @@ def build_libphc(jobs=8):
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(compile_one, zip(srcs, objs)))
     tmp = LIBPHC + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ROCTX_LIBS)
     os.replace(tmp, LIBPHC)
     return LIBPHC
 
@@ -116,7 +119,8 @@ def sanitize_host(jobs=8):
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(compile_one, zip(srcs, objs)))
     lib = os.path.join(out, "libphc.so")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Xarch_host", "-fsanitize=address", "-o", lib] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Xarch_host", "-fsanitize=address", "-o", lib] + objs +
+         ROCTX_LIBS)
     exe = os.path.join(out, "host_asan_check")
     _run([HIPCC, "-std=c++17"] + san + ["-I" + INCLUDE, "-I" + HOST,
                                        os.path.join(ROOT, "tests", "cpp", "host_asan_check.cpp"), "-o", exe,

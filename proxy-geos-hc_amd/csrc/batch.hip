@@ -487,6 +487,7 @@ phc_status phc_batch_set_receivers(phc_batch *h, int nrec, const int *ijk)
 
 phc_status phc_batch_run(phc_batch *h, int it0, int nsteps)
 {
+    RoctxRange rr_("phc_batch_run");
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
     if (nsteps < 0) return fail(PHC_ERR_INVALID, "negative step count");
     if (nsteps == 0) return PHC_OK;
@@ -617,6 +618,7 @@ phc_status phc_batch_synchronize(phc_batch *h)
 phc_status phc_batch_autotune(phc_batch *h, int steps_per_trial, int *tile_j, int *block_waves, int *chunk_i,
                               int *blocks_per_cu)
 {
+    RoctxRange rr_("phc_batch_autotune");
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
     if (steps_per_trial < 1 || steps_per_trial > 1000)
         return fail(PHC_ERR_INVALID, "steps_per_trial %d outside 1..1000", steps_per_trial);

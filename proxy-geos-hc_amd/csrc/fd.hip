@@ -620,6 +620,7 @@ phc_status phc_fd_add_rhs_value(phc_fd *h, int cb, int i, int j, int k, float rh
 
 phc_status phc_fd_compute_one_step_pml(phc_fd *h, int ca, int cb)
 {
+    RoctxRange rr_("phc_fd_compute_one_step_pml");
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
     if (!h->pml_on) return fail(PHC_ERR_STATE, "no PML set (phc_fd_set_pml)");
     if (ca < 0 || ca > 1 || cb < 0 || cb > 1 || ca == cb)
@@ -632,6 +633,7 @@ phc_status phc_fd_compute_one_step_pml(phc_fd *h, int ca, int cb)
 
 phc_status phc_fd_compute_one_step_sb(phc_fd *h, int ca, int cb)
 {
+    RoctxRange rr_("phc_fd_compute_one_step_sb");
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
     if (h->pml_on) return fail(PHC_ERR_STATE, "a PML is set: use phc_fd_compute_one_step_pml");
     if (ca < 0 || ca > 1 || cb < 0 || cb > 1 || ca == cb)
@@ -709,6 +711,7 @@ extern "C" {
 
 phc_status phc_fd_run(phc_fd *h, int it0, int nsteps, int *cur)
 {
+    RoctxRange rr_("phc_fd_run");
     if (!h || !cur) return fail(PHC_ERR_INVALID, "handle or cur is NULL");
     if (*cur < 0 || *cur > 1) return fail(PHC_ERR_INVALID, "cur level %d is not 0 or 1", *cur);
     if (nsteps < 0) return fail(PHC_ERR_INVALID, "negative step count");
@@ -1033,6 +1036,7 @@ static bool fd_tune_lookup(const char *path, const phc_fd *h, int *ty, int *tj, 
 extern "C" phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char *cache_path,
                                       int *tile_j, int *block_waves, int *chunk_i)
 {
+    RoctxRange rr_("phc_fd_autotune");
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
     if (steps_per_trial < 1) return fail(PHC_ERR_INVALID, "steps_per_trial must be >= 1");
     int ty = 0, tj = 0, ci = 0, ku = 0;

@@ -1068,6 +1068,7 @@ phc_status phc_lb_add_samples(phc_lb *h, int face, const float *host, int64_t n)
 phc_status phc_lb_collect(phc_lb *h, phc_fd *wide, int cur, const int *origin, const int *dims, float halo,
                           float keep, float s_min, uint64_t seed)
 {
+    RoctxRange rr_("phc_lb_collect");
     if (!h || !wide || !origin || !dims) return fail(PHC_ERR_INVALID, "NULL argument");
     if (cur < 0 || cur > 1) return fail(PHC_ERR_INVALID, "cur level %d is not 0 or 1", cur);
     if (h->cap < 1) return fail(PHC_ERR_STATE, "no sample store: call phc_lb_set_capacity first");
@@ -1114,6 +1115,7 @@ phc_status phc_lb_collect(phc_lb *h, phc_fd *wide, int cur, const int *origin, c
 phc_status phc_lb_collect_onpolicy(phc_lb *h, phc_fd *thin, int thin_cur, phc_fd *wide, int wide_cur,
                                    const int *origin, float halo, float keep, float s_min, uint64_t seed)
 {
+    RoctxRange rr_("phc_lb_collect_onpolicy");
     if (!h || !thin || !wide || !origin) return fail(PHC_ERR_INVALID, "NULL argument");
     if (thin_cur < 0 || thin_cur > 1 || wide_cur < 0 || wide_cur > 1)
         return fail(PHC_ERR_INVALID, "levels must be 0 or 1");
@@ -1275,6 +1277,7 @@ static phc_status lb_train_check(phc_lb *h, int steps, int batch_per_face, float
 // repeats bit for bit.
 phc_status phc_lb_train(phc_lb *h, int steps, int batch_per_face, float lr, uint64_t seed, float *loss_history)
 {
+    RoctxRange rr_("phc_lb_train");
     int64_t c[LB_FACES];
     PHC_TRY(lb_train_check(h, steps, batch_per_face, lr, c));
     int nblocks = 0;

@@ -9,9 +9,20 @@
 #include <cstdio>
 #include <string>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "phc.h"
 
 namespace phc {
+
+// A roctx range over a host-side API call (the run, the step, the tuning):
+// rocprofv3 --marker-trace shows it beside the kernels it enqueued.
+struct RoctxRange {
+    explicit RoctxRange(const char *name) { roctxRangePushA(name); }
+    ~RoctxRange() { roctxRangePop(); }
+    RoctxRange(const RoctxRange &) = delete;
+    RoctxRange &operator=(const RoctxRange &) = delete;
+};
 
 void set_error(const char *fmt, ...);
 phc_status fail(phc_status code, const char *fmt, ...);

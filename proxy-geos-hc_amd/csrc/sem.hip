@@ -1372,6 +1372,7 @@ phc_status phc_sem_sizes(phc_sem *h, int *num_nodes, int *nx, int *ny, int *nz)
 
 phc_status phc_sem_compute_one_step(phc_sem *h, int time_sample, int i1, int i2)
 {
+    RoctxRange rr_("phc_sem_compute_one_step");
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
     if (i1 < 0 || i1 > 1 || i2 < 0 || i2 > 1 || i1 == i2)
         return fail(PHC_ERR_INVALID, "levels i1=%d i2=%d must be 0 and 1", i1, i2);
@@ -1384,6 +1385,7 @@ phc_status phc_sem_compute_one_step(phc_sem *h, int time_sample, int i1, int i2)
 phc_status phc_sem_compute_one_step_rhs(phc_sem *h, int i1, int i2, int nrhs, const int *rhs_element,
                                         const float *rhs_value)
 {
+    RoctxRange rr_("phc_sem_compute_one_step_rhs");
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
     if (i1 < 0 || i1 > 1 || i2 < 0 || i2 > 1 || i1 == i2)
         return fail(PHC_ERR_INVALID, "levels i1=%d i2=%d must be 0 and 1", i1, i2);
@@ -1450,6 +1452,7 @@ static phc_status sem_enqueue_run(phc_sem *h, int it0, int nsteps, int i1, int i
 
 phc_status phc_sem_run(phc_sem *h, int it0, int nsteps, int *i1p, int *i2p)
 {
+    RoctxRange rr_("phc_sem_run");
     if (!h || !i1p || !i2p) return fail(PHC_ERR_INVALID, "NULL argument");
     const int i1 = *i1p, i2 = *i2p;
     if (i1 < 0 || i1 > 1 || i2 < 0 || i2 > 1 || i1 == i2)
@@ -1639,6 +1642,7 @@ phc_status phc_sem_synchronize(phc_sem *h)
 extern "C" phc_status phc_sem_autotune(phc_sem *h, int steps_per_trial, const char *cache_path,
                                        int *tile_j, int *block_waves, int *chunk_i)
 {
+    RoctxRange rr_("phc_sem_autotune");
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
     if (steps_per_trial < 1) return fail(PHC_ERR_INVALID, "steps_per_trial must be >= 1");
     if (h->kernel != PHC_SEM_SEPARABLE) {

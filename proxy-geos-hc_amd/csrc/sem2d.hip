@@ -497,6 +497,7 @@ phc_status phc_sem2d_sizes(phc_sem2d *h, int *num_nodes, int *nx, int *nz)
 
 phc_status phc_sem2d_compute_one_step(phc_sem2d *h, int time_sample, int i1, int i2)
 {
+    RoctxRange rr_("phc_sem2d_compute_one_step");
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
     if (i1 < 0 || i1 > 1 || i2 < 0 || i2 > 1 || i1 == i2)
         return fail(PHC_ERR_INVALID, "levels i1=%d i2=%d must be 0 and 1", i1, i2);
@@ -509,6 +510,7 @@ phc_status phc_sem2d_compute_one_step(phc_sem2d *h, int time_sample, int i1, int
 phc_status phc_sem2d_compute_one_step_rhs(phc_sem2d *h, int i1, int i2, int nrhs, const int *rhs_element,
                                           const float *rhs_value)
 {
+    RoctxRange rr_("phc_sem2d_compute_one_step_rhs");
     if (!h) return fail(PHC_ERR_INVALID, "handle is NULL");
     if (i1 < 0 || i1 > 1 || i2 < 0 || i2 > 1 || i1 == i2)
         return fail(PHC_ERR_INVALID, "levels i1=%d i2=%d must be 0 and 1", i1, i2);
@@ -572,6 +574,7 @@ static phc_status s2_enqueue_run(phc_sem2d *h, int it0, int nsteps, int i1, int 
 
 phc_status phc_sem2d_run(phc_sem2d *h, int it0, int nsteps, int *i1p, int *i2p)
 {
+    RoctxRange rr_("phc_sem2d_run");
     if (!h || !i1p || !i2p) return fail(PHC_ERR_INVALID, "NULL argument");
     const int i1 = *i1p, i2 = *i2p;
     if (i1 < 0 || i1 > 1 || i2 < 0 || i2 > 1 || i1 == i2)

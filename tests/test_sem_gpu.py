@@ -122,7 +122,8 @@ def test_every_block_shape_gives_the_same_bits(gpu, tmp_path, ex, ey, ez, order)
     ref.run(30)
     # chunk -m: the per-XCD schedules (8 >> (m-1) plane ranges x 2^(m-1) column bands)
     for waves, tile_j, chunk in [(16, 16, 8), (16, 32, 12), (8, 16, 1 << 20), (4, 8, 20), (4, 4, 4),
-                                 (16, 16, -1), (4, 8, -1), (16, 16, -2), (4, 8, -3), (16, 16, -4)]:
+                                 (8, 8, 16), (16, 16, -1), (4, 8, -1), (16, 16, -2), (4, 8, -3), (16, 16, -4),
+                                 (8, 8, -1), (8, 16, -2)]:
         banner("shape waves=%d tile_j=%d chunk=%d" % (waves, tile_j, chunk))
         cache = tmp_path / ("s%d_%d_%d.txt" % (waves, tile_j, chunk))
         cache.write_text("sem %d %d %d %d 0 %d %d %d 1.0\n" % (ex, ey, ez, order, waves, tile_j, chunk))
