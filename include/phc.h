@@ -86,8 +86,10 @@ typedef struct {
                         * every array is below 4 GiB) */
     int k_remainder;   /* a last, partial 64-point tile along k: 0 (default)
                         * updated point by point when it holds at most 32
-                        * points, as a whole tile otherwise; 1 always point by
-                        * point; -1 always as a tile.  Same values either way. */
+                        * points, as a whole tile otherwise; likewise a last
+                        * tile along j holding at most a quarter of tile_j
+                        * rows; 1 always point by point; -1 always as a tile.
+                        * Same values either way. */
 } phc_fd_params;
 
 phc_status phc_fd_create(const phc_fd_params *p, phc_fd **out);

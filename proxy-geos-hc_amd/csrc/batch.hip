@@ -54,6 +54,7 @@ struct FdGrid {
     float *trace;
     int ilo, ihi, jlo, jhi, klo, khi;  // every point of the shot
     int krem;
+    int jrem;  // no j-remainder slab in the batch (0)
 };
 
 // Work queues of one step: queue q holds items[start[q] .. start[q]+len[q]),
@@ -109,6 +110,7 @@ fd_batch_kernel(FdStepArgs c, const ShotDesc *__restrict__ shots, const WorkItem
             g.jlo = 0; g.jhi = sd.L.ny;
             g.klo = 0; g.khi = sd.L.nz;
             g.krem = sd.krem;
+            g.jrem = 0;
             if (wi.kt < 0) {
                 constexpr int T = FD_TK * TY;
                 for (int m = 0; m < BATCH_SLAB_PASSES; ++m)

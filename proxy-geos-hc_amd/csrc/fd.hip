@@ -364,21 +364,33 @@ static int fd_num_blocks(const FdStepArgs &a)
     return a.tiles_k * a.tiles_j * chunks;
 }
 
-// k-remainder slab (mode 0: when the box's last k tile would hold 1..32
-// points; 1: whenever it is partial; -1: never): the tiles stop at the last
-// full 64-lane tile and extra blocks update the slab point by point.
+// Remainder slabs (mode 0: the box's last k tile when it would hold 1..32
+// points, and its last j tile when it would hold at most a quarter of its
+// TJ rows; 1: whenever either is partial; -1: never): the tiles stop at the
+// last full tile and extra blocks update the slabs point by point.
 // Returns the number of blocks to launch.
-static int fd_plan_blocks(FdStepArgs &a, int mode, int block_threads)
+static int fd_plan_blocks(FdStepArgs &a, int mode, int block_threads, int TJ)
 {
     a.nb_main = 0;
     a.krem = 0;
+    a.jrem = 0;
     const int full = a.khi / FD_TK, rem = a.khi - full * FD_TK;
-    const bool use = mode >= 0 && full > 0 && rem > 0 && FD_TK * full >= a.klo && (mode == 1 || rem <= FD_TK / 2);
-    if (!use) return fd_num_blocks(a);
-    a.tiles_k = full;
-    a.krem = FD_TK * full;
+    const bool usek = mode >= 0 && full > 0 && rem > 0 && FD_TK * full >= a.klo && (mode == 1 || rem <= FD_TK / 2);
+    const int rows = a.jhi - a.jlo, fullj = rows / TJ, remj = rows - fullj * TJ;
+    const bool usej = mode >= 0 && fullj > 0 && remj > 0 && (mode == 1 || 4 * remj <= TJ);
+    if (!usek && !usej) return fd_num_blocks(a);
+    if (usek) {
+        a.tiles_k = full;
+        a.krem = FD_TK * full;
+    }
+    if (usej) {
+        a.tiles_j = fullj;
+        a.jrem = a.jlo + TJ * fullj;
+    }
     a.nb_main = fd_num_blocks(a);
-    const int64_t pts = (int64_t)rem * (a.jhi - a.jlo) * (a.ihi - a.ilo);
+    const int NI = a.ihi - a.ilo;
+    int64_t pts = usek ? (int64_t)rem * rows * NI : 0;
+    if (usej) pts += (int64_t)((usek ? a.krem : a.khi) - a.klo) * remj * NI;
     return a.nb_main + (int)((pts + block_threads - 1) / block_threads);
 }
 
@@ -439,7 +451,7 @@ static phc_status fd_prepare_sched(phc_fd *h, const FdStepArgs &a)
 // Blocks to launch for `a` (tiles, k-remainder slab, schedule table set in a).
 static phc_status fd_plan_launch(phc_fd *h, FdStepArgs &a, int *nb)
 {
-    *nb = fd_plan_blocks(a, h->krem_eff, FD_TK * h->ty);
+    *nb = fd_plan_blocks(a, h->krem_eff, FD_TK * h->ty, h->ty * h->nj);
     a.sched = nullptr;
     if (h->chunk_i < 0) {
         const int slab = a.nb_main ? *nb - a.nb_main : 0;
@@ -1062,8 +1074,8 @@ extern "C" phc_status phc_fd_autotune(phc_fd *h, int steps_per_trial, const char
         double best = 1e30;
         int bty = h->ty, bnj = h->nj, bci = h->chunk_i, bku = h->krem_eff;
         phc_status st = PHC_OK;
-        const int rem = h->nz % FD_TK;
-        const bool both = h->krem_mode == 0 && rem > 0 && rem <= FD_TK / 2 && h->nz > FD_TK;
+        // with mode 0 the plain partial tiles (-1) are timed as well
+        const bool both = h->krem_mode == 0;
         const int modes[2] = {h->krem_mode, -1};
         for (int mi = 0; mi < (both ? 2 : 1) && st == PHC_OK; ++mi)
         for (auto &sh : shapes) {

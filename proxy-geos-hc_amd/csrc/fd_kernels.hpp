@@ -75,6 +75,10 @@ struct FdStepArgs {
     // from nb_main on update the slab [krem, khi) one point per thread
     int nb_main;        // marching blocks (0: every block marches)
     int krem;
+    // j-remainder slab: when the last TJ-row tile along j would hold at most
+    // a quarter of its rows, the tiles cover j < jrem only and the same extra
+    // blocks also update [klo, krem or khi) x [jrem, jhi) (0: none)
+    int jrem;
     // per-XCD schedule (null: one block per (tile, chunk of chunk_i planes)
     // in XCD block order): hardware block b < nb_main marches tile column
     // sched[b].x (k tile + tiles_k * j tile) over planes [sched[b].y,

@@ -342,26 +342,42 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
     }
 }
 
-// The k-remainder slab [g.krem, khi) x [jlo, jhi) x [ilo, ihi) of one grid,
-// one point per thread: thread t of the call updates point first + t of the
-// slab (k fastest); the tiles cover k < g.krem.  The same update as fd_tile,
-// with the same operations in the same order, so a point gets the same bits
-// from either path; the stencil values come straight from memory (the slab
-// is at most 32 points wide: these loads are L2 hits of the rows the
+// The remainder slabs of one grid, one point per thread: thread t of the call
+// updates point first + t of the k slab [g.krem, khi) x [jlo, jhi) x
+// [ilo, ihi) (when g.krem > 0), then of the j slab [klo, krem or khi) x
+// [g.jrem, jhi) x [ilo, ihi) (when g.jrem > 0), k fastest; the tiles cover
+// the rest.  The same update as fd_tile, with the same operations in the
+// same order, so a point gets the same bits from either path; the stencil
+// values come straight from memory (a slab is at most 32 points wide along
+// k or a quarter tile along j: these loads are L2 hits of the rows the
 // neighbouring tiles stream).
 template <int R, int ARITH, typename IT, bool PML = false, class G>
 __device__ __forceinline__ void fd_rem_points(const FdStepArgs &a, const G &g, int64_t first, int nthreads)
 {
     constexpr IT S = sizeof(IT) == 4 ? 4 : 1;
     const FdLayout &L = g.L;
-    const int W = g.khi - g.krem, NJ = g.jhi - g.jlo;
-    const int64_t t = first + (int64_t)threadIdx.y * blockDim.x + threadIdx.x;
-    if (W <= 0 || t >= (int64_t)W * NJ * (g.ihi - g.ilo)) return;
     (void)nthreads;
-    const int k = g.krem + (int)(t % W);
-    const int64_t rest = t / W;
-    const int j = g.jlo + (int)(rest % NJ);
-    const int i = g.ilo + (int)(rest / NJ);
+    int64_t t = first + (int64_t)threadIdx.y * blockDim.x + threadIdx.x;
+    const int NI = g.ihi - g.ilo;
+    const int64_t nA = g.krem > 0 ? (int64_t)(g.khi - g.krem) * (g.jhi - g.jlo) * NI : 0;
+    int i, j, k;
+    if (t < nA) {
+        const int W = g.khi - g.krem, NJ = g.jhi - g.jlo;
+        k = g.krem + (int)(t % W);
+        const int64_t rest = t / W;
+        j = g.jlo + (int)(rest % NJ);
+        i = g.ilo + (int)(rest / NJ);
+    } else {
+        t -= nA;
+        if (g.jrem <= 0) return;
+        const int kend = g.krem > 0 ? g.krem : g.khi;
+        const int W = kend - g.klo, NJ = g.jhi - g.jrem;
+        if (W <= 0 || NJ <= 0 || t >= (int64_t)W * NJ * NI) return;
+        k = g.klo + (int)(t % W);
+        const int64_t rest = t / W;
+        j = g.jrem + (int)(rest % NJ);
+        i = g.ilo + (int)(rest / NJ);
+    }
     const IT si = (IT)L.pitch_j * S, sj = (IT)L.pitch_k * S;
     const IT o = (IT)fd_off(L, i, j, k) * S;
     const float pc = ldo(g.pn, o);

@@ -124,6 +124,42 @@ def test_k_remainder_slab_bit_exact(gpu, oracle, nz, mode):
     assert np.array_equal(src2, tr2) and np.array_equal(p2.current(), P2.current())
 
 
+@pytest.mark.parametrize("ny,nz,mode,tile_j", [(33, 70, 0, 16), (35, 96, 0, 16), (50, 64, 0, 16), (49, 131, 0, 8),
+                                              (41, 45, 1, 16)])
+def test_j_remainder_slab_bit_exact(gpu, oracle, ny, nz, mode, tile_j):
+    """A last j tile holding a quarter of its rows or fewer is updated point
+    by point by the same extra blocks as the k slab: reference rounding =
+    the oracle, with and without a k slab, in fused runs, in the per-step
+    surface and with the PML; FP32 equal to the plain tiles bit for bit."""
+    banner("j remainder ny=%d nz=%d mode=%d tile_j=%d" % (ny, nz, mode, tile_j))
+    s, P = make_pair(gpu, oracle, 30, ny, nz, sponge=7, nsamples=40, xs=15, ys=ny - 2, zs=nz // 2)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF, k_remainder=mode, tile_j=tile_j, block_waves=tile_j // 2)
+    rec = np.array([[15, ny - 2, nz // 2], [5, ny - 1, nz - 1], [29, ny - 3, 3]], np.int32)
+    sol.set_receivers(rec)
+    src, rv = sol.run(0, 25)
+    tr, orv = P.run(25, receivers=rec)
+    assert np.array_equal(src, tr) and np.array_equal(rv, orv)
+    assert np.array_equal(sol.current(), P.current())
+    for it in range(25, 30):
+        sol.step(it)
+    P.run(5)
+    assert np.array_equal(sol.current(), P.current())
+    assert np.array_equal(sol.previous(), P.previous())
+    f = [gpu.FdSolver(s, arith=gpu.ARITH_FP32, k_remainder=m, tile_j=tile_j, block_waves=tile_j // 2)
+         for m in (-1, mode)]
+    for x in f:
+        x.run(0, 30)
+    assert np.array_equal(f[0].current(), f[1].current())
+    s2 = gpu.FdSetup(nx=28, ny=ny, nz=nz, L=4, f0=30.0, sponge_width=0, model="two_layer", nsamples=30)
+    P2 = oracle.FdProblem(nx=28, ny=ny, nz=nz, L=4, f0=30.0, model=s2.vp.copy(), nsamples=30, sponge_width=0,
+                          pml=True, ndamp=6)
+    p2 = gpu.FdSolver(s2, arith=gpu.ARITH_REF, k_remainder=mode, tile_j=tile_j, block_waves=tile_j // 2)
+    p2.set_pml(ndamp=6, f0=30.0)
+    src2, _ = p2.run(0, 30)
+    tr2, _ = P2.run(30)
+    assert np.array_equal(src2, tr2) and np.array_equal(p2.current(), P2.current())
+
+
 def test_add_rhs_value_equals_add_rhs(gpu):
     """addRHS with the caller's sample (phc_fd_add_rhs_value) gives the bits of
     addRHS from the handle's samples."""
