@@ -382,6 +382,7 @@ def bench_learned(phc, n=256, steps=2000):
     train_s = time.perf_counter() - t0
     r = learned.compare(lb, n=n, steps=steps)
     r.pop("truth_traces")
+    r["rollout_8000"] = learned.rollout(lb, n=n, steps=8000)
     r["train_wall_s"] = round(train_s, 2)
     r["train_final_loss"] = float(hist[-1])
     r["train_input_noise"] = LB_TRAIN_NOISE

@@ -188,14 +188,26 @@ def layer_inputs(pn, pnm1, vp, origin, n, f, h, c1, chat=None):
     return X, s, d, v, tgt
 
 
-def corrections(theta, pn, pnm1, vp, n, h, c1):
-    """Per face: vp2 c_1 s y at every layer point of a whole grid (compact order)."""
+def ymax(chat, d):
+    """Bound of y at layer depth d: sum_{l>d} |c_l / c_1|, summed in float32
+    (the run-time clamp of the correction pass, csrc/lb.hpp lb_ymax)."""
+    m = np.float32(0.0)
+    for l in range(d + 1, R + 1):
+        m = np.float32(m + np.float32(abs(np.float32(chat[l]))))
+    return float(m)
+
+
+def corrections(theta, pn, pnm1, vp, n, h, c1, chat):
+    """Per face: vp2 c_1 s y at every layer point of a whole grid (compact
+    order), y clamped to +-ymax(depth)."""
     th = bf16(np.asarray(theta, np.float32).reshape(FACES, NP))
+    bound = np.array([ymax(chat, d) for d in range(W)])
     out = []
     for f in range(FACES):
         X, s, d, v, _ = layer_inputs(pn, pnm1, vp, (0, 0, 0), n, f, h, c1)
         y, _ = forward(th[f], X, d)
-        out.append(v * c1 * s * y)
+        yb = bound[d.astype(int)]
+        out.append(v * c1 * s * np.clip(y, -yb, yb))
     return out
 
 

@@ -18,6 +18,14 @@
 // and the new value at the point gains vp^2dt^2 * c_1 * s * y.  Scaling by s
 // makes the prediction proportional to the local amplitude (the wave
 // equation is linear), so one network serves every source strength.
+//
+// At run time y is clamped to +-ymax(d) = sum_{l=d+1..4} |c_l / c_1|: the
+// largest |E| / (c_1 s) a field no larger beyond the edge than inside the
+// stencil window can give.  Without the bound a trained layer slowly feeds
+// energy back in long runs (its network, evaluated on the low-amplitude
+// remainder it meets late in a run, is outside what it was trained on);
+// with it an 8000-step run stays below the bare grid's amplitude while the
+// 2000-step traces are unchanged to 1%.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -57,6 +65,14 @@ struct LbInfer {
     float chat[LB_R + 1];  // c_l / c_1
     float ldamp[LB_W];   // new value at layer depth d multiplied by ldamp[d] (per face holding it)
 };
+
+// Bound of the network's output at layer depth d (see above), in float.
+__host__ __device__ inline float lb_ymax(const float chat[LB_R + 1], int d)
+{
+    float m = 0.f;
+    for (int l = d + 1; l <= LB_R; ++l) m += chat[l] < 0.f ? -chat[l] : chat[l];
+    return m;
+}
 
 // Launch the correction pass (before the step: reads the current and previous
 // levels) and the apply pass (after the step: adds the corrections to the new

@@ -627,6 +627,7 @@ struct LbCorrectTileArgs {
     int64_t off[LB_FACES + 1];
     int blk0[LB_FACES + 1];  // first block of each face
     float h, c1;
+    float ymax[LB_W];        // bound of y at each depth (lb.hpp)
 };
 
 __global__ void __launch_bounds__(256) lb_correct_tile_kernel(LbCorrectTileArgs A)
@@ -703,7 +704,7 @@ __global__ void __launch_bounds__(256) lb_correct_tile_kernel(LbCorrectTileArgs 
         const float vraw = x[18];
         const float s = lb_finish_inputs(d, A.h, A.c1, x);
         lb_lane_inputs(x, xk);
-        const float y = lb_mlp_wave(W, b1s, xk, d);
+        const float y = fminf(fmaxf(lb_mlp_wave(W, b1s, xk, d), -A.ymax[d]), A.ymax[d]);
         if ((threadIdx.x & 32) == 0 && vok)
             A.corr[A.off[f] + ((int64_t)d * g.nu + u) * g.nv + v] = vraw * A.c1 * s * y;
     }
@@ -803,6 +804,7 @@ hipError_t lb_enqueue_correct(const LbInfer &lb, const FdLayout &L, const float 
     T.h = lb.halo;
     T.c1 = lb.c1;
     T.blk0[0] = 0;
+    for (int d = 0; d < LB_W; ++d) T.ymax[d] = lb_ymax(lb.chat, d);
     for (int f = 0; f <= LB_FACES; ++f) T.off[f] = lb.corr_off[f];
     const int nn[3] = {L.nx, L.ny, L.nz};
     for (int f = 0; f < LB_FACES; ++f) {

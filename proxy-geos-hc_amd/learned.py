@@ -252,6 +252,35 @@ def compare(lb, n=256, steps=2000, f0=10.0, seed=20240512, src=None, with_none=T
     return out
 
 
+def rollout(lb, n=256, steps=8000, every=1000, f0=10.0, seed=20240512, layer_damping=0.0):
+    """One shot on an n^3 region run long: the largest |p| on the grid after
+    every `every` steps, with the learned layer and on the bare grid (its edge
+    reflects, the energy stays).  The Ricker source has faded after a few
+    hundred steps, so with an absorbing layer the maximum keeps falling, while
+    a layer that feeds energy back makes it grow past the bare grid's."""
+    rng = np.random.default_rng(seed)
+    inter, vel = np.sort(rng.choice(np.arange(8, n - 8), 5, replace=False)), np.linspace(VMIN, VMAX, 6)
+    dt = fd_dt()
+    out = {"region": n, "steps": steps, "every": every}
+    for name, model in (("learned", lb), ("none", None)):
+        N = n + 2 * LAYER
+        vp = layered_vp2dt2((N, N, N), LAYER, inter, vel, dt)
+        s = phc.FdSetup(N, N, N, L=4, dx=DX, xs=n // 2 + LAYER, ys=n // 2 + LAYER, zs=n // 2 + LAYER, f0=f0,
+                        cfl=CFL, sponge_width=0, model=vp, nsamples=steps + 1)
+        sol = phc.FdSolver(s, arith=phc.ARITH_FP32, init_value=0.0)
+        if model is not None:
+            sol.set_learned_boundary(model, halo=0.0, layer_damping=layer_damping)
+        maxima = []
+        for it in range(0, steps, every):
+            sol.run(it, min(every, steps - it))
+            f = sol.current()
+            maxima.append(float(np.abs(f).max()) if np.isfinite(f).all() else float("inf"))
+        out[name] = maxima
+        sol.close()
+    out["bounded"] = bool(max(out["learned"]) <= 2.0 * max(out["none"]))
+    return out
+
+
 def save_params(lb, path):
     lb.params().astype("<f4").tofile(path)
 

@@ -118,6 +118,12 @@ def thin_solver(gpu, n, steps, seed=4, src=None):
     return s
 
 
+def chat_of(s):
+    """c_l / c_1 in float32, as the FD handle forms them for the layer."""
+    c = np.asarray(s.cx, np.float32)
+    return (c / c[1]).astype(np.float32)
+
+
 def test_correction_pass_matches_oracle(gpu):
     banner("correction pass vs oracle")
     rng = np.random.default_rng(13)
@@ -131,7 +137,7 @@ def test_correction_pass_matches_oracle(gpu):
     got = sol.lb_corrections()
     pn, pm = sol.current().astype(np.float64), sol.previous().astype(np.float64)
     vp3 = s.vp.reshape(s.nx, s.ny, s.nz)
-    ref = lo.corrections(theta, pn, pm, vp3, (s.nx, s.ny, s.nz), 0.0, float(s.cx[1]))
+    ref = lo.corrections(theta, pn, pm, vp3, (s.nx, s.ny, s.nz), 0.0, float(s.cx[1]), chat_of(s))
     for f in range(lo.FACES):
         scale = max(np.max(np.abs(ref[f])), 1e-30)
         assert np.max(np.abs(got[f] - ref[f])) <= 3e-4 * scale, f
@@ -158,7 +164,7 @@ def test_learned_steps_match_cpu_emulation(gpu, oracle):
     nsteps = 70
     for it in range(nsteps):
         corr = lo.corrections(theta, P.current().astype(np.float64), P.previous().astype(np.float64), vp3,
-                              (n, n, n), np.float64(np.float32(1e-6)), c1)
+                              (n, n, n), np.float64(np.float32(1e-6)), c1, chat_of(s))
         P.run(1)
         cur = P.current()
         upd = cur.astype(np.float64)
@@ -207,7 +213,7 @@ def test_learned_steps_full_size_264(gpu, oracle):
     c1 = float(s.cx[1])
     for it in range(3):
         corr = lo.corrections(theta, P.current().astype(np.float64), P.previous().astype(np.float64), vp3,
-                              (n, n, n), 0.0, c1)
+                              (n, n, n), 0.0, c1, chat_of(s))
         assert max(np.abs(c).max() for c in corr) > 0
         P.run(1)
         cur = P.current()

@@ -135,3 +135,28 @@ def test_inputs_on_a_linear_ramp():
         assert X[q, 18] == pytest.approx(np.sqrt(0.04 * 4.0))
         exp_t = sum([0, 1, 0.5, 0.25, 0.125][l] * (2.0 + 0.5 * (i - l)) for l in range(dd + 1, 5)) / s[q]
         assert t[q] == pytest.approx(exp_t)
+
+
+def test_output_bound_closed_form_and_clamp():
+    """ymax(d) = sum_{l>d} |c_l / c_1| for the radius-4 central weights
+    (8/5, -1/5, 8/315, -1/560): at depth 0 (on the edge) all four terms are
+    missing, 1 + 1/8 + 1/63 + 1/896, down to 1/896 at depth 3; corrections()
+    clamps y there (a network whose output is a large constant gives exactly
+    v c_1 s ymax)."""
+    c = np.array([0.0, 8 / 5, -1 / 5, 8 / 315, -1 / 560], np.float32)
+    chat = (c / c[1]).astype(np.float32)
+    exact = [1 + 1 / 8 + 1 / 63 + 1 / 896, 1 / 8 + 1 / 63 + 1 / 896, 1 / 63 + 1 / 896, 1 / 896]
+    for d in range(lo.W):
+        assert abs(lo.ymax(chat, d) - exact[d]) <= 1e-7
+    n = (20, 18, 22)
+    rng = np.random.default_rng(5)
+    ext = (n[0] + 2 * lo.R, n[1] + 2 * lo.R, n[2] + 2 * lo.R)
+    pn, pm = rng.normal(0, 1, ext), rng.normal(0, 1, ext)
+    vp = rng.uniform(0.1, 0.4, n)
+    theta = np.zeros((lo.FACES, lo.NP), np.float32)
+    theta[:, lo.OFF_B3] = 64.0            # y = 64 everywhere (exact in bf16)
+    corr = lo.corrections(theta, pn, pm, vp, n, 0.0, float(c[1]), chat)
+    for f in range(lo.FACES):
+        X, s, d, v, _ = lo.layer_inputs(pn, pm, vp, (0, 0, 0), n, f, 0.0, float(c[1]))
+        bound = np.array([lo.ymax(chat, k) for k in range(lo.W)])[d.astype(int)]
+        assert np.allclose(corr[f], v * float(c[1]) * s * bound, rtol=1e-12, atol=0)

@@ -1,8 +1,8 @@
 """The learned boundary end to end at a small size: training data from
 wide-sponge shots, GPU training, then one shot on a region with the thin
 learned layer, with no absorbing boundary and with the 40-point sponge,
-against the 2x-wide FP64 run.  The trained layer must be stable and must
-reflect markedly less than the bare thin grid."""
+against the 2x-wide FP64 run.  The trained layer must be stable (also over
+a 4000-step run) and must reflect markedly less than the bare thin grid."""
 import numpy as np
 import pytest
 
@@ -26,3 +26,12 @@ def test_trained_layer_beats_the_bare_thin_grid(gpu):
     print(res, flush=True)
     assert res["learned"]["finite"] and res["none"]["finite"] and res["sponge40"]["finite"]
     assert res["learned"]["trace_rel_l2"] < 0.5 * res["none"]["trace_rel_l2"]
+    # a long run: the layer (its output clamped to the largest missing term a
+    # field no larger beyond the edge than inside can give) keeps the field
+    # below the bare reflecting grid's amplitude over 4000 steps
+    print("[lb-pipeline] 4000-step rollout", flush=True)
+    ro = learned.rollout(lb, n=48, steps=4000, every=500, f0=12.0)
+    print(ro, flush=True)
+    assert np.all(np.isfinite(ro["learned"]))
+    assert max(ro["learned"][2:]) <= max(ro["none"][2:])
+    assert ro["learned"][-1] < ro["learned"][0]
