@@ -330,6 +330,8 @@ def bench_sem_elem(phc, steps=100, steps_p4=20):
 # std of the Gaussian noise on the learned layer's training inputs: without it
 # trained layers grow an instability in long rollouts in most runs
 LB_TRAIN_NOISE = 0.02
+# wide-sponge training shots (64^3 regions)
+LB_TRAIN_SHOTS = 400
 
 
 def bench_learned_step(phc, n=256, steps=300):
@@ -371,13 +373,14 @@ def bench_learned_step(phc, n=256, steps=300):
 
 
 def bench_learned(phc, n=256, steps=2000):
-    """Learned thin boundary: trained from scratch on the GPU (200 wide-sponge
+    """Learned thin boundary: trained from scratch on the GPU (400 wide-sponge
     shots, Adam), then one shot on an n^3 region: the 4-point learned layer,
-    the 40-point sponge and the bare grid against a 2x-wide FP64 domain."""
+    the 40-point sponge, the PML and the bare grid against a 2x-wide FP64
+    domain, and an 8000-step run of the layer beside the bare grid."""
     import learned
     t0 = time.perf_counter()
     lb = phc.LearnedBoundary(seed=11)
-    learned.training_data(lb, n_shots=200, region=64)
+    learned.training_data(lb, n_shots=LB_TRAIN_SHOTS, region=64)
     hist = learned.train(lb, steps=6000, noise=LB_TRAIN_NOISE)
     train_s = time.perf_counter() - t0
     r = learned.compare(lb, n=n, steps=steps)

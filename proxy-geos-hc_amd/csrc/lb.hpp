@@ -21,11 +21,12 @@
 //
 // At run time y is clamped to +-ymax(d) = sum_{l=d+1..4} |c_l / c_1|: the
 // largest |E| / (c_1 s) a field no larger beyond the edge than inside the
-// stencil window can give.  Without the bound a trained layer slowly feeds
-// energy back in long runs (its network, evaluated on the low-amplitude
-// remainder it meets late in a run, is outside what it was trained on);
-// with it an 8000-step run stays below the bare grid's amplitude while the
-// 2000-step traces are unchanged to 1%.
+// stencil window can give.  Without the bound a layer trained with input
+// noise slowly feeds energy back in long runs (its network, evaluated on the
+// low-amplitude remainder it meets late in a run, is outside what it was
+// trained on); with it an 8000-step run stays below the bare grid's
+// amplitude while the 2000-step traces are unchanged to 1%.  The bound does
+// not rescue a layer trained without input noise.
 #pragma once
 
 #include <hip/hip_runtime.h>
