@@ -374,7 +374,10 @@ typedef struct {
     float init_value;         /* initial value of both levels */
     int arith;                /* PHC_FD_ARITH_* */
     int tile_j, block_waves;  /* 0 = default (16 x 16) */
-    int chunk_i;              /* 0 = default (64) */
+    int chunk_i;              /* planes per work item, 0 = default (64); -m
+                               * (m = 1..4): each shot's per-XCD schedule
+                               * (as phc_fd_params.chunk_i), XCD x's entries
+                               * in queue x, shot after shot */
     int blocks_per_cu;        /* 0 = default (1) */
     int k_remainder;          /* a shot's last, partial k tile: as in phc_fd_params */
 } phc_batch_params;

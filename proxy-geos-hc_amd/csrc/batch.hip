@@ -18,6 +18,7 @@
 
 #include "fd_kernels.hpp"
 #include "fd_tile.hpp"
+#include "xcd_sched.hpp"
 #include "march.hpp"
 #include "phc_common.hpp"
 
@@ -37,8 +38,10 @@ struct ShotDesc {
 // One unit of work: tile column (kt, jt) of i-chunk ic of one shot; kt < 0:
 // part jt of the shot's k-remainder slab, BATCH_SLAB_PASSES x the block's
 // threads points in slab order.
+// A tile column (k tile kt, j tile jt) of one shot marched over planes
+// [i0, i1); kt < 0: part jt of the shot's remainder slab.
 struct WorkItem {
-    int shot, kt, jt, ic;
+    int shot, kt, jt, i0, i1;
 };
 constexpr int BATCH_SLAB_PASSES = 8;
 
@@ -117,8 +120,7 @@ fd_batch_kernel(FdStepArgs c, const ShotDesc *__restrict__ shots, const WorkItem
                     fd_rem_points<R, ARITH, IT>(c, g, ((int64_t)wi.jt * BATCH_SLAB_PASSES + m) * T, T);
                 continue;
             }
-            const int i0 = wi.ic * c.chunk_i;
-            fd_tile<R, TY, NJ, ARITH, IT>(c, g, wi.kt, wi.jt, i0, min(i0 + c.chunk_i, g.ihi), tile);
+            fd_tile<R, TY, NJ, ARITH, IT>(c, g, wi.kt, wi.jt, wi.i0, wi.i1, tile);
         }
     }
 }
@@ -212,54 +214,95 @@ static void batch_drop_graph(phc_batch *h)
     h->g_it0 = h->g_n = h->g_cur = h->g_pending = h->g_nrec = -1;
 }
 
-// Work items of every shot for the current shape in spatial order (shot,
-// i-chunk, j-tile, k-tile, then the shot's remainder slab parts), cut into
-// BATCH_QUEUES contiguous runs of about equal cost (planes incl. the 2R
-// lead-in x rows; a slab point costs a few tile points: it reads its stencil
-// from memory).  A shot whose last k tile would hold 1..32 points (mode 0;
-// any partial tile with mode 1, none with -1) gets the slab instead.
+// Work items of every shot for the current shape, dealt to BATCH_QUEUES
+// queues (one per XCD).  A shot whose last k tile would hold 1..32 points
+// (mode 0; any partial tile with mode 1, none with -1) gets its remainder
+// slab as point-by-point items instead.
+// * chunk_i > 0: items in spatial order (shot, i-chunk, j-tile, k-tile,
+//   then the slab parts), cut into contiguous runs of about equal cost
+//   (planes incl. the 2R lead-in x rows; a slab point costs a few tile
+//   points: it reads its stencil from memory).
+// * chunk_i = -m: each shot's per-XCD schedule (xcd_sched.hpp, split m, one
+//   round = the XCD's resident blocks): queue x takes the entries of XCD x,
+//   shot after shot, so an XCD's blocks march one shot's plane range over a
+//   band of its columns in phase, as the single-grid schedule does; slab
+//   parts go to the queue with the least work so far.
 static phc_status batch_build_items(phc_batch *h)
 {
+    std::vector<std::vector<WorkItem>> qitems(BATCH_QUEUES);
+    std::vector<double> qcost(BATCH_QUEUES, 0.0);
     std::vector<WorkItem> items;
     std::vector<double> cost;
     const int TJ = h->ty * h->nj;
     const int T = FD_TK * h->ty;
+    const bool sched = h->chunk_i < 0;
+    const int slots = std::max(h->cus / BATCH_QUEUES, 1) * h->blocks_per_cu;
+    std::vector<int4> tab;
     for (int s = 0; s < h->nshots; ++s) {
         const FdLayout &L = h->L[s];
         const int full = L.nz / FD_TK, rem = L.nz - full * FD_TK;
         const bool slab = h->krem_eff >= 0 && full > 0 && rem > 0 && (h->krem_eff == 1 || rem <= FD_TK / 2);
         h->hdesc[s].krem = slab ? FD_TK * full : 0;
         const int tk = slab ? full : (L.nz + FD_TK - 1) / FD_TK, tj = (L.ny + TJ - 1) / TJ;
-        const int nc = (L.nx + h->chunk_i - 1) / h->chunk_i;
-        for (int ic = 0; ic < nc; ++ic)
-            for (int jt = 0; jt < tj; ++jt)
-                for (int kt = 0; kt < tk; ++kt) {
-                    items.push_back({s, kt, jt, ic});
-                    const int planes = std::min(h->chunk_i, L.nx - ic * h->chunk_i);
-                    const int rows = std::min(TJ, L.ny - jt * TJ);
-                    cost.push_back((double)(planes + 2 * h->R) * rows);
-                }
+        auto tile_cost = [&](int jt, int planes) {
+            return (double)(planes + 2 * h->R) * std::min(TJ, L.ny - jt * TJ);
+        };
+        if (sched) {
+            build_xcd_schedule(L.nx, 0, tk * tj, slots, -h->chunk_i, 1, tab);
+            for (size_t b = 0; b < tab.size(); ++b) {
+                const int4 e = tab[b];
+                if (e.y >= e.z) continue;
+                const int x = (int)(b % BATCH_QUEUES), kt = e.x % tk, jt = e.x / tk;
+                qitems[x].push_back({s, kt, jt, e.y, e.z});
+                qcost[x] += tile_cost(jt, e.z - e.y);
+            }
+        } else {
+            const int nc = (L.nx + h->chunk_i - 1) / h->chunk_i;
+            for (int ic = 0; ic < nc; ++ic)
+                for (int jt = 0; jt < tj; ++jt)
+                    for (int kt = 0; kt < tk; ++kt) {
+                        const int i0 = ic * h->chunk_i, i1 = std::min(i0 + h->chunk_i, L.nx);
+                        items.push_back({s, kt, jt, i0, i1});
+                        cost.push_back(tile_cost(jt, i1 - i0));
+                    }
+        }
         if (slab) {
             const int64_t pts = (int64_t)rem * L.ny * L.nx, per = (int64_t)BATCH_SLAB_PASSES * T;
             for (int64_t q = 0; q * per < pts; ++q) {
-                items.push_back({s, -1, (int)q, 0});
-                cost.push_back(3.0 * (double)std::min(per, pts - q * per) / FD_TK);
+                const double c = 3.0 * (double)std::min(per, pts - q * per) / FD_TK;
+                if (sched) {
+                    const int x = (int)(std::min_element(qcost.begin(), qcost.end()) - qcost.begin());
+                    qitems[x].push_back({s, -1, (int)q, 0, 0});
+                    qcost[x] += c;
+                } else {
+                    items.push_back({s, -1, (int)q, 0, 0});
+                    cost.push_back(c);
+                }
             }
         }
     }
     // the descriptors carry each shot's slab start
     PHC_HIP(hipMemcpy(h->shots.ptr, h->hdesc.data(), sizeof(ShotDesc) * h->nshots, hipMemcpyHostToDevice));
-    const double total = std::accumulate(cost.begin(), cost.end(), 0.0);
-    int q = 0;
-    double acc = 0.0;
-    h->queues.start[0] = 0;
-    for (size_t i = 0; i < items.size(); ++i) {
-        acc += cost[i];
-        while (q + 1 < BATCH_QUEUES && acc >= total * (q + 1) / BATCH_QUEUES) h->queues.start[++q] = (int)i + 1;
+    if (sched) {
+        for (int k = 0; k < BATCH_QUEUES; ++k) {
+            h->queues.start[k] = (int)items.size();
+            h->queues.len[k] = (int)qitems[k].size();
+            items.insert(items.end(), qitems[k].begin(), qitems[k].end());
+        }
+    } else {
+        const double total = std::accumulate(cost.begin(), cost.end(), 0.0);
+        int q = 0;
+        double acc = 0.0;
+        h->queues.start[0] = 0;
+        for (size_t i = 0; i < items.size(); ++i) {
+            acc += cost[i];
+            while (q + 1 < BATCH_QUEUES && acc >= total * (q + 1) / BATCH_QUEUES) h->queues.start[++q] = (int)i + 1;
+        }
+        while (q + 1 < BATCH_QUEUES) h->queues.start[++q] = (int)items.size();
+        for (int k = 0; k < BATCH_QUEUES; ++k)
+            h->queues.len[k] =
+                (k + 1 < BATCH_QUEUES ? h->queues.start[k + 1] : (int)items.size()) - h->queues.start[k];
     }
-    while (q + 1 < BATCH_QUEUES) h->queues.start[++q] = (int)items.size();
-    for (int k = 0; k < BATCH_QUEUES; ++k)
-        h->queues.len[k] = (k + 1 < BATCH_QUEUES ? h->queues.start[k + 1] : (int)items.size()) - h->queues.start[k];
     h->nitems = (int)items.size();
     PHC_TRY(h->items.alloc(sizeof(WorkItem) * items.size()));
     PHC_HIP(hipMemcpy(h->items.ptr, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice));
@@ -355,8 +398,8 @@ phc_status phc_batch_create(const phc_batch_params *p, phc_batch **out)
     if (p->arith != PHC_FD_ARITH_REF && p->arith != PHC_FD_ARITH_FP32)
         return fail(PHC_ERR_INVALID, "unknown arithmetic mode %d", p->arith);
     if (p->nsamples < 1) return fail(PHC_ERR_INVALID, "nsamples must be >= 1 (got %d)", p->nsamples);
-    if (p->chunk_i < 0 || p->blocks_per_cu < 0 || p->blocks_per_cu > 8)
-        return fail(PHC_ERR_INVALID, "bad chunk_i or blocks_per_cu");
+    if (p->chunk_i < -4 || p->blocks_per_cu < 0 || p->blocks_per_cu > 8)
+        return fail(PHC_ERR_INVALID, "bad chunk_i (planes per item, or -1..-4) or blocks_per_cu");
     if (p->k_remainder < -1 || p->k_remainder > 1)
         return fail(PHC_ERR_INVALID, "k_remainder must be -1, 0 or 1 (got %d)", p->k_remainder);
     const int ty = p->block_waves ? p->block_waves : 16;
@@ -637,7 +680,7 @@ phc_status phc_batch_autotune(phc_batch *h, int steps_per_trial, int *tile_j, in
     PHC_HIP(hipEventCreate(&e0));
     PHC_HIP(hipEventCreate(&e1));
     static const int shapes[][2] = {{16, 1}, {8, 2}, {4, 2}, {4, 4}};
-    static const int chunks[] = {32, 64, 128, 256, 512};
+    static const int chunks[] = {32, 64, 128, 256, 512, -1, -2, -3, -4};
     static const int bpcs[] = {1, 2, 4};
     double best = 1e30;
     int bty = h->ty, bnj = h->nj, bci = h->chunk_i, bbp = h->blocks_per_cu, bku = h->krem_eff;

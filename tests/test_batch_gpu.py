@@ -42,14 +42,15 @@ def oracle_of(oracle, s):
     return P
 
 
-@pytest.mark.parametrize("krem", [0, -1, 1])
-def test_ref_mode_bit_exact_with_oracle(gpu, oracle, krem):
+@pytest.mark.parametrize("krem,chunk", [(0, 16), (-1, 16), (1, 16), (0, -1), (1, -3), (-1, -4)])
+def test_ref_mode_bit_exact_with_oracle(gpu, oracle, krem, chunk):
     # krem: shots whose last k tile is partial (nz 70, 130; nz 35 has no full tile)
     # update it point by point (0: when it holds <= 32 points) or march it
-    banner("ref mode vs oracle, %d shots, k remainder mode %d" % (len(DIMS), krem))
+    # chunk -m: every shot's per-XCD schedule, XCD x's entries in queue x
+    banner("ref mode vs oracle, %d shots, k remainder mode %d, chunk %d" % (len(DIMS), krem, chunk))
     setups = make_shots(gpu)
     rec = receivers(setups)
-    b = gpu.FdBatch(setups, arith=gpu.ARITH_REF, chunk_i=16, k_remainder=krem)
+    b = gpu.FdBatch(setups, arith=gpu.ARITH_REF, chunk_i=chunk, k_remainder=krem)
     b.set_receivers(rec)
     n = 45
     src, rv = b.run(n)
@@ -63,7 +64,8 @@ def test_ref_mode_bit_exact_with_oracle(gpu, oracle, krem):
 
 
 @pytest.mark.parametrize("tile_j,waves,chunk_i,bpc", [
-    (16, 16, 64, 1), (16, 8, 5, 2), (8, 4, 32, 4), (16, 4, 7, 1), (16, 16, 1000, 3)])
+    (16, 16, 64, 1), (16, 8, 5, 2), (8, 4, 32, 4), (16, 4, 7, 1), (16, 16, 1000, 3),
+    (16, 16, -1, 1), (16, 8, -2, 2), (8, 4, -3, 4), (16, 4, -4, 1)])
 def test_fp32_bit_exact_with_single_grid(gpu, tile_j, waves, chunk_i, bpc):
     banner("fp32 shape tile_j=%d waves=%d chunk_i=%d blocks/CU=%d" % (tile_j, waves, chunk_i, bpc))
     setups = make_shots(gpu, sponge=6, nsamples=40)
