@@ -26,6 +26,11 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + INCLUDE, "-I" + CSRC]
 
+# Per-source additions.  mlp.hip: MFMA results in VGPRs (the correction
+# pass's network reads each result with VALU, which AGPR results would first
+# have to copy out, and takes its biases as the accumulator operand).
+FILE_FLAGS = {"mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
 # roctx ranges around the host API calls (rocprofv3 --marker-trace)
 ROCTX_LIBS = ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx"]
 
@@ -47,7 +52,7 @@ def build_libphc(jobs=8):
 
     def compile_one(pair):
         src, obj = pair
-        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        _run([HIPCC] + HIP_FLAGS + FILE_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj])
 
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(compile_one, zip(srcs, objs)))
@@ -114,7 +119,7 @@ def sanitize_host(jobs=8):
 
     def compile_one(pair):
         src, obj = pair
-        _run([HIPCC] + flags + ["-c", os.path.join(CSRC, src), "-o", obj])
+        _run([HIPCC] + flags + FILE_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj])
 
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(compile_one, zip(srcs, objs)))

@@ -483,20 +483,28 @@ __global__ void lb_to_bf16_kernel(const float *theta, uint16_t *w, int n)
 }
 
 // ------------------------------------------------------------ inference
-// MFMA 32x32x8 (bf16 in, FP32 accumulate), one wavefront per 32 points of one
-// face.  The network is evaluated transposed, H^T = W x^T, so that lane l
-// always holds point l%32: operand A = a weight matrix (lane l: row l%32,
-// k = 4*(l/32)+0..3 of the k-chunk), operand B = activations (lane l: column
-// l%32, the same k), result (lane l: column l%32, rows 8*(v/4) + 4*(l/32) +
-// v%4 for v = 0..15).  The rows a lane receives for chunk c of the next layer
-// are exactly the result rows it holds with v/4 = c, so activations never
-// leave the lane.  Activations (inputs and hidden) are split into a bf16 high
-// part and a bf16 remainder, two products each, so they carry ~16 bits.
-typedef short lb_bf16x4 __attribute__((ext_vector_type(4)));
+// MFMA 32x32x16 (bf16 in, FP32 accumulate; twice the products per cycle of
+// 32x32x8 on gfx950), one wavefront per 32 points of one face.  The network
+// is evaluated transposed, H^T = W x^T, so that lane l always holds point
+// l%32: operand A = a weight matrix (lane l: row l%32, k = 8*(l/32)+0..7 of
+// the 16-wide k-chunk), operand B = activations (lane l: column l%32, the
+// same k), result (lane l: column l%32, rows 8*(v/4) + 4*(l/32) + v%4 for
+// v = 0..15).
+//   Layer 1: chunk 0 holds the 9 field inputs' first 8 (half-wave 0: the
+// current level's, half-wave 1: the previous level's), chunk 1 the ninth and
+// (half-wave 0) the velocity feature; the weight columns are placed to match,
+// so each lane feeds only its own level's inputs.
+//   Layer 2: the 16 rows a lane holds are hidden units; chunk c takes the
+// lane's results v = 8c..8c+7, and W2's columns are loaded in that order, so
+// activations never leave the lane.
+//   Biases enter as the accumulator operand of the first product of a layer.
+// Activations (inputs and hidden) are split into a bf16 high part and a bf16
+// remainder, two products each, so they carry ~16 bits.
+typedef __bf16 lb_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float lb_f32x16 __attribute__((ext_vector_type(16)));
-
 typedef __bf16 lb_bf16x2_t __attribute__((ext_vector_type(2)));
 typedef float lb_f32x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t lb_u32x4 __attribute__((ext_vector_type(4)));
 
 // Two floats to packed bf16, round to nearest even (one v_cvt_pk_bf16_f32;
 // the same values as f2bf for finite inputs), element 0 in the low half.
@@ -507,23 +515,33 @@ __device__ inline uint32_t f2bf2(float a, float b)
 }
 
 // a = hi + lo with hi = bf16(a) and lo = bf16(a - hi): ~16 bits of a.
-__device__ inline void lb_split(const float *a, lb_bf16x4 &hi, lb_bf16x4 &lo)
+__device__ inline void lb_split8(const float a[8], lb_bf16x8 &hi, lb_bf16x8 &lo)
 {
-    const uint32_t h01 = f2bf2(a[0], a[1]), h23 = f2bf2(a[2], a[3]);
-    const uint32_t l01 = f2bf2(a[0] - __uint_as_float(h01 << 16), a[1] - __uint_as_float(h01 & 0xffff0000u));
-    const uint32_t l23 = f2bf2(a[2] - __uint_as_float(h23 << 16), a[3] - __uint_as_float(h23 & 0xffff0000u));
-    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-    const u32x2 hv = {h01, h23}, lv = {l01, l23};
-    hi = __builtin_bit_cast(lb_bf16x4, hv);
-    lo = __builtin_bit_cast(lb_bf16x4, lv);
+    lb_u32x4 h, l;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        h[j] = f2bf2(a[2 * j], a[2 * j + 1]);
+        l[j] = f2bf2(a[2 * j] - __uint_as_float(h[j] << 16), a[2 * j + 1] - __uint_as_float(h[j] & 0xffff0000u));
+    }
+    hi = __builtin_bit_cast(lb_bf16x8, h);
+    lo = __builtin_bit_cast(lb_bf16x8, l);
 }
 
-// One face's weights in the registers of a wavefront, in MFMA operand
-// layout (A operands of layers 1 and 2) or per result row (b2, w3); the
-// per-depth first-layer biases stay in LDS.
+// Input of layer-1 slot (chunk c, half-wave h, element e), or -1 (zero weight).
+__host__ __device__ constexpr int lb_in_slot(int c, int h, int e)
+{
+    return c == 0 ? (h == 0 ? e : 9 + e) : (e == 0 ? (h == 0 ? 8 : 17) : (e == 1 && h == 0 ? LB_NIN - 1 : -1));
+}
+
+// Hidden unit held by result register v of half-wave h.
+__host__ __device__ constexpr int lb_row(int v, int h) { return 8 * (v >> 2) + 4 * h + (v & 3); }
+
+// One face's weights in the registers of a wavefront, in MFMA operand layout
+// (A operands of layers 1 and 2) or per result register (b2, w3).
 struct LbWaveWeights {
-    lb_bf16x4 a1[3], a2[4];
-    float b2[16], w3[16];
+    lb_bf16x8 a1[2], a2[2];
+    lb_f32x16 b2;
+    float w3[16];
     float b3;
 };
 
@@ -532,92 +550,108 @@ __device__ inline void lb_load_wave_weights(const uint16_t *__restrict__ w, LbWa
     const int lane = threadIdx.x & 63;
     const int row = lane & 31, half = lane >> 5;
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 2; ++c)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int k = 8 * c + 4 * half + e;
-            W.a1[c][e] = (short)(k < LB_NIN ? w[LB_OFF_W1 + row * LB_NIN + k] : (uint16_t)0);
+        for (int e = 0; e < 8; ++e) {
+            const int i0 = lb_in_slot(c, 0, e), i1 = lb_in_slot(c, 1, e);
+            const int in = half ? i1 : i0;
+            W.a1[c][e] = __builtin_bit_cast(__bf16, in >= 0 ? w[LB_OFF_W1 + row * LB_NIN + max(in, 0)] : (uint16_t)0);
+            // layer-2 column: the hidden unit of the lane's result 8c + e
+            W.a2[c][e] = __builtin_bit_cast(__bf16, w[LB_OFF_W2 + row * LB_H + lb_row(8 * c + e, half)]);
         }
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) W.a2[c][e] = (short)w[LB_OFF_W2 + row * LB_H + 8 * c + 4 * half + e];
-#pragma unroll
     for (int v = 0; v < 16; ++v) {
-        const int i = 8 * (v >> 2) + 4 * half + (v & 3);
-        W.b2[v] = bf2f(w[LB_OFF_B2 + i]);
-        W.w3[v] = bf2f(w[LB_OFF_W3 + i]);
+        W.b2[v] = bf2f(w[LB_OFF_B2 + lb_row(v, half)]);
+        W.w3[v] = bf2f(w[LB_OFF_W3 + lb_row(v, half)]);
     }
     W.b3 = bf2f(w[LB_OFF_B3]);
 }
 
-// y for the wave's 32 points.  xk: this lane's 12 inputs (k = 8c + 4*half + e,
-// c = 0..2, e = 0..3); d: depth of this lane's point; b1: the face's
-// per-depth first-layer biases (LB_W x LB_H floats, LDS).  Returns y (valid
-// in both halves).
-__device__ inline float lb_mlp_wave(const LbWaveWeights &W, const float *b1, const float xk[12], int d)
+// max(x, 0) and max(a, b) for finite values as one v_med3_f32 (fmaxf on a
+// product's result adds a quieting v_max first).
+__device__ inline float lb_relu(float x) { return __builtin_amdgcn_fmed3f(x, 0.f, 3.0e38f); }
+__device__ inline float lb_fmax(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, 3.0e38f); }
+
+// Sum of a value over the lane's point in both half-waves (lane p and
+// p + 32), the lower half's value first: the same float in every lane.
+__device__ inline float lb_pair_sum(float x)
 {
-    const int half = (threadIdx.x & 63) >> 5;
-    lb_f32x16 acc;
-#pragma unroll
-    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-    // layer 1: H1^T[32 x 32] = W1[32 x 24] x^T[24 x 32]
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        lb_bf16x4 bh, bl;
-        lb_split(xk + 4 * c, bh, bl);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a1[c], bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a1[c], bl, acc, 0, 0, 0);
-    }
-    float h1[16];
-#pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        const int i = 8 * (v >> 2) + 4 * half + (v & 3);
-        h1[v] = fmaxf(acc[v] + b1[d * LB_H + i], 0.f);
-        acc[v] = 0.f;
-    }
-    // layer 2: H2^T = W2 H1^T
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        lb_bf16x4 bh, bl;
-        lb_split(h1 + 4 * c, bh, bl);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a2[c], bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W.a2[c], bl, acc, 0, 0, 0);
-    }
-    float y = 0.f;
-#pragma unroll
-    for (int v = 0; v < 16; ++v) y = fmaf(W.w3[v], fmaxf(acc[v] + W.b2[v], 0.f), y);
-    y += __shfl_xor(y, 32);
-    return y + W.b3;
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
 
-// this lane's 12 of the 24 padded inputs: k = 8c + e (lanes 0-31) or
-// 8c + 4 + e (lanes 32-63)
-__device__ inline void lb_lane_inputs(const float x[LB_NIN], float xk[12])
+__device__ inline float lb_pair_max(float x)
 {
-    const bool upper = (threadIdx.x & 32) != 0;
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return lb_fmax(__uint_as_float(r[0]), __uint_as_float(r[1]));
+}
+
+// y for the wave's 32 points.  F: this lane's 9 scaled field inputs (its
+// half-wave's level), xv: the velocity feature, b1: the layer-1 biases of
+// this lane's 16 result rows (the point's depth).  Returns y (every lane).
+__device__ inline float lb_mlp_wave(const LbWaveWeights &W, const float F[9], float xv, const lb_f32x16 &b1)
+{
+    lb_bf16x8 bh, bl;
+    lb_split8(F, bh, bl);
+    lb_f32x16 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.a1[0], bh, b1, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.a1[0], bl, acc, 0, 0, 0);
+    const float t[8] = {F[8], xv, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    lb_split8(t, bh, bl);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.a1[1], bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.a1[1], bl, acc, 0, 0, 0);
+    float h1[16];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int v = 0; v < 16; ++v) h1[v] = lb_relu(acc[v]);
+    lb_split8(h1, bh, bl);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.a2[0], bh, W.b2, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.a2[0], bl, acc, 0, 0, 0);
+    lb_split8(h1 + 8, bh, bl);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.a2[1], bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.a2[1], bl, acc, 0, 0, 0);
+    float y = 0.f;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int k0 = 8 * c + e, k1 = 8 * c + 4 + e;
-            const float v0 = k0 < LB_NIN ? x[k0] : 0.f;
-            const float v1 = k1 < LB_NIN ? x[k1] : 0.f;
-            xk[4 * c + e] = upper ? v1 : v0;
-        }
+    for (int v = 0; v < 16; ++v) y = fmaf(W.w3[v], lb_relu(acc[v]), y);
+    return lb_pair_sum(y) + W.b3;
+}
+
+// The scaling of lb_finish_inputs on a lane's 9 field inputs, already
+// offset by h and zero beyond the region's edge: s over both half-waves
+// (both levels), inputs times 1/s (approximate reciprocal).  Returns s.
+__device__ inline float lb_scale_lane_inputs(float F[9])
+{
+    float m = 0.f;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m = lb_fmax(m, fabsf(F[q]));
+    const float s = lb_pair_max(m);
+    const float inv = s > LB_S_TINY ? __builtin_amdgcn_rcpf(s) : 0.f;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) F[q] *= inv;
+    return s;
+}
+
+// Layer-1 biases of this lane's result rows for depth d (LDS, LB_W x LB_H).
+__device__ inline lb_f32x16 lb_bias1(const float *b1s, int d)
+{
+    const int half = (threadIdx.x & 63) >> 5;
+    lb_f32x16 b;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) b[v] = b1s[d * LB_H + lb_row(v, half)];
+    return b;
 }
 
 // Correction pass, staged: a block takes a tile of one face's layer, the
-// LB_W depths x LB_TU rows (u) x LB_TV columns (v), and first copies the
-// 2 LB_R + LB_W planes along the normal that the tile's normal stencils read
-// (halo included) of both levels into LDS, with loads along the grid's
-// contiguous axis.  Wavefront w then evaluates depth w: LB_TU groups of 32
-// consecutive v, each lane's 18 field inputs read from LDS.  The inputs,
-// network and stores are those of lb_correct_kernel, so the corrections are
-// the same floats.
+// LB_W depths x LB_TU rows (u) x LB_TV columns (v).  Thread (u, v) copies
+// the LB_W + LB_R planes along the normal that lie inside the grid (of the
+// 2 LB_R + LB_W that the tile's normal stencils reach) of both levels into
+// LDS, offset by h (the planes beyond the grid's edge are zero: those
+// inputs are), and vp c1 and its square root at the layer points.  LDS plane
+// P is inward index P - R from the face (P < R: beyond the edge), so input m
+// of depth d is plane d + m on either side.  Wavefront w then evaluates
+// depth w: LB_TU groups of 32 consecutive v.
 constexpr int LB_TU = 8, LB_TV = 32;
 constexpr int LB_NPL = 2 * LB_R + LB_W;  // normal planes of a tile (12)
-static_assert(LB_TV == 32 && LB_W == 4, "a group is 32 consecutive v; one wavefront per depth");
+constexpr int LB_NIG = LB_W + LB_R;      // of which inside the grid (8)
+static_assert(LB_TV == 32 && LB_W == 4 && LB_TU * LB_TV == 256, "one thread per (u, v); one wavefront per depth");
 
 struct LbCorrectTileArgs {
     const float *pn, *pnm1, *vp;
@@ -634,7 +668,8 @@ __global__ void __launch_bounds__(256) lb_correct_tile_kernel(LbCorrectTileArgs 
 {
     __shared__ float b1s[LB_W * LB_H];
     __shared__ float S[2][LB_NPL][LB_TU][LB_TV];
-    __shared__ float Svp[LB_W][LB_TU][LB_TV];  // vp at the tile's layer points [depth][u][v]
+    __shared__ float Svc[LB_W][LB_TU][LB_TV];  // vp c1 at the tile's layer points [depth][u][v]
+    __shared__ float Svs[LB_W][LB_TU][LB_TV];  // sqrt(vp c1), the velocity feature
     int f = 0;
     while (f + 1 < LB_FACES && (int)blockIdx.x >= A.blk0[f + 1]) ++f;
     const int n[3] = {A.L.nx, A.L.ny, A.L.nz};
@@ -643,74 +678,69 @@ __global__ void __launch_bounds__(256) lb_correct_tile_kernel(LbCorrectTileArgs 
     const int tb = (int)blockIdx.x - A.blk0[f];
     const int ntv = (g.nv + LB_TV - 1) / LB_TV;
     const int u0 = (tb / ntv) * LB_TU, v0 = (tb % ntv) * LB_TV;
-    // normal index of plane 0 of the box: -R at the low face, n - W - R at
-    // the high one (depths n-1-d, offsets -R..R)
-    const int nrm0 = g.side ? n[a] - LB_W - LB_R : -LB_R;
     const uint16_t *w = A.w + f * LB_NP;
-    for (int t = threadIdx.x; t < LB_W * LB_H; t += blockDim.x) b1s[t] = bf2f(w[LB_OFF_B1 + t]);
-    // stage the box: for the i- and j-faces v (= k) runs fastest, for the
-    // k-faces the normal (= k) does
-    constexpr int NBOX = LB_NPL * LB_TU * LB_TV;
-    for (int t = threadIdx.x; t < NBOX; t += blockDim.x) {
-        int pl, uu, vv;
-        if (a == 2) {
-            pl = t % LB_NPL;
-            const int r = t / LB_NPL;
-            vv = r % LB_TV;
-            uu = r / LB_TV;
-        } else {
-            vv = t % LB_TV;
-            const int r = t / LB_TV;
-            uu = r % LB_TU;
-            pl = r / LB_TU;
-        }
-        const int u = min(u0 + uu, g.nu - 1), v = min(v0 + vv, g.nv - 1);
-        int c[3];
-        c[a] = nrm0 + pl;
-        c[ua] = u;
-        c[va] = v;
-        const int64_t o = fd_off(A.L, c[0], c[1], c[2]);
-        S[0][pl][uu][vv] = A.pn[o];
-        S[1][pl][uu][vv] = A.pnm1[o];
+    static_assert(LB_W * LB_H <= 256, "one layer-1 bias per thread");
+    const uint16_t b1raw = w[LB_OFF_B1 + min((int)threadIdx.x, LB_W * LB_H - 1)];
+    // in-grid plane q (normal index q, or n - W - R + q at the high face)
+    // is LDS plane R + q (low face) or W + 2R - 1 - q (high face)
+    const int tu = (int)threadIdx.x / LB_TV, tv = (int)threadIdx.x % LB_TV;
+    int c[3];
+    c[a] = g.side ? n[a] - LB_NIG : 0;
+    c[ua] = min(u0 + tu, g.nu - 1);
+    c[va] = min(v0 + tv, g.nv - 1);
+    const int64_t o0 = fd_off(A.L, c[0], c[1], c[2]);
+    const int64_t step = a == 0 ? A.L.pitch_j : (a == 1 ? A.L.pitch_k : 1);
+    // every load is issued before the first is waited for
+    float r0[LB_NIG], r1[LB_NIG], rv[LB_W];
+#pragma unroll
+    for (int q = 0; q < LB_NIG; ++q) {
+        r0[q] = A.pn[o0 + q * step];
+        r1[q] = A.pnm1[o0 + q * step];
     }
-    for (int t = threadIdx.x; t < LB_W * LB_TU * LB_TV; t += blockDim.x) {
-        const int vv = t % LB_TV, uu = (t / LB_TV) % LB_TU, dd = t / (LB_TV * LB_TU);
-        int c[3];
-        c[a] = g.side ? n[a] - 1 - dd : dd;
-        c[ua] = min(u0 + uu, g.nu - 1);
-        c[va] = min(v0 + vv, g.nv - 1);
-        Svp[dd][uu][vv] = A.vp[fd_off(A.L, c[0], c[1], c[2])];
+#pragma unroll
+    for (int dd = 0; dd < LB_W; ++dd) rv[dd] = A.vp[o0 + (g.side ? LB_NIG - 1 - dd : dd) * step];
+#pragma unroll
+    for (int q = 0; q < LB_NIG; ++q) {
+        const int P = g.side ? LB_NPL - 1 - q : LB_R + q;
+        S[0][P][tu][tv] = r0[q] - A.h;
+        S[1][P][tu][tv] = r1[q] - A.h;
     }
+#pragma unroll
+    for (int P = 0; P < LB_R; ++P) {  // beyond the edge
+        S[0][P][tu][tv] = 0.f;
+        S[1][P][tu][tv] = 0.f;
+    }
+#pragma unroll
+    for (int dd = 0; dd < LB_W; ++dd) {
+        const float vc = rv[dd] * A.c1;
+        Svc[dd][tu][tv] = vc;
+        Svs[dd][tu][tv] = sqrtf(vc);
+    }
+    if (threadIdx.x < LB_W * LB_H) b1s[threadIdx.x] = bf2f(b1raw);
     __syncthreads();
     LbWaveWeights W;
     lb_load_wave_weights(w, W);
     const int d = threadIdx.x >> 6;  // this wavefront's depth
     const int p = threadIdx.x & 31;  // this lane's v in the group (both half-waves)
+    const int half = (threadIdx.x >> 5) & 1;
     const int v = v0 + p;
     const bool vok = v < g.nv;
+    const float ymax = A.ymax[d];
 #pragma unroll 1
     for (int uu = 0; uu < LB_TU; ++uu) {
         const int u = u0 + uu;
         if (u >= g.nu) break;  // uniform per wavefront
-        float x[LB_NIN], xk[12];
+        float F[9];
 #pragma unroll
-        for (int m = 0; m < 9; ++m) {
-            // offset m - 4 along the inward normal
-            const int pl = g.side ? LB_R + LB_W - 1 - d + LB_R - m : d + m;
-            x[m] = S[0][pl][uu][p];
-            x[9 + m] = S[1][pl][uu][p];
-        }
-        x[18] = Svp[d][uu][p];
-        const float vraw = x[18];
-        const float s = lb_finish_inputs(d, A.h, A.c1, x);
-        lb_lane_inputs(x, xk);
-        const float y = fminf(fmaxf(lb_mlp_wave(W, b1s, xk, d), -A.ymax[d]), A.ymax[d]);
-        if ((threadIdx.x & 32) == 0 && vok)
-            A.corr[A.off[f] + ((int64_t)d * g.nu + u) * g.nv + v] = vraw * A.c1 * s * y;
+        for (int m = 0; m < 9; ++m) F[m] = S[half][d + m][uu][p];  // offset m - 4 along the inward normal
+        const float s = lb_scale_lane_inputs(F);
+        const float y = __builtin_amdgcn_fmed3f(lb_mlp_wave(W, F, Svs[d][uu][p], lb_bias1(b1s, d)), -ymax, ymax);
+        if (half == 0 && vok) A.corr[A.off[f] + ((int64_t)d * g.nu + u) * g.nv + v] = Svc[d][uu][p] * s * y;
     }
 }
 
-// MFMA inference on given inputs (testing): y[n] for samples of one face.
+// MFMA inference on given inputs (testing): y[n] for samples of one face,
+// inputs as lb_inputs forms them.
 __global__ void __launch_bounds__(256) lb_infer_kernel(const uint16_t *w, const float *samples, int64_t n, float *y)
 {
     __shared__ float b1s[LB_W * LB_H];
@@ -722,12 +752,13 @@ __global__ void __launch_bounds__(256) lb_infer_kernel(const uint16_t *w, const 
     lb_load_wave_weights(w, W);
     const int64_t idx = wv * 32 + (threadIdx.x & 31);
     const int64_t ci = idx < n ? idx : n - 1;
-    float x[LB_NIN], xk[12];
-    for (int m = 0; m < LB_NIN; ++m) x[m] = samples[ci * LB_NSTRIDE + m];
+    const int half = (threadIdx.x >> 5) & 1;
+    float F[9];
+    for (int m = 0; m < 9; ++m) F[m] = samples[ci * LB_NSTRIDE + 9 * half + m];
+    const float xv = samples[ci * LB_NSTRIDE + LB_NIN - 1];
     const int d = min(max((int)samples[ci * LB_NSTRIDE + LB_NIN + 1], 0), LB_W - 1);
-    lb_lane_inputs(x, xk);
-    const float r = lb_mlp_wave(W, b1s, xk, d);
-    if ((threadIdx.x & 32) == 0 && idx < n) y[idx] = r;
+    const float r = lb_mlp_wave(W, F, xv, lb_bias1(b1s, d));
+    if (half == 0 && idx < n) y[idx] = r;
 }
 
 // Apply pass: one thread per point of the union of the layers; adds the

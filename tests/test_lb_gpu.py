@@ -225,6 +225,14 @@ def test_learned_steps_full_size_264(gpu, oracle):
     rel = np.linalg.norm((g - r).astype(np.float64)) / np.linalg.norm(r.astype(np.float64))
     print("[lb-gpu] 264^3 learned steps rel L2 %.3e" % rel, flush=True)
     assert rel < 1e-4, rel
+    # the fused, graph-captured run gives the per-step surface bit for bit
+    sol2 = gpu.FdSolver(s, arith=gpu.ARITH_REF, init_value=0.0)
+    sol2.set_field(sol2.i2, f1)
+    sol2.set_field(sol2.i1, f0)
+    sol2.set_learned_boundary(lb, halo=0.0)
+    sol2.run(0, 3)
+    assert np.array_equal(sol2.current(), g)
+    assert np.array_equal(sol2.previous(), sol.previous())
 
 
 def test_training_reduces_the_loss(gpu):
