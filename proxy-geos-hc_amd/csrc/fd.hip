@@ -46,7 +46,7 @@ __device__ __forceinline__ void fd_block_range(const FdStepArgs &a, int &kt, int
         i0 = e.y;
         i1 = e.z;
     } else {
-        const int b = xcd_block(blockIdx.x, a.nb_main ? a.nb_main : gridDim.x);
+        const int b = xcd_block(blockIdx.x, a.nb_main ? a.nb_main : gridDim.x - (a.rec_n > 0 ? 1 : 0));
         const int nt = a.tiles_k * a.tiles_j, ic = b / nt;
         w = b - ic * nt;
         i0 = a.ilo + ic * a.chunk_i;
@@ -56,12 +56,25 @@ __device__ __forceinline__ void fd_block_range(const FdStepArgs &a, int &kt, int
     jt = w / a.tiles_k;
 }
 
+// The extra block's receiver sampling (FdStepArgs::rec_n).
+__device__ __forceinline__ void fd_sample_prev(const FdStepArgs &a)
+{
+    for (int r = (int)(threadIdx.y * blockDim.x + threadIdx.x); r < a.rec_n; r += (int)(blockDim.x * blockDim.y)) {
+        const int64_t o = a.rec_off[r];
+        a.rec_out[(int64_t)r * a.rec_stride + a.rec_step] = (o == a.src_off) ? a.rec_src[a.rec_step] : a.pn[o];
+    }
+}
+
 template <int R, int TY, int NJ, int ARITH, typename IT>
 __global__ void __launch_bounds__(FD_TK *TY)
 fd_step_kernel(FdStepArgs a)
 {
     using SH = FdTileShape<R, TY, NJ>;
     __shared__ float tile[2][SH::LH][SH::LW];
+    if (a.rec_n > 0 && blockIdx.x == gridDim.x - 1) {
+        fd_sample_prev(a);
+        return;
+    }
     if (a.nb_main && (int)blockIdx.x >= a.nb_main) {
         fd_rem_points<R, ARITH, IT>(a, a, (int64_t)(blockIdx.x - a.nb_main) * (FD_TK * TY), FD_TK * TY);
         return;
@@ -81,6 +94,10 @@ fd_step_pml_kernel(FdStepArgs a)
 {
     using SH = FdTileShape<R, TY, NJ>;
     __shared__ float tile[2][SH::LH][SH::LW];
+    if (a.rec_n > 0 && blockIdx.x == gridDim.x - 1) {
+        fd_sample_prev(a);
+        return;
+    }
     if (a.nb_main && (int)blockIdx.x >= a.nb_main) {
         fd_rem_points<R, ARITH, IT, true>(a, a, (int64_t)(blockIdx.x - a.nb_main) * (FD_TK * TY), FD_TK * TY);
         return;
@@ -470,6 +487,7 @@ static phc_status fd_launch_step(phc_fd *h, FdStepArgs a, bool damp)
     a.damp_prev = damp ? 1 : 0;
     int nb = 0;
     PHC_TRY(fd_plan_launch(h, a, &nb));
+    if (a.rec_n > 0) ++nb;  // the sampling block
     hipLaunchKernelGGL(fn, dim3(nb), dim3(FD_TK, h->ty), 0, h->stream, a);
     PHC_HIP(hipGetLastError());
     return PHC_OK;
@@ -705,16 +723,24 @@ static phc_status fd_enqueue_run(phc_fd *h, int it0, int nsteps, int c0, bool da
             a.inj = 1;
             a.rhs_next = h->hrhs[it + 1];
         }
+        if (h->nrec > 0 && s > 0) {  // the previous step's receivers, by this launch
+            a.rec_off = h->recoff.as<int64_t>();
+            a.rec_out = h->rec.as<float>();
+            a.rec_src = h->trace.as<float>();
+            a.rec_n = h->nrec;
+            a.rec_stride = nsteps;
+            a.rec_step = s - 1;
+        }
         if (ev) PHC_HIP(hipEventRecord(ev[2 * s], h->stream));
         PHC_TRY(fd_launch_full_step(h, a, s == 0 ? damp0 : (h->sw > 0), ca, cb));
         if (ev) PHC_HIP(hipEventRecord(ev[2 * s + 1], h->stream));
-        if (h->nrec > 0) {
-            hipLaunchKernelGGL(fd_sample_kernel, dim3((h->nrec + 255) / 256), dim3(256), 0,
-                               h->stream, h->lev[ca].as<float>(), h->recoff.as<int64_t>(), h->nrec,
-                               h->rec.as<float>(), nsteps, s, a.src_off, h->trace.as<float>());
-            PHC_HIP(hipGetLastError());
-        }
         c = ca;
+    }
+    if (h->nrec > 0) {  // the last step's receivers
+        hipLaunchKernelGGL(fd_sample_kernel, dim3((h->nrec + 255) / 256), dim3(256), 0, h->stream,
+                           h->lev[c].as<float>(), h->recoff.as<int64_t>(), h->nrec, h->rec.as<float>(), nsteps,
+                           nsteps - 1, fd_off(h->L, h->xs, h->ys, h->zs), h->trace.as<float>());
+        PHC_HIP(hipGetLastError());
     }
     return PHC_OK;
 }

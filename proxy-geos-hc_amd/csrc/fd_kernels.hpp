@@ -84,6 +84,15 @@ struct FdStepArgs {
     // sched[b].x (k tile + tiles_k * j tile) over planes [sched[b].y,
     // sched[b].z); an empty range does nothing
     const int4 *sched;
+    // receiver sampling of the level this launch reads as current (the
+    // previous step's result), by one extra block after the others:
+    // rec_out[r * rec_stride + rec_step] = pn[rec_off[r]], or
+    // rec_src[rec_step] (the value recorded before the fused injection) for a
+    // receiver on the source point (0 receivers: no extra block)
+    const int64_t *rec_off;
+    float *rec_out;
+    const float *rec_src;
+    int rec_n, rec_stride, rec_step;
 };
 
 } // namespace phc

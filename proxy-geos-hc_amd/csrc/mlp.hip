@@ -639,15 +639,16 @@ __device__ inline lb_f32x16 lb_bias1(const float *b1s, int d)
     return b;
 }
 
-// Correction pass, staged: a block takes a tile of one face's layer, the
-// LB_W depths x LB_TU rows (u) x LB_TV columns (v).  Thread (u, v) copies
+// Correction pass, staged: a block takes 256 consecutive (u, v) points of
+// one face (flat index u nv + v, so no tile is partly empty but the face's
+// last) at the LB_W depths.  Thread (u, v) copies
 // the LB_W + LB_R planes along the normal that lie inside the grid (of the
 // 2 LB_R + LB_W that the tile's normal stencils reach) of both levels into
 // LDS, offset by h (the planes beyond the grid's edge are zero: those
 // inputs are), and vp c1 and its square root at the layer points.  LDS plane
 // P is inward index P - R from the face (P < R: beyond the edge), so input m
 // of depth d is plane d + m on either side.  Wavefront w then evaluates
-// depth w: LB_TU groups of 32 consecutive v.
+// depth w: LB_TU groups of 32 consecutive points.
 constexpr int LB_TU = 8, LB_TV = 32;
 constexpr int LB_NPL = 2 * LB_R + LB_W;  // normal planes of a tile (12)
 constexpr int LB_NIG = LB_W + LB_R;      // of which inside the grid (8)
@@ -675,19 +676,19 @@ __global__ void __launch_bounds__(256) lb_correct_tile_kernel(LbCorrectTileArgs 
     const int n[3] = {A.L.nx, A.L.ny, A.L.nz};
     const LbFaceGeom g = lb_face(n, f);
     const int a = g.axis, ua = (a == 0) ? 1 : 0, va = (a == 2) ? 1 : 2;
-    const int tb = (int)blockIdx.x - A.blk0[f];
-    const int ntv = (g.nv + LB_TV - 1) / LB_TV;
-    const int u0 = (tb / ntv) * LB_TU, v0 = (tb % ntv) * LB_TV;
+    const int npts = g.nu * g.nv;
+    const int flat0 = ((int)blockIdx.x - A.blk0[f]) * (LB_TU * LB_TV);
     const uint16_t *w = A.w + f * LB_NP;
     static_assert(LB_W * LB_H <= 256, "one layer-1 bias per thread");
     const uint16_t b1raw = w[LB_OFF_B1 + min((int)threadIdx.x, LB_W * LB_H - 1)];
     // in-grid plane q (normal index q, or n - W - R + q at the high face)
     // is LDS plane R + q (low face) or W + 2R - 1 - q (high face)
     const int tu = (int)threadIdx.x / LB_TV, tv = (int)threadIdx.x % LB_TV;
+    const int flat = min(flat0 + (int)threadIdx.x, npts - 1);
     int c[3];
     c[a] = g.side ? n[a] - LB_NIG : 0;
-    c[ua] = min(u0 + tu, g.nu - 1);
-    c[va] = min(v0 + tv, g.nv - 1);
+    c[ua] = flat / g.nv;
+    c[va] = flat - c[ua] * g.nv;
     const int64_t o0 = fd_off(A.L, c[0], c[1], c[2]);
     const int64_t step = a == 0 ? A.L.pitch_j : (a == 1 ? A.L.pitch_k : 1);
     // every load is issued before the first is waited for
@@ -699,6 +700,8 @@ __global__ void __launch_bounds__(256) lb_correct_tile_kernel(LbCorrectTileArgs 
     }
 #pragma unroll
     for (int dd = 0; dd < LB_W; ++dd) rv[dd] = A.vp[o0 + (g.side ? LB_NIG - 1 - dd : dd) * step];
+    LbWaveWeights W;  // its loads in flight with the staging's
+    lb_load_wave_weights(w, W);
 #pragma unroll
     for (int q = 0; q < LB_NIG; ++q) {
         const int P = g.side ? LB_NPL - 1 - q : LB_R + q;
@@ -718,24 +721,21 @@ __global__ void __launch_bounds__(256) lb_correct_tile_kernel(LbCorrectTileArgs 
     }
     if (threadIdx.x < LB_W * LB_H) b1s[threadIdx.x] = bf2f(b1raw);
     __syncthreads();
-    LbWaveWeights W;
-    lb_load_wave_weights(w, W);
     const int d = threadIdx.x >> 6;  // this wavefront's depth
-    const int p = threadIdx.x & 31;  // this lane's v in the group (both half-waves)
+    const int p = threadIdx.x & 31;  // this lane's point in the group (both half-waves)
     const int half = (threadIdx.x >> 5) & 1;
-    const int v = v0 + p;
-    const bool vok = v < g.nv;
     const float ymax = A.ymax[d];
+    float *corr = A.corr + A.off[f] + (int64_t)d * npts;
 #pragma unroll 1
     for (int uu = 0; uu < LB_TU; ++uu) {
-        const int u = u0 + uu;
-        if (u >= g.nu) break;  // uniform per wavefront
+        const int fg = flat0 + uu * LB_TV;  // the group's first point
+        if (fg >= npts) break;  // uniform per wavefront
         float F[9];
 #pragma unroll
         for (int m = 0; m < 9; ++m) F[m] = S[half][d + m][uu][p];  // offset m - 4 along the inward normal
         const float s = lb_scale_lane_inputs(F);
         const float y = __builtin_amdgcn_fmed3f(lb_mlp_wave(W, F, Svs[d][uu][p], lb_bias1(b1s, d)), -ymax, ymax);
-        if (half == 0 && vok) A.corr[A.off[f] + ((int64_t)d * g.nu + u) * g.nv + v] = Svc[d][uu][p] * s * y;
+        if (half == 0 && fg + p < npts) corr[fg + p] = Svc[d][uu][p] * s * y;
     }
 }
 
@@ -840,7 +840,7 @@ hipError_t lb_enqueue_correct(const LbInfer &lb, const FdLayout &L, const float 
     const int nn[3] = {L.nx, L.ny, L.nz};
     for (int f = 0; f < LB_FACES; ++f) {
         const LbFaceGeom g = lb_face(nn, f);
-        T.blk0[f + 1] = T.blk0[f] + ((g.nu + LB_TU - 1) / LB_TU) * ((g.nv + LB_TV - 1) / LB_TV);
+        T.blk0[f + 1] = T.blk0[f] + (g.nu * g.nv + LB_TU * LB_TV - 1) / (LB_TU * LB_TV);
     }
     hipLaunchKernelGGL(lb_correct_tile_kernel, dim3((unsigned)T.blk0[LB_FACES]), dim3(256), 0, stream, T);
     return hipGetLastError();
