@@ -40,6 +40,7 @@ def banner(msg):
     ((37, 29, 70), 4, 8),       # ragged: partial j and k tiles
     ((24, 30, 131), 4, 20),     # grid narrower than twice the sponge on two axes
     ((33, 65, 64), 4, 0),       # no sponge
+    ((12, 14, 450), 4, 6),      # row pitch padded (a multiple of 2 KiB otherwise)
 ])
 def test_per_step_surface_bit_exact(gpu, oracle, shape, L, sponge):
     nx, ny, nz = shape
