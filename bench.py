@@ -15,7 +15,8 @@ Beside the headline line: "batch" reports the batched multi-shot workload
 run one at a time; "sem_p4" the SEM order-4 workload (401^3 nodes, FP32
 separable kernel, and its FP32-vs-FP64 precision over 2000 steps);
 "sem2d" the SEM2D step on the reference's published 2D mesh and on a large 2D
-mesh; "pml" the FD step with the reference's PML; "sem_element_kernels" the
+mesh; "pml" the FD step with the reference's PML; "fd_sizes" the headline FD
+workload on 264^3 to 448^3 grids; "sem_element_kernels" the
 element kernels (atomic and 8-colour scatter) on the reference's CPU-runnable
 SEM case; "learned_boundary" the learned thin layer trained from scratch on the
 GPU and compared with the 40-point sponge against a 2x-wide FP64 domain.
@@ -289,6 +290,30 @@ def bench_pml(phc, n=512, steps=200):
                          "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": read_traffic(PML_TRAFFIC_FILE)}}
 
 
+FD_SIZES = (264, 336, 384, 401, 448)
+
+
+def bench_fd_sizes(phc, sizes=FD_SIZES, steps=100):
+    """The headline FD workload (radius 4, 20-point sponge, 6 seeded layers,
+    Ricker 10 Hz, CFL 0.4, FP32) on other cubic grids: shape autotuned per
+    size, then `steps` steps with HIP events around each launch.  16
+    algorithmic bytes per point update, as for the headline."""
+    out = {"workload": "FD n^3 radius 4, 20-pt sponge, 6 layers, %d steps, FP32" % steps, "sizes": {}}
+    for n in sizes:
+        setup = phc.FdSetup(nx=n, model=("layered", 6, 20240512), f0=10.0, cfl=0.4, nsamples=steps + 21)
+        sol = phc.FdSolver(setup, arith=phc.ARITH_FP32)
+        shape = sol.autotune(steps_per_trial=10)
+        sol.set_timing(True)
+        sol.run(0, 20)
+        sol.run(20, steps)
+        kms = sol.last_kernel_ms()
+        sol.close()
+        out["sizes"][str(n)] = {"shape": list(shape), "kernel_ms": round(kms, 5),
+                                "value": round(n ** 3 / (kms * 1e-3) / 1e6, 1), "unit": "Mpoint-updates/s",
+                                "roofline_frac": round(BYTES_PER_UPDATE * n ** 3 / (kms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+    return out
+
+
 def bench_sem_elem(phc, steps=100, steps_p4=20):
     """The reference's CPU-runnable SEM case (50^3 elements of order 2, 101^3
     nodes, c = 1500, f0 = 5 Hz, dt = 1e-3) through the SEM step kernels:
@@ -405,7 +430,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched multi-shot measurement reported beside the headline")
     ap.add_argument("--workload", default="fd", choices=["fd", "batch", "sem", "sem2d", "pml", "semelem", "learned",
-                                                        "learned-step"],
+                                                        "learned-step", "fd-sizes"],
                     help="fd: the 512^3 headline; batch: only the batched multi-shot workload "
                          "(--steps steps, default 800); sem: only the SEM order-4 workload")
     ap.add_argument("--batch-shape", default=None,
@@ -431,6 +456,15 @@ def main():
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
                "data": "synthetic: seeded layered vp, untrained layer", "config": {"workload": r["workload"]},
                "learned_step": r}
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == "fd-sizes":
+        r = bench_fd_sizes(phc)
+        big = r["sizes"][str(FD_SIZES[-1])]
+        out = {"metric": METRIC, "value": big["value"], "unit": big["unit"], "n_gpus": 1, "steps": 100,
+               "warmup": 20, "ms_per_step": big["kernel_ms"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32", "data": "synthetic: seeded 6-layer vp model, Ricker source",
+               "config": {"workload": r["workload"]}, "fd_sizes": r}
         print(json.dumps(out), flush=True)
         return
     if args.workload == "learned":
@@ -588,6 +622,8 @@ def main():
         progress("SEM2D done")
         out["pml"] = bench_pml(phc)
         progress("PML done")
+        out["fd_sizes"] = bench_fd_sizes(phc)
+        progress("FD sizes done")
         out["sem_element_kernels"] = bench_sem_elem(phc)
         progress("SEM element kernels done")
         out["learned_boundary"] = bench_learned(phc)
