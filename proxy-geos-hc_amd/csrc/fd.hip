@@ -259,7 +259,7 @@ struct phc_fd {
     // learned thin boundary (phc_fd_set_learned_boundary)
     int lb_on = 0;
     LbInfer lbi;
-    DevBuf lb_w, lb_corr;
+    DevBuf lb_w, lb_corr, lb_kvcs, lb_kslab;
     // cached graph of the last phc_fd_run
     hipGraphExec_t gexec = nullptr;
     int g_it0 = -1, g_n = -1, g_cur = -1, g_damp = -1, g_nrec = -1;
@@ -497,11 +497,12 @@ static phc_status fd_launch_step(phc_fd *h, FdStepArgs a, bool damp)
 // new level): one marching launch (with the PML slabs when a PML is set);
 // with the learned boundary, its correction pass before and its apply pass
 // after the stencil launch.
-static phc_status fd_launch_full_step(phc_fd *h, const FdStepArgs &a, bool damp, int ca, int cb)
+static phc_status fd_launch_full_step(phc_fd *h, const FdStepArgs &a, bool damp, int ca, int cb,
+                                      const float *kslab_in = nullptr, float *kslab_out = nullptr)
 {
     if (h->lb_on)
         PHC_HIP(lb_enqueue_correct(h->lbi, h->L, h->lev[cb].as<float>(), h->lev[ca].as<float>(),
-                                   h->vp.as<float>(), h->stream));
+                                   h->vp.as<float>(), h->stream, kslab_in, kslab_out));
     PHC_TRY(fd_launch_step(h, a, damp));
     if (h->lb_on) PHC_HIP(lb_enqueue_apply(h->lbi, h->L, h->lev[ca].as<float>(), h->stream));
     return PHC_OK;
@@ -612,8 +613,8 @@ phc_status phc_fd_destroy(phc_fd *h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     fd_drop_graph(h);
     for (DevBuf *b : {&h->lev[0], &h->lev[1], &h->vp, &h->fx, &h->fy, &h->fz, &h->rhs, &h->trace,
-                      &h->rec, &h->recoff, &h->lb_w, &h->lb_corr, &h->pml_ex, &h->pml_ey, &h->pml_ez,
-                      &h->pml_phi})
+                      &h->rec, &h->recoff, &h->lb_w, &h->lb_corr, &h->lb_kvcs, &h->lb_kslab, &h->pml_ex,
+                      &h->pml_ey, &h->pml_ez, &h->pml_phi})
         b->release();
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -732,7 +733,11 @@ static phc_status fd_enqueue_run(phc_fd *h, int it0, int nsteps, int c0, bool da
             a.rec_step = s - 1;
         }
         if (ev) PHC_HIP(hipEventRecord(ev[2 * s], h->stream));
-        PHC_TRY(fd_launch_full_step(h, a, s == 0 ? damp0 : (h->sw > 0), ca, cb));
+        // learned layer: step s's correction pass hands its k-face planes to
+        // step s + 1's through store s % 2 (the first step reads the grid)
+        float *kout = h->lb_on ? h->lbi.kslab[s & 1] : nullptr;
+        const float *kin = h->lb_on && s > 0 ? h->lbi.kslab[(s + 1) & 1] : nullptr;
+        PHC_TRY(fd_launch_full_step(h, a, s == 0 ? damp0 : (h->sw > 0), ca, cb, kin, kout));
         if (ev) PHC_HIP(hipEventRecord(ev[2 * s + 1], h->stream));
         c = ca;
     }
@@ -981,6 +986,8 @@ phc_status phc_fd_set_learned_boundary(phc_fd *h, const phc_lb *lb, float halo, 
         h->lb_on = 0;
         h->lb_w.release();
         h->lb_corr.release();
+        h->lb_kvcs.release();
+        h->lb_kslab.release();
         return PHC_OK;
     }
     FdFieldView fv;
@@ -1008,6 +1015,17 @@ phc_status phc_fd_set_learned_boundary(phc_fd *h, const phc_lb *lb, float halo, 
     for (int l = 0; l <= LB_R; ++l) h->lbi.chat[l] = fv.chat[l];
     for (int d = 0; d < LB_W; ++d)
         h->lbi.ldamp[d] = (float)std::exp((double)layer_damping * (double)(LB_W - d) * (LB_W - d));
+    // the k-faces' compact velocity terms (vp is fixed after create) and the
+    // two stores the fused run's correction passes hand their planes on in
+    const int64_t npk = (int64_t)h->nx * h->ny;
+    PHC_TRY(h->lb_kvcs.alloc(sizeof(float) * (size_t)(2 * 2 * LB_W * npk)));
+    PHC_TRY(h->lb_kslab.alloc(sizeof(float) * (size_t)(2 * 2 * (LB_W + LB_R) * npk)));
+    h->lbi.npts_k = (int)npk;
+    h->lbi.kslab[0] = h->lb_kslab.as<float>();
+    h->lbi.kslab[1] = h->lbi.kslab[0] + 2 * (LB_W + LB_R) * npk;
+    h->lbi.kvcs = nullptr;
+    PHC_HIP(lb_enqueue_kface_vp(h->lbi, h->L, h->vp.as<float>(), h->lb_kvcs.as<float>(), h->stream));
+    h->lbi.kvcs = h->lb_kvcs.as<float>();
     h->lb_on = 1;
     return PHC_OK;
 }

@@ -65,6 +65,15 @@ struct LbInfer {
     float c1;            // c_1 of the stencil
     float chat[LB_R + 1];  // c_l / c_1
     float ldamp[LB_W];   // new value at layer depth d multiplied by ldamp[d] (per face holding it)
+    // k-faces (f = 4, 5; their rows end at the grid's contiguous axis, so a
+    // row end costs a whole line): vp c1 and sqrt(vp c1) at the layer points,
+    // kvcs[((f - 4) * 2 + which) * LB_W + d][u nv + v], computed once; and two
+    // stores of the LB_W + LB_R in-grid planes of a level, kslab[s][((f - 4)
+    // * (LB_W + LB_R) + q) * npts + u nv + v], which a correction pass can
+    // write (its current level) and the next one read (as its previous).
+    const float *kvcs;
+    float *kslab[2];
+    int npts_k;          // u nv + v range of a k-face (nx ny)
 };
 
 // Bound of the network's output at layer depth d (see above), in float.
@@ -78,8 +87,15 @@ __host__ __device__ inline float lb_ymax(const float chat[LB_R + 1], int d)
 // Launch the correction pass (before the step: reads the current and previous
 // levels) and the apply pass (after the step: adds the corrections to the new
 // level at layer points).
+// With kslab_in the k-faces' previous-level planes come from that store
+// (written by the previous pass) instead of pnm1; with kslab_out the pass
+// writes its current level's k-face planes there.
 hipError_t lb_enqueue_correct(const LbInfer &lb, const FdLayout &L, const float *pn, const float *pnm1,
-                              const float *vp, hipStream_t stream);
+                              const float *vp, hipStream_t stream, const float *kslab_in = nullptr,
+                              float *kslab_out = nullptr);
+// vp c1 and sqrt(vp c1) at the k-faces' layer points into lb.kvcs.
+hipError_t lb_enqueue_kface_vp(const LbInfer &lb, const FdLayout &L, const float *vp, float *kvcs,
+                               hipStream_t stream);
 hipError_t lb_enqueue_apply(const LbInfer &lb, const FdLayout &L, float *pnew, hipStream_t stream);
 // Size of the compact correction store for a grid, filling corr_off.
 int64_t lb_corr_layout(const FdLayout &L, int64_t *corr_off);

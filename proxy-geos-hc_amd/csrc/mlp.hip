@@ -663,6 +663,10 @@ struct LbCorrectTileArgs {
     int blk0[LB_FACES + 1];  // first block of each face
     float h, c1;
     float ymax[LB_W];        // bound of y at each depth (lb.hpp)
+    const float *kvcs;       // k-faces: vp c1 and sqrt(vp c1), compact (or null: from vp)
+    const float *kslab_in;   // k-faces: previous level's planes, compact (or null: from pnm1)
+    float *kslab_out;        // k-faces: the current level's planes written here (or null)
+    int npts_k;
 };
 
 __global__ void __launch_bounds__(256) lb_correct_tile_kernel(LbCorrectTileArgs A)
@@ -691,17 +695,38 @@ __global__ void __launch_bounds__(256) lb_correct_tile_kernel(LbCorrectTileArgs 
     c[va] = flat - c[ua] * g.nv;
     const int64_t o0 = fd_off(A.L, c[0], c[1], c[2]);
     const int64_t step = a == 0 ? A.L.pitch_j : (a == 1 ? A.L.pitch_k : 1);
-    // every load is issued before the first is waited for
-    float r0[LB_NIG], r1[LB_NIG], rv[LB_W];
+    // every load is issued before the first is waited for.  On the k-faces
+    // (a row end costs a whole line) the previous level and the velocity
+    // terms come from compact stores when given, and the current level's
+    // planes are written to one for the next pass.
+    const bool kf = a == 2;
+    const int64_t kb = (int64_t)(f - 4) * LB_NIG * A.npts_k + flat;  // k-face store element (q = 0)
+    const float *kin = kf ? A.kslab_in : nullptr;
+    float *kout = kf ? A.kslab_out : nullptr;
+    const bool kvc = kf && A.kvcs;
+    float r0[LB_NIG], r1[LB_NIG], rv[LB_W], rs[LB_W];
 #pragma unroll
     for (int q = 0; q < LB_NIG; ++q) {
         r0[q] = A.pn[o0 + q * step];
-        r1[q] = A.pnm1[o0 + q * step];
+        r1[q] = kin ? kin[kb + (int64_t)q * A.npts_k] : A.pnm1[o0 + q * step];
     }
+    if (kvc) {
+        const int64_t vb = (int64_t)(f - 4) * 2 * LB_W * A.npts_k + flat;
 #pragma unroll
-    for (int dd = 0; dd < LB_W; ++dd) rv[dd] = A.vp[o0 + (g.side ? LB_NIG - 1 - dd : dd) * step];
+        for (int dd = 0; dd < LB_W; ++dd) {
+            rv[dd] = A.kvcs[vb + (int64_t)dd * A.npts_k];
+            rs[dd] = A.kvcs[vb + (int64_t)(LB_W + dd) * A.npts_k];
+        }
+    } else {
+#pragma unroll
+        for (int dd = 0; dd < LB_W; ++dd) rv[dd] = A.vp[o0 + (g.side ? LB_NIG - 1 - dd : dd) * step];
+    }
     LbWaveWeights W;  // its loads in flight with the staging's
     lb_load_wave_weights(w, W);
+    if (kout && flat0 + (int)threadIdx.x < npts) {
+#pragma unroll
+        for (int q = 0; q < LB_NIG; ++q) kout[kb + (int64_t)q * A.npts_k] = r0[q];
+    }
 #pragma unroll
     for (int q = 0; q < LB_NIG; ++q) {
         const int P = g.side ? LB_NPL - 1 - q : LB_R + q;
@@ -715,9 +740,9 @@ __global__ void __launch_bounds__(256) lb_correct_tile_kernel(LbCorrectTileArgs 
     }
 #pragma unroll
     for (int dd = 0; dd < LB_W; ++dd) {
-        const float vc = rv[dd] * A.c1;
+        const float vc = kvc ? rv[dd] : rv[dd] * A.c1;
         Svc[dd][tu][tv] = vc;
-        Svs[dd][tu][tv] = sqrtf(vc);
+        Svs[dd][tu][tv] = kvc ? rs[dd] : sqrtf(vc);
     }
     if (threadIdx.x < LB_W * LB_H) b1s[threadIdx.x] = bf2f(b1raw);
     __syncthreads();
@@ -819,8 +844,33 @@ __global__ void __launch_bounds__(256) lb_apply_kernel(LbApplyArgs A)
     A.p[o] = mul_nc(add_nc(A.p[o], add), damp);
 }
 
+// vp c1 and sqrt(vp c1) at the k-faces' layer points (the same operations as
+// the correction pass's staging): kvcs[((f - 4) * 2 + which) * LB_W + d][i ny + j].
+__global__ void lb_kface_vp_kernel(const float *vp, FdLayout L, float c1, float *kvcs)
+{
+    const int64_t npts = (int64_t)L.nx * L.ny;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * LB_W * npts) return;
+    const int fk = (int)(t / (LB_W * npts)), d = (int)((t / npts) % LB_W);
+    const int64_t flat = t % npts;
+    const int i = (int)(flat / L.ny), j = (int)(flat % L.ny);
+    const int k = fk ? L.nz - 1 - d : d;
+    const float vc = vp[fd_off(L, i, j, k)] * c1;
+    kvcs[((int64_t)fk * 2 * LB_W + d) * npts + flat] = vc;
+    kvcs[((int64_t)fk * 2 * LB_W + LB_W + d) * npts + flat] = sqrtf(vc);
+}
+
+hipError_t lb_enqueue_kface_vp(const LbInfer &lb, const FdLayout &L, const float *vp, float *kvcs,
+                               hipStream_t stream)
+{
+    const int64_t n = 2 * LB_W * (int64_t)L.nx * L.ny;
+    hipLaunchKernelGGL(lb_kface_vp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, vp, L, lb.c1,
+                       kvcs);
+    return hipGetLastError();
+}
+
 hipError_t lb_enqueue_correct(const LbInfer &lb, const FdLayout &L, const float *pn, const float *pnm1,
-                              const float *vp, hipStream_t stream)
+                              const float *vp, hipStream_t stream, const float *kslab_in, float *kslab_out)
 {
     // grids under the learned layer have at least 4 LB_W points per side
     // (phc_fd_set_learned_boundary), so every tile's box of normal planes
@@ -835,6 +885,10 @@ hipError_t lb_enqueue_correct(const LbInfer &lb, const FdLayout &L, const float 
     T.h = lb.halo;
     T.c1 = lb.c1;
     T.blk0[0] = 0;
+    T.kvcs = lb.kvcs;
+    T.kslab_in = kslab_in;
+    T.kslab_out = kslab_out;
+    T.npts_k = L.nx * L.ny;
     for (int d = 0; d < LB_W; ++d) T.ymax[d] = lb_ymax(lb.chat, d);
     for (int f = 0; f <= LB_FACES; ++f) T.off[f] = lb.corr_off[f];
     const int nn[3] = {L.nx, L.ny, L.nz};
