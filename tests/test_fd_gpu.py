@@ -2,8 +2,8 @@
 
 Reference-rounding mode (ARITH_REF) must be bit-exact with the oracle, which
 itself reproduces the reference's published prints.  FP32 mode is held to a
-relative L2 of 1e-5 after the stated number of steps (the north-star
-tolerance is 1e-4)."""
+relative L2 of 1e-5 after the stated number of steps (the floating-point
+parity bar is 1e-4)."""
 import json
 import os
 

@@ -3,8 +3,8 @@ against the CPU oracle (reference-semantics FP32 element loop and FP64).
 
 The GPU assembles in a different order than the reference's element loop
 (FP64 setup, sum-factorised or separable operator), so FP32 results agree to
-rounding, not bit for bit.  Tolerances are stated per test; the north-star
-tolerance is 1e-4 relative L2 against the reference."""
+rounding, not bit for bit.  Tolerances are stated per test; the
+floating-point parity bar is 1e-4 relative L2 against the reference."""
 import ctypes
 
 import numpy as np
