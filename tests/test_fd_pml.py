@@ -105,6 +105,32 @@ def test_gpu_pml_bit_exact(gpu, oracle, shape, ndamp, src):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("chunk,waves", [(-1, 0), (-3, 0), (-4, 0), (8, 0), (-2, 8), (6, 16)])
+def test_gpu_pml_bit_exact_schedules(gpu, oracle, chunk, waves):
+    # the PML march under the per-XCD schedule (chunk < 0) and under short
+    # plane chunks, with block shapes of 8 and 16 waves, on a grid whose tile
+    # columns lie inside the inner box, straddle a slab edge, or sit in a
+    # slab: bit-identical to the oracle's computeOneStepPML
+    print("\n[pml] schedule chunk %d waves %d" % (chunk, waves), flush=True)
+    s, P = pair(gpu, oracle, (48, 40, 140), xs=10, ys=20, zs=70, ndamp=6, nsamples=60)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF, chunk_i=chunk, block_waves=waves)
+    assert sol.set_pml(ndamp=6, f0=s.f0) == 6
+    for it in range(10):
+        sol.step(it)
+    tr, _ = P.run(10)
+    assert np.array_equal(sol.current(), P.current())
+    rec = np.array([[s.xs, s.ys, s.zs], [1, 1, 1], [47, 39, 139], [24, 20, 135]], np.int32)
+    sol.set_receivers(rec)
+    src_t, rv = sol.run(10, 45)
+    otr, orv = P.run(45, receivers=rec)
+    # the pulse has reached the i-slab's points (stored planes 4..9: i = 0..5)
+    assert np.abs(otr).max() > 0 and np.abs(P.current()[4:10] - 1e-6).max() > 1.0
+    assert np.array_equal(src_t, otr) and np.array_equal(rv, orv)
+    assert np.array_equal(sol.current(), P.current())
+    assert np.array_equal(sol.previous(), P.previous())
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("L,index64,ly,lz", [(1, False, None, None), (2, False, None, None),
                                              (3, False, None, None), (5, False, None, None),
                                              (6, False, None, None), (4, True, None, None),
