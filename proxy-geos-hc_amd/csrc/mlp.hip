@@ -836,9 +836,10 @@ __global__ void __launch_bounds__(256) lb_apply_kernel(LbApplyArgs A)
         const int nv = (a == 2) ? ny : nz;
         const int u = (a == 0) ? j : i;
         const int v = (a == 2) ? j : k;
-        const float cv = A.corr[A.off[2 * a + side] + ((int64_t)d * nu + u) * nv + v];
-        add += (lo || hi) ? cv : 0.f;
-        damp *= (lo || hi) ? A.ldamp[d] : 1.f;
+        if (lo || hi) {  // only the faces whose layer holds the point are read
+            add += A.corr[A.off[2 * a + side] + ((int64_t)d * nu + u) * nv + v];
+            damp *= A.ldamp[d];
+        }
     }
     const int64_t o = fd_off(A.L, i, j, k);
     A.p[o] = mul_nc(add_nc(A.p[o], add), damp);
