@@ -161,6 +161,30 @@ def test_j_remainder_slab_bit_exact(gpu, oracle, ny, nz, mode, tile_j):
     assert np.array_equal(src2, tr2) and np.array_equal(p2.current(), P2.current())
 
 
+@pytest.mark.parametrize("shape", [(1, 9, 70), (5, 1, 130), (3, 4, 1), (2, 2, 2), (1, 1, 1), (9, 1, 1)])
+def test_degenerate_grids_bit_exact(gpu, oracle, shape):
+    """Grids one or two points thick along some axes (the stencil then reads
+    mostly halo): one partial tile, a march of one plane, a single row or a
+    single lane; the fused run and the per-step surface stay bit-exact with
+    the oracle."""
+    nx, ny, nz = shape
+    banner("degenerate grid %s" % (shape,))
+    s, P = make_pair(gpu, oracle, nx, ny, nz, sponge=0, nsamples=40)
+    sol = gpu.FdSolver(s, arith=gpu.ARITH_REF)
+    rec = np.array([[s.xs, s.ys, s.zs], [0, 0, 0], [nx - 1, ny - 1, nz - 1]], np.int32)
+    sol.set_receivers(rec)
+    src, rv = sol.run(0, 20)
+    tr, orv = P.run(20, receivers=rec)
+    assert np.abs(tr).max() > 0
+    assert np.array_equal(src, tr) and np.array_equal(rv, orv)
+    assert np.array_equal(sol.current(), P.current())
+    for it in range(20, 25):
+        sol.step(it)
+    P.run(5)
+    assert np.array_equal(sol.current(), P.current())
+    assert np.array_equal(sol.previous(), P.previous())
+
+
 def test_add_rhs_value_equals_add_rhs(gpu):
     """addRHS with the caller's sample (phc_fd_add_rhs_value) gives the bits of
     addRHS from the handle's samples."""
