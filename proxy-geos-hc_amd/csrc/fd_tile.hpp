@@ -203,11 +203,13 @@ __device__ __forceinline__ void fd_tile(const FdStepArgs &a, const G &g, int kt,
         }
         if constexpr (PML) {
             // phi only where plane p's point is in a slab; other points read
-            // element 0 (one cached line), so the load needs no branch
+            // element 0, so the load needs no branch.  A cached load: as a
+            // non-temporal one, every wavefront's read of that one line went
+            // to L2 (2% of the PML step at 512^3)
             const bool sl = p < a.px3 || p >= a.px4;
 #pragma unroll
             for (int jj = 0; jj < NJ; ++jj)
-                ph[set][jj] = ldo_nt(a.pml_phi, (sl || pslab_jk[jj]) ? colq[jj] + po : (IT)0);
+                ph[set][jj] = ldo(a.pml_phi, (sl || pslab_jk[jj]) ? colq[jj] + po : (IT)0);
             ex[set][0] = a.pml_ex[p];
             ex[set][1] = a.pml_ex[p + 1];
             ex[set][2] = a.pml_ex[p + 2];
